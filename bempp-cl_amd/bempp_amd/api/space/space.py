@@ -1,0 +1,436 @@
+"""Function spaces on a grid: dof maps, multipliers, support and localised spaces.
+
+Counterpart of bempp_cl/api/space/space.py:7-130 (`function_space`), 283-405
+(`FunctionSpace`), 866-894 (`_process_segments`), 915-952 (`make_localised_space`),
+scalar_spaces.py:11-59, 117-216, 339-425 (DP0, DP1, P1 and the P1 dof map) and
+maxwell_spaces.py:15-61, 155-202, 545-620 (RWG, SNC and their dof map).
+
+A space is described by
+* local2global (F, nshape) uint32: global dof of each local shape function,
+* local_multipliers (F, nshape) float64: sign/weight of each local shape function
+  (0 for shape functions that carry no dof),
+* normal_multipliers (F,) int32: +1, or -1 on segments with swapped normals,
+* support (F,) bool and the shapeset identifier.
+Spaces requiring dof transformations (barycentric, dual) are not provided: the dense
+assembler rejects them (core/dense_assembler.py:22-23).
+"""
+
+import itertools as _itertools
+
+import numpy as _np
+
+_SHAPESETS = {
+    "p0_discontinuous": 1,
+    "p1_discontinuous": 3,
+    "rwg0": 3,
+    "snc0": 3,
+}
+
+_SPACE_COUNTER = _itertools.count()
+
+
+class Shapeset(object):
+    """Identifier and size of a reference shapeset (shapesets.py:46-158)."""
+
+    def __init__(self, identifier):
+        if identifier not in _SHAPESETS:
+            raise ValueError(f"Unknown shapeset {identifier}")
+        self.identifier = identifier
+        self.number_of_shape_functions = _SHAPESETS[identifier]
+
+    def evaluate(self, local_coordinates):
+        """Shapeset values (dimension, nshape, npoints) at reference points (2, npoints)."""
+        x = _np.asarray(local_coordinates)
+        if self.identifier == "p0_discontinuous":
+            return _np.ones((1, 1, x.shape[1]), dtype=x.dtype)
+        if self.identifier == "p1_discontinuous":
+            return _np.vstack([1 - x[0] - x[1], x[0], x[1]])[None, :, :]
+        vals = _np.zeros((2, 3, x.shape[1]), dtype=x.dtype)
+        vals[0] = _np.vstack([x[0], x[0] - 1, x[0]])
+        vals[1] = _np.vstack([x[1] - 1, x[1], x[1]])
+        return vals
+
+
+class FunctionSpace(object):
+    """A function space over a grid (space.py:283-405)."""
+
+    def __init__(
+        self,
+        grid,
+        identifier,
+        shapeset,
+        codomain_dimension,
+        order,
+        local2global,
+        local_multipliers,
+        support,
+        normal_multipliers,
+        is_localised,
+    ):
+        from scipy.sparse import coo_matrix, identity
+
+        self._grid = grid
+        self._identifier = identifier
+        self._shapeset = Shapeset(shapeset)
+        self._codomain_dimension = codomain_dimension
+        self._order = order
+        self._local2global = _np.ascontiguousarray(local2global, dtype=_np.uint32)
+        self._local_multipliers = _np.ascontiguousarray(local_multipliers, dtype=_np.float64)
+        self._support = _np.asarray(support, dtype=bool)
+        self._normal_multipliers = _np.asarray(normal_multipliers, dtype=_np.int32)
+        self._support_elements = _np.flatnonzero(self._support).astype(_np.uint32)
+        self._number_of_support_elements = len(self._support_elements)
+        self._id = f"space{next(_SPACE_COUNTER)}"
+        self._grid_dof_count = int(1 + _np.max(self._local2global)) if self._local2global.size else 0
+        nshape = self.number_of_shape_functions
+        nsup = self._number_of_support_elements
+
+        self._map_to_localised_space = coo_matrix(
+            (
+                self._local_multipliers[self._support].ravel(),
+                (_np.arange(nshape * nsup), self._local2global[self._support].ravel()),
+            ),
+            shape=(nshape * nsup, self._grid_dof_count),
+            dtype=_np.float64,
+        ).tocsr()
+        self._map_to_full_grid = coo_matrix(
+            (
+                self._local_multipliers[self._support].ravel(),
+                (
+                    nshape * _np.repeat(self._support_elements.astype(_np.int64), nshape)
+                    + _np.tile(_np.arange(nshape), nsup),
+                    self._local2global[self._support].ravel(),
+                ),
+            ),
+            shape=(nshape * grid.number_of_elements, self._grid_dof_count),
+            dtype=_np.float64,
+        ).tocsr()
+        self._dof_transformation = identity(self._grid_dof_count, dtype=_np.float64, format="csr")
+        self._localised_space = self if is_localised else _make_localised_space(self)
+
+    # -------------------------------------------------------------- properties
+    @property
+    def grid(self):
+        return self._grid
+
+    @property
+    def identifier(self):
+        return self._identifier
+
+    @property
+    def shapeset(self):
+        return self._shapeset
+
+    @property
+    def codomain_dimension(self):
+        return self._codomain_dimension
+
+    @property
+    def order(self):
+        return self._order
+
+    @property
+    def local2global(self):
+        return self._local2global
+
+    @property
+    def local_multipliers(self):
+        return self._local_multipliers
+
+    @property
+    def normal_multipliers(self):
+        return self._normal_multipliers
+
+    @property
+    def support(self):
+        return self._support
+
+    @property
+    def support_elements(self):
+        return self._support_elements
+
+    @property
+    def number_of_support_elements(self):
+        return self._number_of_support_elements
+
+    @property
+    def number_of_shape_functions(self):
+        return self._shapeset.number_of_shape_functions
+
+    @property
+    def global_dof_count(self):
+        return self._dof_transformation.shape[1]
+
+    @property
+    def grid_dof_count(self):
+        return self._grid_dof_count
+
+    @property
+    def localised_space(self):
+        return self._localised_space
+
+    @property
+    def is_localised(self):
+        return self._localised_space is self
+
+    @property
+    def map_to_localised_space(self):
+        return self._map_to_localised_space
+
+    @property
+    def map_to_full_grid(self):
+        return self._map_to_full_grid
+
+    @property
+    def dof_transformation(self):
+        return self._dof_transformation
+
+    @property
+    def requires_dof_transformation(self):
+        return False
+
+    @property
+    def id(self):
+        return self._id
+
+    def cell_dofs(self, cell_index):
+        """Global dofs of a cell, None where the multiplier is zero."""
+        return [
+            None if _np.isclose(m, 0) else int(d)
+            for d, m in zip(self._local2global[cell_index], self._local_multipliers[cell_index])
+        ]
+
+    def is_compatible(self, other):
+        return self == other
+
+    def __eq__(self, other):
+        if self is other:
+            return True
+        if not isinstance(other, FunctionSpace):
+            return False
+        return (
+            self._grid == other._grid
+            and self._identifier == other._identifier
+            and _np.array_equal(self._local2global, other._local2global)
+            and _np.array_equal(self._local_multipliers, other._local_multipliers)
+            and _np.array_equal(self._normal_multipliers, other._normal_multipliers)
+            and _np.array_equal(self._support, other._support)
+        )
+
+    def __hash__(self):
+        return hash(self._id)
+
+
+def _make_localised_space(space):
+    """Element-wise space with one dof per (support element, shape function) (space.py:915-952)."""
+    nel = space.grid.number_of_elements
+    nshape = space.number_of_shape_functions
+    nsup = space.number_of_support_elements
+    l2g = _np.zeros((nel, nshape), dtype=_np.uint32)
+    l2g[space.support] = _np.arange(nshape * nsup, dtype=_np.uint32).reshape(nsup, nshape)
+    mult = _np.zeros((nel, nshape), dtype=_np.float64)
+    mult[space.support] = 1
+    return FunctionSpace(
+        space.grid,
+        space.identifier + "_localised",
+        space.shapeset.identifier,
+        space.codomain_dimension,
+        space.order,
+        l2g,
+        mult,
+        space.support,
+        space.normal_multipliers,
+        True,
+    )
+
+
+def _process_segments(grid, support_elements, segments, swapped_normals):
+    """Support mask and normal multipliers (space.py:866-894)."""
+    if support_elements is not None and segments is not None:
+        raise ValueError("Only one of 'support_elements' and 'segments' must be nonzero.")
+    nel = grid.number_of_elements
+    swapped = set() if swapped_normals is None else set(swapped_normals)
+    normal_multipliers = _np.where(_np.isin(grid.domain_indices, list(swapped)), -1, 1).astype(_np.int32)
+    if support_elements is not None:
+        support = _np.zeros(nel, dtype=bool)
+        support[_np.asarray(support_elements, dtype=_np.int64)] = True
+    elif segments is not None:
+        support = _np.isin(grid.domain_indices, list(segments))
+    else:
+        support = _np.ones(nel, dtype=bool)
+    return support, normal_multipliers
+
+
+def p0_discontinuous_function_space(grid, support_elements=None, segments=None, swapped_normals=None, **kwargs):
+    """Piecewise constants (scalar_spaces.py:11-59)."""
+    support, normal_multipliers = _process_segments(grid, support_elements, segments, swapped_normals)
+    nsup = int(support.sum())
+    l2g = _np.zeros((grid.number_of_elements, 1), dtype=_np.uint32)
+    l2g[support, 0] = _np.arange(nsup, dtype=_np.uint32)
+    mult = _np.zeros((grid.number_of_elements, 1))
+    mult[support] = 1
+    return FunctionSpace(grid, "p0_discontinuous", "p0_discontinuous", 1, 0, l2g, mult, support, normal_multipliers, True)
+
+
+def p1_discontinuous_function_space(grid, support_elements=None, segments=None, swapped_normals=None, **kwargs):
+    """Discontinuous piecewise linears (scalar_spaces.py:117-166)."""
+    support, normal_multipliers = _process_segments(grid, support_elements, segments, swapped_normals)
+    nsup = int(support.sum())
+    l2g = _np.zeros((grid.number_of_elements, 3), dtype=_np.uint32)
+    l2g[support] = _np.arange(3 * nsup, dtype=_np.uint32).reshape(nsup, 3)
+    mult = _np.zeros((grid.number_of_elements, 3))
+    mult[support] = 1
+    return FunctionSpace(grid, "p1_discontinuous", "p1_discontinuous", 1, 1, l2g, mult, support, normal_multipliers, True)
+
+
+def _p1_dof_map(grid, support, include_boundary_dofs, truncate_at_segment_edge):
+    """Continuous P1 dof map (scalar_spaces.py:339-425)."""
+    nel = grid.number_of_elements
+    nv = grid.number_of_vertices
+    el = grid.elements.astype(_np.int64)
+    non_support_count = _np.bincount(el[:, ~support].ravel(), minlength=nv)
+    vert_local = -_np.ones((nel, 3), dtype=_np.int64)
+    vertex_is_dof = _np.zeros(nv, dtype=bool)
+    sup = _np.flatnonzero(support)
+    verts = el[:, sup].T  # (nsup, 3)
+    interior = (non_support_count[verts] == 0) & ~grid.vertex_on_boundary[verts]
+    take = _np.ones_like(interior) if include_boundary_dofs else interior
+    vert_local[sup] = _np.where(take, verts, -1)
+    vertex_is_dof[verts[take]] = True
+    extended = set()
+    if include_boundary_dofs and not truncate_at_segment_edge:
+        idx, ptr = grid.vertex_neighbors
+        for v in _np.unique(verts[non_support_count[verts] > 0]):
+            for en in idx[ptr[v] : ptr[v + 1]]:
+                if not support[en]:
+                    extended.add(int(en))
+                    li = int(_np.flatnonzero(el[:, en] == v)[0])
+                    vert_local[en, li] = v
+    dofs = -_np.ones(nv, dtype=_np.int64)
+    used = _np.flatnonzero(vertex_is_dof)
+    dofs[used] = _np.arange(len(used))
+    elems = _np.concatenate([sup, _np.array(sorted(extended), dtype=_np.int64)])
+    l2g = _np.zeros((nel, 3), dtype=_np.int64)
+    mult = _np.zeros((nel, 3))
+    support_final = _np.zeros(nel, dtype=bool)
+    vl = vert_local[elems]
+    has = vl >= 0
+    mapped = _np.where(has, dofs[_np.maximum(vl, 0)], 0)
+    support_final[elems] = has.any(axis=1)
+    maxdof = mapped.max(axis=1)
+    l2g[elems] = _np.where(has, mapped, maxdof[:, None])
+    mult[elems] = has.astype(_np.float64)
+    l2g[~support_final] = 0
+    mult[~support_final] = 0
+    return l2g.astype(_np.uint32), mult, support_final
+
+
+def p1_continuous_function_space(
+    grid,
+    support_elements=None,
+    segments=None,
+    swapped_normals=None,
+    include_boundary_dofs=False,
+    truncate_at_segment_edge=True,
+    **kwargs,
+):
+    """Continuous piecewise linears (scalar_spaces.py:169-216)."""
+    support, normal_multipliers = _process_segments(grid, support_elements, segments, swapped_normals)
+    l2g, mult, support = _p1_dof_map(grid, support, include_boundary_dofs, truncate_at_segment_edge)
+    return FunctionSpace(grid, "p1_continuous", "p1_discontinuous", 1, 1, l2g, mult, support, normal_multipliers, False)
+
+
+def _rwg0_dof_map(grid, support, include_boundary_dofs, truncate_at_segment_edge):
+    """RWG/SNC dof map: one dof per edge with two supported neighbours (maxwell_spaces.py:545-620)."""
+    support = support.copy()
+    nel = grid.number_of_elements
+    ned = grid.number_of_edges
+    ee = grid.element_edges.astype(_np.int64)  # (3, F)
+    nb_idx, nb_ptr = grid.edge_neighbors_flat
+    orig = _np.flatnonzero(support)
+    if not truncate_at_segment_edge and include_boundary_dofs:
+        # elements next to supported boundary edges join the support
+        count0 = _np.bincount(ee[:, orig].ravel(), minlength=ned)
+        for e in _np.flatnonzero(count0 == 1):
+            for cell in nb_idx[nb_ptr[e] : nb_ptr[e + 1]]:
+                support[cell] = True
+    count = _np.bincount(ee[:, support].ravel(), minlength=ned)
+    valid = (count == 2) | ((count == 1) & bool(include_boundary_dofs))
+    # dofs are numbered by first occurrence over the original support elements
+    seq = ee[:, orig].T.ravel()
+    edge_dofs = -_np.ones(ned, dtype=_np.int64)
+    cand = seq[valid[seq]]
+    if len(cand):
+        uniq, first = _np.unique(cand, return_index=True)
+        order = _np.argsort(first, kind="stable")
+        edge_dofs[uniq[order]] = _np.arange(len(uniq))
+    has_dof = valid[ee[:, orig]].any(axis=0)
+    support[orig[~has_dof]] = False
+    sup = _np.flatnonzero(support)
+    # smallest supported neighbour per edge
+    min_nb = _np.full(ned, _np.iinfo(_np.int64).max, dtype=_np.int64)
+    for l in range(3):
+        _np.minimum.at(min_nb, ee[l, sup], sup)
+    count = _np.bincount(ee[:, sup].ravel(), minlength=ned)
+    l2g = _np.zeros((nel, 3), dtype=_np.int64)
+    mult = _np.zeros((nel, 3))
+    edges_s = ee[:, sup].T  # (nsup, 3)
+    d = edge_dofs[edges_s]
+    isdof = d >= 0
+    sign = _np.where(count[edges_s] == 1, 1.0, _np.where(sup[:, None] == min_nb[edges_s], 1.0, -1.0))
+    m = _np.where(isdof, sign, 0.0)
+    first_nz = _np.argmax(m != 0, axis=1)
+    fill = d[_np.arange(len(sup)), first_nz]
+    l2g[sup] = _np.where(m != 0, d, fill[:, None])
+    mult[sup] = m
+    return l2g.astype(_np.uint32), mult, support
+
+
+def rwg0_function_space(
+    grid,
+    support_elements=None,
+    segments=None,
+    swapped_normals=None,
+    include_boundary_dofs=False,
+    truncate_at_segment_edge=True,
+    **kwargs,
+):
+    """Lowest-order Raviart-Thomas / RWG space (maxwell_spaces.py:15-61)."""
+    support, normal_multipliers = _process_segments(grid, support_elements, segments, swapped_normals)
+    l2g, mult, support = _rwg0_dof_map(grid, support, include_boundary_dofs, truncate_at_segment_edge)
+    return FunctionSpace(grid, "rwg0", "rwg0", 3, 0, l2g, mult, support, normal_multipliers, False)
+
+
+def snc0_function_space(
+    grid,
+    support_elements=None,
+    segments=None,
+    swapped_normals=None,
+    include_boundary_dofs=False,
+    truncate_at_segment_edge=True,
+    **kwargs,
+):
+    """Scaled Nedelec (n x RWG) space (maxwell_spaces.py:155-202)."""
+    support, normal_multipliers = _process_segments(grid, support_elements, segments, swapped_normals)
+    l2g, mult, support = _rwg0_dof_map(grid, support, include_boundary_dofs, truncate_at_segment_edge)
+    return FunctionSpace(grid, "snc0", "snc0", 3, 0, l2g, mult, support, normal_multipliers, False)
+
+
+def function_space(grid, kind, degree, **kwargs):
+    """Create a function space of the given kind ("DP", "P", "RWG"/"RT", "SNC"/"NC") and degree (space.py:7-130)."""
+    if "support_elements" in kwargs and "segments" in kwargs:
+        raise ValueError("Only one of 'support_elements' and 'segments' must be nonzero.")
+    builders = {
+        ("DP", 0): p0_discontinuous_function_space,
+        ("DP", 1): p1_discontinuous_function_space,
+        ("P", 1): p1_continuous_function_space,
+        ("RWG", 0): rwg0_function_space,
+        ("RT", 0): rwg0_function_space,
+        ("SNC", 0): snc0_function_space,
+        ("NC", 0): snc0_function_space,
+    }
+    kwargs.pop("scatter", None)
+    key = (kind, int(degree))
+    if key not in builders:
+        raise ValueError(f"Requested space not implemented: {kind} {degree}")
+    return builders[key](grid, **kwargs)

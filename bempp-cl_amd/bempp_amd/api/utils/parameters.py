@@ -1,0 +1,43 @@
+"""Parameter object (bempp_cl/api/utils/parameters.py:22-71), restricted to dense assembly."""
+
+
+class _Quadrature(object):
+    def __init__(self):
+        self.regular = 4
+        self.singular = 4
+
+
+class _DenseAssembly(object):
+    def __init__(self):
+        self.workgroup_size_multiple = 2
+
+
+class _Assembly(object):
+    def __init__(self):
+        self.dense = _DenseAssembly()
+        self.always_promote_to_double = False
+        self.discretization_type = "galerkin"
+
+
+class _Hip(object):
+    """Options of the HIP device interface.
+
+    mixed_near_field_factor: in single precision, element pairs whose centroid distance is
+    below this factor times the larger element diameter are integrated in FP64; the others
+    in FP32 with a compensated sum.
+    kahan: use compensated (Kahan) accumulation in the FP32 far-field sums.
+    """
+
+    def __init__(self):
+        self.mixed_near_field_factor = 2.0
+        self.kahan = True
+        self.tile_config = None
+
+
+class DefaultParameters(object):
+    """Default parameters: regular and singular quadrature order 4."""
+
+    def __init__(self):
+        self.quadrature = _Quadrature()
+        self.assembly = _Assembly()
+        self.hip = _Hip()

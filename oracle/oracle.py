@@ -1,0 +1,183 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes front end of the FP64 CPU oracle (bem_oracle.c).
+
+Only tests, the smoke check and the CPU-baseline leg of the benchmark use this module;
+the product never imports it. It assembles dense Galerkin matrices and dense potentials
+in FP64 following the reference's Numba algorithm (bempp_cl/core/numba_kernels.py,
+bempp_cl/core/numba_assemblers.py, bempp_cl/core/dense_assembler.py) and offers
+independent closed-mesh dof maps for P1 (vertex = dof, scalar_spaces.py:339-425) and
+RWG/SNC (edge = dof, +1 on the lower-index neighbour, maxwell_spaces.py:545-620).
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libbem_oracle.so")
+_LIB = None
+
+OPS = {
+    "laplace_single_layer": 0,
+    "laplace_double_layer": 1,
+    "laplace_adjoint_double_layer": 2,
+    "helmholtz_single_layer": 3,
+    "helmholtz_double_layer": 4,
+    "helmholtz_adjoint_double_layer": 5,
+    "maxwell_electric_field": 6,
+}
+
+SHAPES = {"p0_discontinuous": 0, "p1_discontinuous": 1, "rwg0": 2, "snc0": 2}
+
+
+def build():
+    """Compile libbem_oracle.so with the Makefile next to this file."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        i = ctypes.c_int
+        d = ctypes.c_double
+        L.bem_oracle_assemble.argtypes = [i, i, P, i, P, i, P, P, P, P, i, P, P, P, P, i, i, d, d, i, i, i, P, i, i, i, i]
+        L.bem_oracle_assemble.restype = i
+        L.bem_oracle_potential.argtypes = [i, i, P, i, P, i, P, P, P, i, P, i, d, d, P, i]
+        L.bem_oracle_potential.restype = i
+        L.bem_oracle_gauss01.argtypes = [i, P, P]
+        L.bem_oracle_duffy.argtypes = [i, i, P, P, P, P, P]
+        L.bem_oracle_duffy.restype = i
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class SpaceArrays(object):
+    """Plain dof-map arrays of one space."""
+
+    def __init__(self, shapeset, local2global, local_multipliers, normal_multipliers, support, global_dof_count):
+        self.shapeset = shapeset
+        self.local2global = np.ascontiguousarray(local2global, dtype=np.int32)
+        self.local_multipliers = np.ascontiguousarray(local_multipliers, dtype=np.float64)
+        self.normal_multipliers = np.ascontiguousarray(normal_multipliers, dtype=np.int32)
+        self.support = np.ascontiguousarray(support, dtype=np.uint8)
+        self.global_dof_count = int(global_dof_count)
+
+    @staticmethod
+    def from_space(space):
+        return SpaceArrays(
+            space.shapeset.identifier,
+            space.local2global,
+            space.local_multipliers,
+            space.normal_multipliers,
+            space.support,
+            space.global_dof_count,
+        )
+
+
+def _mesh(vertices, elements):
+    v = np.ascontiguousarray(np.asarray(vertices, dtype=np.float64).T)
+    e = np.ascontiguousarray(np.asarray(elements, dtype=np.int64).T.astype(np.int32))
+    return v, e
+
+
+def assemble(op, vertices, elements, domain, dual_to_range, wavenumber=0.0, regular_order=4, singular_order=4,
+             parts=3, test_range=None, nthreads=0):
+    """FP64 dense matrix (rows = dual_to_range dofs, cols = domain dofs) of the operator `op`."""
+    v, e = _mesh(vertices, elements)
+    if not isinstance(domain, SpaceArrays):
+        domain = SpaceArrays.from_space(domain)
+    if not isinstance(dual_to_range, SpaceArrays):
+        dual_to_range = SpaceArrays.from_space(dual_to_range)
+    is_complex = op.startswith("helmholtz") or op.startswith("maxwell")
+    rows, cols = dual_to_range.global_dof_count, domain.global_dof_count
+    out = np.zeros((rows, cols, 2) if is_complex else (rows, cols), dtype=np.float64)
+    k = complex(wavenumber)
+    t0, t1 = (0, -1) if test_range is None else test_range
+    rc = lib().bem_oracle_assemble(
+        OPS[op], v.shape[0], _ptr(v), e.shape[0], _ptr(e),
+        SHAPES[dual_to_range.shapeset], _ptr(dual_to_range.local2global), _ptr(dual_to_range.local_multipliers),
+        _ptr(dual_to_range.normal_multipliers), _ptr(dual_to_range.support),
+        SHAPES[domain.shapeset], _ptr(domain.local2global), _ptr(domain.local_multipliers),
+        _ptr(domain.normal_multipliers), _ptr(domain.support),
+        regular_order, singular_order, k.real, k.imag, rows, cols, int(is_complex), _ptr(out), parts, t0, t1, nthreads,
+    )
+    if rc != 0:
+        raise ValueError(f"oracle assembly failed with code {rc}")
+    if is_complex:
+        return out[..., 0] + 1j * out[..., 1]
+    return out
+
+
+def potential(op, vertices, elements, space, coefficients, points, wavenumber=0.0, order=4, nthreads=0):
+    """FP64 potential (kdim, M) of a density given by its space coefficients (dof vector)."""
+    v, e = _mesh(vertices, elements)
+    if not isinstance(space, SpaceArrays):
+        space = SpaceArrays.from_space(space)
+    nf = e.shape[0]
+    nshape = space.local2global.shape[1]
+    coeffs = np.asarray(coefficients, dtype=np.complex128)
+    full = np.zeros((nf, nshape), dtype=np.complex128)
+    sup = space.support.astype(bool)
+    full[sup] = coeffs[space.local2global[sup]] * space.local_multipliers[sup]
+    cbuf = np.ascontiguousarray(np.stack([full.real, full.imag], axis=-1))
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).T)
+    kdim = 3 if op == "maxwell_electric_field" else 1
+    out = np.zeros((kdim, pts.shape[0], 2))
+    k = complex(wavenumber)
+    rc = lib().bem_oracle_potential(
+        OPS[op], v.shape[0], _ptr(v), nf, _ptr(e), SHAPES[space.shapeset], _ptr(space.normal_multipliers),
+        _ptr(space.support), _ptr(cbuf), pts.shape[0], _ptr(pts), order, k.real, k.imag, _ptr(out), nthreads,
+    )
+    if rc != 0:
+        raise ValueError(f"oracle potential failed with code {rc}")
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def gauss01(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().bem_oracle_gauss01(n, _ptr(x), _ptr(w))
+    return x, w
+
+
+def duffy(order, kind):
+    n = 6 * order**4
+    arrs = [np.zeros(n) for _ in range(5)]
+    m = lib().bem_oracle_duffy(order, {"coincident": 0, "edge_adjacent": 1, "vertex_adjacent": 2}[kind], *[_ptr(a) for a in arrs])
+    return [a[:m] for a in arrs]
+
+
+def p1_closed_dofmap(elements, nvertices):
+    """P1 dof map of a closed mesh with full support: dof = vertex index."""
+    el = np.asarray(elements).T.astype(np.int32)
+    return SpaceArrays("p1_discontinuous", el, np.ones(el.shape), np.ones(el.shape[0]), np.ones(el.shape[0]), nvertices)
+
+
+def rwg_closed_dofmap(elements, shapeset="rwg0"):
+    """RWG/SNC dof map of a closed mesh: edges numbered by first occurrence over (element, local edge)
+    with local edges (0,1), (2,0), (1,2); multiplier +1 on the lower-index neighbour, -1 on the other."""
+    el = np.asarray(elements).T.astype(np.int64)
+    nf = el.shape[0]
+    local = [(0, 1), (2, 0), (1, 2)]
+    index = {}
+    l2g = np.zeros((nf, 3), dtype=np.int32)
+    owner = {}
+    mult = np.zeros((nf, 3))
+    for f in range(nf):
+        for li, (a, b) in enumerate(local):
+            key = tuple(sorted((el[f, a], el[f, b])))
+            if key not in index:
+                index[key] = len(index)
+                owner[key] = f
+            l2g[f, li] = index[key]
+            mult[f, li] = 1.0 if owner[key] == f else -1.0
+    return SpaceArrays(shapeset, l2g, mult, np.ones(nf), np.ones(nf), len(index))
