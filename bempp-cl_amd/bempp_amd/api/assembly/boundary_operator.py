@@ -1,0 +1,106 @@
+"""Boundary operators with cached weak forms (bempp_cl/api/assembly/boundary_operator.py:4-131)."""
+
+
+class BoundaryOperator(object):
+    """Base class of boundary operators."""
+
+    def __init__(self, domain, range_, dual_to_range, parameters):
+        self._domain = domain
+        self._range = range_
+        self._dual_to_range = dual_to_range
+        self._parameters = parameters
+        self._cached = None
+
+    @property
+    def domain(self):
+        return self._domain
+
+    @property
+    def range(self):
+        return self._range
+
+    @property
+    def dual_to_range(self):
+        return self._dual_to_range
+
+    @property
+    def parameters(self):
+        return self._parameters
+
+    def weak_form(self):
+        """Return the discrete weak form, assembling it on first use."""
+        if self._cached is None:
+            self._cached = self._assemble()
+        return self._cached
+
+    def __mul__(self, other):
+        import numpy as np
+
+        if np.isscalar(other):
+            return _ScaledBoundaryOperator(self, other)
+        if isinstance(other, BoundaryOperator):
+            raise NotImplementedError("Products of boundary operators are not provided.")
+        from bempp_amd.api.assembly.grid_function import GridFunction
+
+        if isinstance(other, GridFunction):
+            if not self.domain.is_compatible(other.space):
+                raise ValueError("Operator domain space does not match GridFunction space.")
+            return GridFunction(self.range, projections=self.weak_form() @ other.coefficients,
+                                dual_space=self.dual_to_range)
+        return NotImplemented
+
+    def __rmul__(self, other):
+        import numpy as np
+
+        if np.isscalar(other):
+            return _ScaledBoundaryOperator(self, other)
+        return NotImplemented
+
+    def __add__(self, other):
+        return _SumBoundaryOperator(self, other)
+
+    def __neg__(self):
+        return self.__mul__(-1.0)
+
+    def __sub__(self, other):
+        return self.__add__(-other)
+
+
+class BoundaryOperatorWithAssembler(BoundaryOperator):
+    """A boundary operator assembled by an AssemblerInterface (boundary_operator.py:99-131)."""
+
+    def __init__(self, domain, range_, dual_to_range, assembler, operator_descriptor):
+        super().__init__(domain, range_, dual_to_range, assembler.parameters)
+        self._assembler = assembler
+        self._operator_descriptor = operator_descriptor
+
+    @property
+    def assembler(self):
+        return self._assembler
+
+    @property
+    def descriptor(self):
+        return self._operator_descriptor
+
+    def _assemble(self):
+        return self.assembler.assemble(self.descriptor)
+
+
+class _SumBoundaryOperator(BoundaryOperator):
+    def __init__(self, op1, op2):
+        if not (op1.domain.is_compatible(op2.domain) and op1.dual_to_range.is_compatible(op2.dual_to_range)):
+            raise ValueError("Spaces not compatible.")
+        super().__init__(op1.domain, op1.range, op1.dual_to_range, None)
+        self._op1, self._op2 = op1, op2
+
+    def _assemble(self):
+        return self._op1.weak_form() + self._op2.weak_form()
+
+
+class _ScaledBoundaryOperator(BoundaryOperator):
+    def __init__(self, op, alpha):
+        super().__init__(op.domain, op.range, op.dual_to_range, op.parameters)
+        self._op, self._alpha = op, alpha
+
+    def _assemble(self):
+        return self._op.weak_form() * self._alpha

@@ -125,7 +125,7 @@ def export(filename, grid=None):
         f.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n")
         f.write(f"{grid.number_of_vertices}\n")
         for i, v in enumerate(grid.vertices.T):
-            f.write(f"{i + 1} {v[0]!r} {v[1]!r} {v[2]!r}\n")
+            f.write(f"{i + 1} {float(v[0])!r} {float(v[1])!r} {float(v[2])!r}\n")
         f.write("$EndNodes\n$Elements\n")
         f.write(f"{grid.number_of_elements}\n")
         for i, (e, d) in enumerate(zip(grid.elements.T, grid.domain_indices)):

@@ -1,0 +1,3 @@
+"""Boundary operators: Laplace, Helmholtz and Maxwell (electric field)."""
+
+from . import laplace, helmholtz, maxwell  # noqa: F401

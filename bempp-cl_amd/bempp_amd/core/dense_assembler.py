@@ -1,0 +1,138 @@
+"""Dense assembly of boundary operators on the device (bempp_cl/core/dense_assembler.py:7-100).
+
+`DenseAssembler.assemble` allocates the (rows, cols) result in HBM, runs the regular-pair
+kernel over all test x trial support elements and, when the grids coincide, the FP64
+singular kernel over the Sauter-Schwab pairs, which adds its values straight into the
+result with device atomics (the reference's np.add.at step, dense_assembler.py:78-98).
+
+Precision: "double" runs everything in FP64 and returns float64/complex128; "single"
+runs the MIXED mode (FP32 far pairs, FP64 near and singular pairs) and returns
+float32/complex64.
+"""
+
+import numpy as _np
+
+from bempp_amd.api.utils.helpers import get_type
+
+
+class DenseAssembler(object):
+    """Dense assembler of integral operators (dense_assembler.py:7-46)."""
+
+    def __init__(self, domain, dual_to_range, parameters=None):
+        import bempp_amd.api as api
+
+        self._domain = domain
+        self._dual_to_range = dual_to_range
+        self._parameters = api.assign_parameters(parameters)
+
+    @property
+    def domain(self):
+        return self._domain
+
+    @property
+    def dual_to_range(self):
+        return self._dual_to_range
+
+    @property
+    def parameters(self):
+        return self._parameters
+
+    def assemble(self, operator_descriptor, device_interface, precision, *args, **kwargs):
+        from bempp_amd.api.assembly.discrete_boundary_operator import DenseDiscreteBoundaryOperator
+        from bempp_amd.core.dispatcher import dense_assembler_dispatcher
+
+        if self.domain.requires_dof_transformation or self.dual_to_range.requires_dof_transformation:
+            raise ValueError("Spaces that require dof transformations not supported for dense assembly.")
+        mat = dense_assembler_dispatcher(
+            device_interface, self.domain, self.dual_to_range, self.parameters, operator_descriptor
+        )
+        if self.parameters.assembly.always_promote_to_double:
+            import torch
+
+            if mat.dtype == torch.float32:
+                mat = mat.double()
+            elif mat.dtype == torch.complex64:
+                mat = mat.to(torch.complex128)
+        return DenseDiscreteBoundaryOperator(mat)
+
+
+def _kernel_name(descriptor):
+    if descriptor.assembly_type == "maxwell_electric_field":
+        return "maxwell_electric_field"
+    if descriptor.assembly_type != "default_scalar":
+        raise KeyError(f"Unsupported assembly type {descriptor.assembly_type}")
+    return descriptor.kernel_type
+
+
+def _wavenumber(descriptor):
+    opts = list(descriptor.options)
+    if len(opts) == 0:
+        return 0.0
+    return complex(opts[0], opts[1] if len(opts) > 1 else 0.0)
+
+
+def _is_localised_injective(space):
+    """True if every (support element, shape function) owns its own dof (DP0/DP1)."""
+    return space.is_localised
+
+
+def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, out=None, singular=True):
+    """Assemble the dense matrix into a new (or the given) device tensor and return it."""
+    import torch
+    from bempp_amd.api.integration.triangle_gauss import rule as triangle_rule
+    from bempp_amd.core import hip_backend as hb
+    from bempp_amd.core.singular_assembler import singular_rule
+
+    op = _kernel_name(operator_descriptor)
+    if op not in hb.OPERATORS:
+        raise KeyError(f"Unsupported kernel {op}")
+    precision = operator_descriptor.precision
+    types = get_type(precision)
+    dtype = types.complex if operator_descriptor.is_complex else types.real
+    tdtype = getattr(torch, dtype)
+    rows, cols = dual_to_range.global_dof_count, domain.global_dof_count
+    dev = hb.device()
+    grids_identical = domain.grid == dual_to_range.grid
+
+    test = hb.device_space(dual_to_range)
+    trial = hb.device_space(domain)
+    points, weights = triangle_rule(parameters.quadrature.regular)
+    rule = hb.quad_rule(points, weights)
+    mode = hb.PRECISION_MIXED if precision == "single" else hb.PRECISION_FP64
+    direct = _is_localised_injective(domain) and _is_localised_injective(dual_to_range)
+    full_cover = (
+        direct
+        and dual_to_range.number_of_support_elements * dual_to_range.number_of_shape_functions == rows
+        and domain.number_of_support_elements * domain.number_of_shape_functions == cols
+    )
+
+    if out is None:
+        out = torch.empty((rows, cols), dtype=tdtype, device=dev)
+        if not full_cover:
+            hb.fill_zero(out)
+        accumulate = 0
+    else:
+        accumulate = 1
+    hip_opts = getattr(parameters, "hip", None)
+    near = getattr(hip_opts, "mixed_near_field_factor", 2.0) if hip_opts is not None else 2.0
+    kahan = getattr(hip_opts, "kahan", True) if hip_opts is not None else True
+
+    hb.regular_assemble(
+        op, test, trial, grids_identical, rule, _wavenumber(operator_descriptor), mode, out,
+        scatter=0 if direct else 1, accumulate=accumulate if direct else 0, near_factor=near, kahan=kahan,
+    )
+    if grids_identical and singular:
+        srule = singular_rule(dual_to_range.grid, parameters.quadrature.singular, dual_to_range.support, domain.support)
+        hb.singular_assemble(
+            op, test, trial, srule.pairs, srule.test_points, srule.trial_points, srule.weights,
+            _wavenumber(operator_descriptor), out,
+        )
+    return out
+
+
+def assemble_dense(domain, dual_to_range, parameters, operator_descriptor, device_interface):
+    """Assemble and return the dense matrix as a numpy array (dense_assembler.py:49-100)."""
+    from bempp_amd.core.dispatcher import dense_assembler_dispatcher
+
+    mat = dense_assembler_dispatcher(device_interface, domain, dual_to_range, parameters, operator_descriptor)
+    return mat.cpu().numpy()

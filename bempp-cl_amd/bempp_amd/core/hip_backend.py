@@ -1,0 +1,259 @@
+"""ctypes binding of libbempp_hip.so and device-resident copies of grids and spaces.
+
+This is the "hip" device interface: the counterpart of the reference's backend modules
+bempp_cl/core/numba_assemblers.py and bempp_cl/core/opencl_assemblers.py. Device memory
+and streams come from PyTorch-ROCm; all compute happens in the HIP library. There is no
+CPU fallback: if the library is missing or no GPU is visible, the calls raise.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lib", "libbempp_hip.so")
+_LIB = None
+
+OPERATORS = {
+    "laplace_single_layer": 0,
+    "laplace_double_layer": 1,
+    "laplace_adjoint_double_layer": 2,
+    "helmholtz_single_layer": 3,
+    "helmholtz_double_layer": 4,
+    "helmholtz_adjoint_double_layer": 5,
+    "maxwell_electric_field": 6,
+}
+SHAPESETS = {"p0_discontinuous": 0, "p1_discontinuous": 1, "rwg0": 2, "snc0": 2}
+DTYPES = {"float32": 0, "float64": 1, "complex64": 2, "complex128": 3}
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+GEOM_STRIDE = 16
+
+_ERRORS = {1: ValueError, 2: KeyError, 3: RuntimeError, 4: ValueError}
+
+
+class BemSpace(ctypes.Structure):
+    _fields_ = [
+        ("n_elements", ctypes.c_int32),
+        ("shapeset", ctypes.c_int32),
+        ("elements", ctypes.c_void_p),
+        ("geometry", ctypes.c_void_p),
+        ("vertex_ids", ctypes.c_void_p),
+        ("local2global", ctypes.c_void_p),
+        ("multipliers", ctypes.c_void_p),
+        ("normal_multipliers", ctypes.c_void_p),
+    ]
+
+
+class BemQuadRule(ctypes.Structure):
+    _fields_ = [
+        ("n_points", ctypes.c_int32),
+        ("x", ctypes.c_double * 16),
+        ("y", ctypes.c_double * 16),
+        ("w", ctypes.c_double * 16),
+    ]
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def lib():
+    """Load libbempp_hip.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(
+                f"libbempp_hip.so not found at {_LIB_PATH}; build it with `python bempp-cl_amd/build_hip.py`"
+            )
+        L = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        i = ctypes.c_int
+        d = ctypes.c_double
+        i64 = ctypes.c_int64
+        sz = ctypes.c_size_t
+        SP = ctypes.POINTER(BemSpace)
+        QP = ctypes.POINTER(BemQuadRule)
+        L.bem_regular_assemble.argtypes = [i, SP, SP, i, QP, d, d, i, d, i, P, i64, i, i, i, P]
+        L.bem_regular_assemble.restype = i
+        L.bem_singular_assemble.argtypes = [i, SP, SP, i, P, P, P, P, d, d, P, i64, i, P]
+        L.bem_singular_assemble.restype = i
+        L.bem_potential_workspace_size.argtypes = [i, SP, QP, i, i]
+        L.bem_potential_workspace_size.restype = sz
+        L.bem_potential_evaluate.argtypes = [i, SP, P, i, P, QP, d, d, i, P, i, P, sz, P]
+        L.bem_potential_evaluate.restype = i
+        L.bem_fill_zero.argtypes = [P, sz, P]
+        L.bem_fill_zero.restype = i
+        L.bem_last_error.restype = ctypes.c_char_p
+        L.bem_version.restype = ctypes.c_char_p
+        if hasattr(L, "bem_kbatch_assemble"):
+            L.bem_kbatch_assemble.argtypes = [SP, SP, QP, P, i, d, i, P, i64, i64, P, i, P]
+            L.bem_kbatch_assemble.restype = i
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    """Raise the reference's exception class for a non-zero return code."""
+    if rc != 0:
+        msg = lib().bem_last_error().decode()
+        raise _ERRORS.get(rc, RuntimeError)(f"libbempp_hip error {rc}: {msg}")
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("device_interface='hip' needs a ROCm GPU; none is visible")
+    return torch
+
+
+def device():
+    return _torch().device("cuda", _torch().cuda.current_device())
+
+
+def stream_handle():
+    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def quad_rule(points, weights):
+    r = BemQuadRule()
+    n = len(weights)
+    if n > 16:
+        raise ValueError("quadrature rules with more than 16 points are not supported")
+    r.n_points = n
+    for k in range(n):
+        r.x[k] = float(points[0, k])
+        r.y[k] = float(points[1, k])
+        r.w[k] = float(weights[k])
+    return r
+
+
+# ------------------------------------------------------------------ device caches
+_GRID_CACHE = {}
+_SPACE_CACHE = {}
+
+
+def geometry_table(grid):
+    """(F, 16) float64 host table: v0, v1-v0, v2-v0, normal, integration element, diameter."""
+    F = grid.number_of_elements
+    table = np.zeros((F, GEOM_STRIDE), dtype=np.float64)
+    v = grid.vertices
+    el = grid.elements
+    table[:, 0:3] = v[:, el[0]].T
+    table[:, 3:6] = grid.jacobians[:, :, 0]
+    table[:, 6:9] = grid.jacobians[:, :, 1]
+    table[:, 9:12] = grid.normals
+    table[:, 12] = grid.integration_elements
+    table[:, 13] = grid.diameters
+    return table
+
+
+class DeviceGrid(object):
+    """Geometry table and vertex ids of a grid in HBM."""
+
+    def __init__(self, grid):
+        torch = _torch()
+        dev = device()
+        self.geometry = torch.from_numpy(geometry_table(grid)).to(dev)
+        vids = np.zeros((grid.number_of_elements, 4), dtype=np.int32)
+        vids[:, :3] = grid.elements.T
+        self.vertex_ids = torch.from_numpy(vids).to(dev)
+
+
+def device_grid(grid):
+    key = grid.id
+    if key not in _GRID_CACHE:
+        _GRID_CACHE[key] = DeviceGrid(grid)
+    return _GRID_CACHE[key]
+
+
+class DeviceSpace(object):
+    """Dof map of a space in HBM plus its C descriptor."""
+
+    def __init__(self, space, elements=None):
+        torch = _torch()
+        dev = device()
+        g = device_grid(space.grid)
+        if elements is None:
+            elements = space.support_elements
+        self.grid = g
+        self.elements = torch.from_numpy(np.ascontiguousarray(elements, dtype=np.int32)).to(dev)
+        self.local2global = torch.from_numpy(np.ascontiguousarray(space.local2global, dtype=np.int32)).to(dev)
+        self.multipliers = torch.from_numpy(np.ascontiguousarray(space.local_multipliers, dtype=np.float64)).to(dev)
+        self.normal_multipliers = torch.from_numpy(np.ascontiguousarray(space.normal_multipliers, dtype=np.int32)).to(dev)
+        self.shapeset = SHAPESETS[space.shapeset.identifier]
+        self.struct = BemSpace(
+            int(len(elements)),
+            self.shapeset,
+            self.elements.data_ptr(),
+            g.geometry.data_ptr(),
+            g.vertex_ids.data_ptr(),
+            self.local2global.data_ptr(),
+            self.multipliers.data_ptr(),
+            self.normal_multipliers.data_ptr(),
+        )
+
+
+def device_space(space):
+    key = space.id
+    if key not in _SPACE_CACHE:
+        _SPACE_CACHE[key] = DeviceSpace(space)
+    return _SPACE_CACHE[key]
+
+
+def clear_caches():
+    _GRID_CACHE.clear()
+    _SPACE_CACHE.clear()
+
+
+# ------------------------------------------------------------------ entry points
+def fill_zero(tensor):
+    check(lib().bem_fill_zero(ptr(tensor), tensor.numel() * tensor.element_size(), stream_handle()))
+
+
+def regular_assemble(op, test, trial, grids_identical, rule, wavenumber, precision, out, scatter, accumulate=0,
+                     near_factor=2.0, kahan=True):
+    k = complex(wavenumber)
+    check(
+        lib().bem_regular_assemble(
+            OPERATORS[op], ctypes.byref(test.struct), ctypes.byref(trial.struct), int(grids_identical),
+            ctypes.byref(rule), k.real, k.imag, precision, float(near_factor), int(bool(kahan)), ptr(out),
+            out.shape[-1], DTYPES[str(out.dtype).replace("torch.", "")], int(scatter), int(accumulate),
+            stream_handle(),
+        )
+    )
+
+
+def singular_assemble(op, test, trial, pairs, test_points, trial_points, weights, wavenumber, out):
+    k = complex(wavenumber)
+    n = 0 if pairs is None else pairs.shape[0]
+    check(
+        lib().bem_singular_assemble(
+            OPERATORS[op], ctypes.byref(test.struct), ctypes.byref(trial.struct), n, ptr(pairs), ptr(test_points),
+            ptr(trial_points), ptr(weights), k.real, k.imag, ptr(out), out.shape[-1],
+            DTYPES[str(out.dtype).replace("torch.", "")], stream_handle(),
+        )
+    )
+
+
+def potential_evaluate(op, space, coefficients, points, rule, wavenumber, precision, out):
+    """coefficients: complex128 (F * nshape) device tensor; points: (M, 3) float64 device tensor."""
+    torch = _torch()
+    k = complex(wavenumber)
+    L = lib()
+    n = points.shape[0]
+    need = L.bem_potential_workspace_size(OPERATORS[op], ctypes.byref(space.struct), ctypes.byref(rule), n, precision)
+    ws = torch.empty(need, dtype=torch.uint8, device=points.device)
+    check(
+        L.bem_potential_evaluate(
+            OPERATORS[op], ctypes.byref(space.struct), ptr(coefficients), n, ptr(points), ctypes.byref(rule), k.real,
+            k.imag, precision, ptr(out), DTYPES[str(out.dtype).replace("torch.", "")], ptr(ws), need, stream_handle(),
+        )
+    )
+    return ws

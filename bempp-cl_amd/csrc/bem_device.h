@@ -1,0 +1,176 @@
+// Device-side helpers shared by the dense BEM kernels (gfx950).
+//
+// Green's functions follow bempp_cl/core/numba_kernels.py:176-398 (regular) and
+// 453-551 (singular); they are the textbook Laplace/Helmholtz kernels
+//   SL  G = e^{ikr} / (4 pi r)
+//   DL  (ikr - 1) e^{ikr} (y - x).n_y / (4 pi r^3)       (Laplace: k = 0)
+//   ADL (ikr - 1) e^{ikr} (x - y).n_x / (4 pi r^3)
+// with complex k = kr + i ki entering as e^{ikr} = e^{i kr r} e^{-ki r}.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "bempp_hip.h"
+
+namespace bem {
+
+constexpr double kInv4Pi = 0.079577471545947667884;
+
+enum Op {
+    kLaplaceSL = BEM_LAPLACE_SINGLE_LAYER,
+    kLaplaceDL = BEM_LAPLACE_DOUBLE_LAYER,
+    kLaplaceADL = BEM_LAPLACE_ADJOINT_DOUBLE_LAYER,
+    kHelmholtzSL = BEM_HELMHOLTZ_SINGLE_LAYER,
+    kHelmholtzDL = BEM_HELMHOLTZ_DOUBLE_LAYER,
+    kHelmholtzADL = BEM_HELMHOLTZ_ADJOINT_DOUBLE_LAYER,
+    kMaxwellEFIE = BEM_MAXWELL_ELECTRIC_FIELD,
+};
+
+template <int OP>
+struct OpTraits {
+    static constexpr bool is_complex = OP >= kHelmholtzSL;
+    static constexpr bool needs_trial_normal = OP == kLaplaceDL || OP == kHelmholtzDL;
+    static constexpr bool needs_test_normal = OP == kLaplaceADL || OP == kHelmholtzADL;
+};
+
+__device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ void sincos_t(double x, double *s, double *c) { sincos(x, s, c); }
+__device__ __forceinline__ void sincos_t(float x, float *s, float *c) { sincosf(x, s, c); }
+__device__ __forceinline__ double exp_t(double x) { return exp(x); }
+__device__ __forceinline__ float exp_t(float x) { return __expf(x); }
+
+// Green's function of operator OP for d = y - x, r2 = |d|^2.
+// nx: test normal (ADL), ny: trial normal (DL). Returns (re, im).
+template <int OP, typename T>
+__device__ __forceinline__ void green(T dx, T dy, T dz, T r2, const T *nx, const T *ny, T kr, T ki, T &gre, T &gim) {
+    T rinv = rsqrt_t(r2);
+    if constexpr (OP == kLaplaceSL) {
+        gre = T(kInv4Pi) * rinv;
+        gim = T(0);
+    } else if constexpr (OP == kLaplaceDL) {
+        T dot = dx * ny[0] + dy * ny[1] + dz * ny[2];
+        gre = -T(kInv4Pi) * dot * rinv * rinv * rinv;
+        gim = T(0);
+    } else if constexpr (OP == kLaplaceADL) {
+        T dot = dx * nx[0] + dy * nx[1] + dz * nx[2];
+        gre = T(kInv4Pi) * dot * rinv * rinv * rinv;
+        gim = T(0);
+    } else if constexpr (OP == kHelmholtzSL || OP == kMaxwellEFIE) {
+        T r = r2 * rinv;
+        T s, c;
+        sincos_t(kr * r, &s, &c);
+        T g = T(kInv4Pi) * rinv;
+        if (ki != T(0)) g *= exp_t(-ki * r);
+        gre = c * g;
+        gim = s * g;
+    } else {  // Helmholtz DL / ADL
+        T dot;
+        if constexpr (OP == kHelmholtzDL) dot = dx * ny[0] + dy * ny[1] + dz * ny[2];
+        else dot = -(dx * nx[0] + dy * nx[1] + dz * nx[2]);
+        T r = r2 * rinv;
+        T s, c;
+        sincos_t(kr * r, &s, &c);
+        T g = T(kInv4Pi) * dot * rinv * rinv * rinv;
+        if (ki != T(0)) g *= exp_t(-ki * r);
+        T fr = c * g, fi = s * g;
+        T a = T(-1) - ki * r, b = kr * r;  // i k r - 1 = a + i b
+        gre = a * fr - b * fi;
+        gim = b * fr + a * fi;
+    }
+}
+
+// Compensated (Kahan) accumulator; plain sum when K = false.
+template <typename T, bool K>
+struct Acc {
+    T s, c;
+    __device__ __forceinline__ void zero() { s = T(0); c = T(0); }
+    __device__ __forceinline__ void add(T x) {
+        if constexpr (K) {
+            T y = x - c;
+            T t = s + y;
+            c = (t - s) - y;
+            s = t;
+        } else {
+            s += x;
+        }
+    }
+};
+
+// Atomic adds to HBM (hardware float atomics on gfx950).
+__device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add(float *p, float v) { unsafeAtomicAdd(p, v); }
+
+struct C2d {
+    double re, im;
+};
+
+// ------------------------------------------------------------------ epilogue
+// Local block value (i, j) from the moment block S, for scalar shapesets or EFIE.
+template <int OP, int NTS, int NSS>
+__device__ __forceinline__ C2d local_value(const C2d (&S)[NTS][NSS], int i, int j, const double *gt,
+                                                  const double *gf, double kr, double ki) {
+    C2d v;
+    if constexpr (OP == kMaxwellEFIE) {
+        // M = E_t^T E_f, E = [v1 - v0, v2 - v0]
+        double M[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                M[a][b] = gt[3 + 3 * a] * gf[3 + 3 * b] + gt[4 + 3 * a] * gf[4 + 3 * b] + gt[5 + 3 * a] * gf[5 + 3 * b];
+        const double ox[3] = {0.0, 1.0, 0.0}, oy[3] = {1.0, 0.0, 0.0};
+        double oi[2] = {ox[i], oy[i]}, oj[2] = {ox[j], oy[j]};
+        // sum (xi - o_i)^T M (eta - o_j) G'
+        C2d term = {0.0, 0.0};
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                double mab = M[a][b];
+                term.re += mab * (S[1 + a][1 + b].re - oi[a] * S[0][1 + b].re - oj[b] * S[1 + a][0].re + oi[a] * oj[b] * S[0][0].re);
+                term.im += mab * (S[1 + a][1 + b].im - oi[a] * S[0][1 + b].im - oj[b] * S[1 + a][0].im + oi[a] * oj[b] * S[0][0].im);
+            }
+        double jj = 1.0 / (gt[12] * gf[12]);
+        // -ik term / (J J) - 4 S00 / (ik J J), ik = -ki + i kr
+        double ikr = -ki, iki = kr;
+        double den = ikr * ikr + iki * iki;
+        double inv_re = ikr / den, inv_im = -iki / den;
+        double t_re = term.re * jj, t_im = term.im * jj;
+        double s_re = 4.0 * S[0][0].re * jj, s_im = 4.0 * S[0][0].im * jj;
+        v.re = -(ikr * t_re - iki * t_im) - (inv_re * s_re - inv_im * s_im);
+        v.im = -(ikr * t_im + iki * t_re) - (inv_re * s_im + inv_im * s_re);
+        // edge lengths l0 = |v1 - v0|, l1 = |v2 - v0|, l2 = |v1 - v2|
+        auto elen = [](const double *g, int k) {
+            double a0, a1, a2;
+            if (k == 0) { a0 = g[3]; a1 = g[4]; a2 = g[5]; }
+            else if (k == 1) { a0 = g[6]; a1 = g[7]; a2 = g[8]; }
+            else { a0 = g[3] - g[6]; a1 = g[4] - g[7]; a2 = g[5] - g[8]; }
+            return sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+        };
+        double l = elen(gt, i) * elen(gf, j);
+        v.re *= l;
+        v.im *= l;
+    } else {
+        // phi_i = sum_a C[a][i] u_a with phi = (u0 - u1 - u2, u1, u2); DP0 uses u0 only
+        auto coef = [](int a, int i) -> double {
+            if (i == 0) return a == 0 ? 1.0 : -1.0;
+            return a == i ? 1.0 : 0.0;
+        };
+        v.re = v.im = 0.0;
+#pragma unroll
+        for (int a = 0; a < NTS; ++a)
+#pragma unroll
+            for (int b = 0; b < NSS; ++b) {
+                double c = (NTS == 1 ? 1.0 : coef(a, i)) * (NSS == 1 ? 1.0 : coef(b, j));
+                v.re += c * S[a][b].re;
+                v.im += c * S[a][b].im;
+            }
+    }
+    return v;
+}
+
+}  // namespace bem

@@ -1,0 +1,66 @@
+// Host-side helpers shared by the library's translation units: error reporting,
+// argument structs passed to kernels by value, dispatch utilities.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "bempp_hip.h"
+
+namespace bem {
+
+void set_error(const char *fmt, ...);
+void clear_error();
+
+// One side of an assembly, copied into kernel arguments.
+struct SpaceArgs {
+    int n;
+    int shape;
+    const int32_t *elements;
+    const double *geom;
+    const int32_t *vids;
+    const int32_t *l2g;
+    const double *mult;
+    const int32_t *nmult;
+};
+
+struct RuleArgs {
+    int n;
+    double x[16], y[16], w[16];
+};
+
+inline SpaceArgs to_args(const bem_space_t *s) {
+    SpaceArgs a;
+    a.n = s->n_elements;
+    a.shape = s->shapeset;
+    a.elements = s->elements;
+    a.geom = s->geometry;
+    a.vids = s->vertex_ids;
+    a.l2g = s->local2global;
+    a.mult = s->multipliers;
+    a.nmult = s->normal_multipliers;
+    return a;
+}
+
+inline RuleArgs to_args(const bem_quad_rule_t *r) {
+    RuleArgs a;
+    a.n = r->n_points;
+    for (int i = 0; i < 16; ++i) {
+        a.x[i] = i < r->n_points ? r->x[i] : 0.0;
+        a.y[i] = i < r->n_points ? r->y[i] : 0.0;
+        a.w[i] = i < r->n_points ? r->w[i] : 0.0;
+    }
+    return a;
+}
+
+inline int nshape_of(int shapeset) { return shapeset == BEM_SHAPE_P0 ? 1 : 3; }
+
+// Validate a space description; returns 0 or an error code (message set).
+int check_space(const bem_space_t *s, const char *which);
+
+// Map a HIP error to BEM_ERR_DEVICE with message.
+int check_launch(const char *what);
+
+}  // namespace bem

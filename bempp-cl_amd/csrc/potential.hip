@@ -1,0 +1,402 @@
+// Dense potential evaluation for gfx950.
+//
+//   u(x_m) = sum_e sum_q G(x_m, y_eq) c_eq,  c_eq = J_e w_q sum_i phi_i(q) x_{e,i}
+// (bempp_cl/core/numba_kernels.py:1788-1849) and the EFIE potential
+//   E(x_m) = sum_eq G(x_m, y_eq) [ ik t1_eq - (x_m - y_eq) (ik r - 1) t2_eq / (ik r^2) ]
+// (numba_kernels.py:2281-2359). Three launches:
+//   1. prep: one lane per (support element, quadrature point) writes the source position,
+//      normal and source weights (c, or t1 and t2) to the workspace;
+//   2. eval: a 256-lane workgroup owns 256 targets and one chunk of sources; sources are
+//      staged through LDS in tiles of 256 and read as broadcasts; partial sums per chunk go
+//      to the workspace;
+//   3. reduce: sums the chunk partials in a fixed order (deterministic result).
+// MIXED: per (target, tile) the distance r0 = |x - c| to the tile centre c and e^{ik r0}
+// are formed in FP64; per source only r - r0 = u.(2v + u) / (r + r0), with v = x - c and
+// u = c - y ... small vectors, is needed, so the FP32 phase error stays ~k * eps * tile size
+// instead of k * eps * r.
+#include "bem_device.h"
+#include "bem_internal.h"
+
+namespace bem {
+
+constexpr int kPotThreads = 256;
+constexpr int kPotTile = 256;
+constexpr int kSrcFields = 14;  // px py pz nx ny nz c_re c_im t1 (6): EFIE uses c as t2
+
+struct PotArgs {
+    SpaceArgs sp;
+    RuleArgs rule;
+    const double *coeffs;  // complex128 per (element, shape)
+    int npts;
+    const double *pts;
+    double kr, ki;
+    double *src;  // [kSrcFields][nsrc_pad]
+    int nsrc, nsrc_pad;
+    double *partial;  // [nchunk][kdim][npts][2]
+    int nchunk, chunk_len;
+    void *out;
+};
+
+template <int OP>
+__global__ __launch_bounds__(256) void potential_prep_kernel(const PotArgs a) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.nsrc_pad) return;
+    double *src = a.src;
+    const int P = a.nsrc_pad;
+    if (s >= a.nsrc) {  // padding sources contribute nothing
+        for (int k = 0; k < kSrcFields; ++k) src[k * P + s] = 0.0;
+        src[0 * P + s] = 1e30;  // far away: G ~ 0 and no division by zero
+        return;
+    }
+    const int nq = a.rule.n;
+    int ei = s / nq, q = s % nq;
+    int e = a.sp.elements[ei];
+    const double *g = a.sp.geom + (int64_t)e * BEM_GEOM_STRIDE;
+    double xi = a.rule.x[q], eta = a.rule.y[q];
+    for (int d = 0; d < 3; ++d) src[d * P + s] = g[d] + g[3 + d] * xi + g[6 + d] * eta;
+    double nm = a.sp.nmult ? (double)a.sp.nmult[e] : 1.0;
+    for (int d = 0; d < 3; ++d) src[(3 + d) * P + s] = g[9 + d] * nm;
+    const double *c = a.coeffs;
+    if constexpr (OP == kMaxwellEFIE) {
+        double t2r = 0, t2i = 0, t1r[3] = {0, 0, 0}, t1i[3] = {0, 0, 0};
+        const double ox[3] = {0.0, 1.0, 0.0}, oy[3] = {1.0, 0.0, 0.0};
+        for (int i = 0; i < 3; ++i) {
+            double a0, a1, a2;
+            if (i == 0) { a0 = g[3]; a1 = g[4]; a2 = g[5]; }
+            else if (i == 1) { a0 = g[6]; a1 = g[7]; a2 = g[8]; }
+            else { a0 = g[3] - g[6]; a1 = g[4] - g[7]; a2 = g[5] - g[8]; }
+            double len = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+            double fr = a.rule.w[q] * len * c[2 * (3 * (int64_t)e + i)];
+            double fi = a.rule.w[q] * len * c[2 * (3 * (int64_t)e + i) + 1];
+            // Piola-mapped RWG value times J: E (xi - o_i)
+            double r0 = xi - ox[i], r1 = eta - oy[i];
+            for (int d = 0; d < 3; ++d) {
+                double b = g[3 + d] * r0 + g[6 + d] * r1;
+                t1r[d] += fr * b;
+                t1i[d] += fi * b;
+            }
+            t2r += 2.0 * fr;
+            t2i += 2.0 * fi;
+        }
+        src[6 * P + s] = t2r;
+        src[7 * P + s] = t2i;
+        for (int d = 0; d < 3; ++d) {
+            src[(8 + 2 * d) * P + s] = t1r[d];
+            src[(9 + 2 * d) * P + s] = t1i[d];
+        }
+    } else {
+        int nsh = a.sp.shape == BEM_SHAPE_P0 ? 1 : 3;
+        double phi[3] = {1.0 - xi - eta, xi, eta};
+        if (nsh == 1) phi[0] = 1.0;
+        double cr = 0, ci = 0;
+        for (int i = 0; i < nsh; ++i) {
+            double f = g[12] * a.rule.w[q] * phi[i];
+            cr += f * c[2 * ((int64_t)e * nsh + i)];
+            ci += f * c[2 * ((int64_t)e * nsh + i) + 1];
+        }
+        src[6 * P + s] = cr;
+        src[7 * P + s] = ci;
+    }
+}
+
+// FP64 evaluation
+template <int OP>
+__global__ __launch_bounds__(kPotThreads) void potential_eval_fp64(const PotArgs a) {
+    constexpr int KD = OP == kMaxwellEFIE ? 3 : 1;
+    __shared__ double sh[kSrcFields][kPotTile];
+    const int m = blockIdx.x * kPotThreads + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int s0 = chunk * a.chunk_len;
+    const int s1 = min(s0 + a.chunk_len, a.nsrc_pad);
+    double x0 = 0, x1 = 0, x2 = 0;
+    if (m < a.npts) {
+        x0 = a.pts[3 * (int64_t)m];
+        x1 = a.pts[3 * (int64_t)m + 1];
+        x2 = a.pts[3 * (int64_t)m + 2];
+    }
+    double accr[KD], acci[KD];
+    for (int d = 0; d < KD; ++d) accr[d] = acci[d] = 0.0;
+    const double kr = a.kr, ki = a.ki;
+    const double zero3[3] = {0.0, 0.0, 0.0};
+    for (int t0 = s0; t0 < s1; t0 += kPotTile) {
+        __syncthreads();
+        {
+            int s = t0 + threadIdx.x;
+            const int nf = OP == kMaxwellEFIE ? kSrcFields : 8;
+            for (int k = 0; k < nf; ++k) sh[k][threadIdx.x] = s < s1 ? a.src[(int64_t)k * a.nsrc_pad + s] : (k == 0 ? 1e30 : 0.0);
+        }
+        __syncthreads();
+        const int nt = min(kPotTile, s1 - t0);
+        for (int j = 0; j < nt; ++j) {
+            double dx = sh[0][j] - x0, dy = sh[1][j] - x1, dz = sh[2][j] - x2;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            double ny[3] = {sh[3][j], sh[4][j], sh[5][j]};
+            double gre, gim;
+            green<OP == kMaxwellEFIE ? kHelmholtzSL : OP, double>(dx, dy, dz, r2, zero3, ny, kr, ki, gre, gim);
+            if constexpr (OP == kMaxwellEFIE) {
+                // term_d = ik t1_d - (x - y)_d (ik r - 1) t2 / (ik r^2), x - y = -d
+                double r = sqrt(r2);
+                double ikr_re = -ki, ikr_im = kr;             // ik
+                double nr = ikr_re * r - 1.0, ni = ikr_im * r;  // ik r - 1
+                double dr = ikr_re * r2, di = ikr_im * r2;      // ik r^2
+                double den = dr * dr + di * di;
+                double qr = (nr * dr + ni * di) / den, qi = (ni * dr - nr * di) / den;
+                double t2r = sh[6][j], t2i = sh[7][j];
+                double qtr = qr * t2r - qi * t2i, qti = qr * t2i + qi * t2r;
+                double diffv[3] = {-dx, -dy, -dz};
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    double t1r = sh[8 + 2 * d][j], t1i = sh[9 + 2 * d][j];
+                    double tr = (ikr_re * t1r - ikr_im * t1i) - diffv[d] * qtr;
+                    double ti = (ikr_re * t1i + ikr_im * t1r) - diffv[d] * qti;
+                    accr[d] += gre * tr - gim * ti;
+                    acci[d] += gre * ti + gim * tr;
+                }
+            } else {
+                double cr = sh[6][j], ci = sh[7][j];
+                accr[0] += gre * cr - gim * ci;
+                acci[0] += gre * ci + gim * cr;
+            }
+        }
+    }
+    if (m < a.npts) {
+        for (int d = 0; d < KD; ++d) {
+            double *p = a.partial + 2 * (((int64_t)chunk * KD + d) * a.npts + m);
+            p[0] = accr[d];
+            p[1] = acci[d];
+        }
+    }
+}
+
+// MIXED evaluation: FP32 per source, FP64 per (target, tile)
+template <int OP>
+__global__ __launch_bounds__(kPotThreads) void potential_eval_mixed(const PotArgs a) {
+    constexpr int KD = OP == kMaxwellEFIE ? 3 : 1;
+    constexpr int NF = OP == kMaxwellEFIE ? kSrcFields : 8;
+    __shared__ float sh[NF][kPotTile];
+    __shared__ double sc[3];
+    const int m = blockIdx.x * kPotThreads + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int s0 = chunk * a.chunk_len;
+    const int s1 = min(s0 + a.chunk_len, a.nsrc);
+    double x0 = 0, x1 = 0, x2 = 0;
+    if (m < a.npts) {
+        x0 = a.pts[3 * (int64_t)m];
+        x1 = a.pts[3 * (int64_t)m + 1];
+        x2 = a.pts[3 * (int64_t)m + 2];
+    }
+    double accr[KD], acci[KD];
+    for (int d = 0; d < KD; ++d) accr[d] = acci[d] = 0.0;
+    const float kr = (float)a.kr, ki = (float)a.ki;
+    for (int t0 = s0; t0 < s1; t0 += kPotTile) {
+        const int nt = min(kPotTile, s1 - t0);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int sm = t0 + nt / 2;
+            for (int d = 0; d < 3; ++d) sc[d] = a.src[(int64_t)d * a.nsrc_pad + sm];
+        }
+        __syncthreads();
+        {
+            int s = t0 + threadIdx.x;
+            bool ok = threadIdx.x < nt;
+            for (int d = 0; d < 3; ++d) sh[d][threadIdx.x] = ok ? (float)(sc[d] - a.src[(int64_t)d * a.nsrc_pad + s]) : 0.f;
+            for (int k = 3; k < NF; ++k) sh[k][threadIdx.x] = ok ? (float)a.src[(int64_t)k * a.nsrc_pad + s] : 0.f;
+        }
+        __syncthreads();
+        // per (target, tile) FP64 reference distance and phase
+        double v0d = x0 - sc[0], v1d = x1 - sc[1], v2d = x2 - sc[2];
+        double r0sq_d = v0d * v0d + v1d * v1d + v2d * v2d;
+        double r0d = sqrt(r0sq_d);
+        double ph_s, ph_c;
+        sincos(a.kr * r0d, &ph_s, &ph_c);
+        double damp0 = a.ki != 0.0 ? exp(-a.ki * r0d) : 1.0;
+        const float v0 = (float)v0d, v1 = (float)v1d, v2 = (float)v2d;
+        const float r0 = (float)r0d, r0sq = (float)r0sq_d;
+        float Ar[KD], Ai[KD];
+        for (int d = 0; d < KD; ++d) Ar[d] = Ai[d] = 0.f;
+        for (int j = 0; j < nt; ++j) {
+            // x - y = v + u, u = c - y
+            float u0 = sh[0][j], u1 = sh[1][j], u2 = sh[2][j];
+            float w0 = v0 + u0, w1 = v1 + u1, w2 = v2 + u2;
+            float tt = u0 * (2.f * v0 + u0) + u1 * (2.f * v1 + u1) + u2 * (2.f * v2 + u2);  // r^2 - r0^2
+            float r2 = r0sq + tt;
+            float rinv = rsqrt_t(r2);
+            float r = r2 * rinv;
+            float delta = tt / (r + r0);
+            float s, c;
+            sincos_t(kr * delta, &s, &c);
+            float damp = ki != 0.f ? __expf(-ki * delta) : 1.f;
+            // e^{ik(r - r0)} / (4 pi r) ...
+            float g = float(kInv4Pi) * rinv * damp;
+            float er = c * g, ei = s * g;
+            if constexpr (OP == kLaplaceSL || OP == kHelmholtzSL) {
+                float cr = sh[6][j], ci = sh[7][j];
+                Ar[0] += er * cr - ei * ci;
+                Ai[0] += er * ci + ei * cr;
+            } else if constexpr (OP == kLaplaceDL || OP == kHelmholtzDL) {
+                // (ikr - 1) (y - x).n_y / r^2 factor; y - x = -w
+                float dot = -(w0 * sh[3][j] + w1 * sh[4][j] + w2 * sh[5][j]);
+                float f = dot * rinv * rinv;
+                float ar = -1.f - ki * r, ai = kr * r;
+                float fr = (ar * er - ai * ei) * f, fi = (ar * ei + ai * er) * f;
+                float cr = sh[6][j], ci = sh[7][j];
+                Ar[0] += fr * cr - fi * ci;
+                Ai[0] += fr * ci + fi * cr;
+            } else {  // EFIE
+                float ikr_re = -ki, ikr_im = kr;
+                float nr = ikr_re * r - 1.f, ni = ikr_im * r;
+                float dr = ikr_re * r2, di = ikr_im * r2;
+                float den = dr * dr + di * di;
+                float qr = (nr * dr + ni * di) / den, qi = (ni * dr - nr * di) / den;
+                float t2r = sh[6][j], t2i = sh[7][j];
+                float qtr = qr * t2r - qi * t2i, qti = qr * t2i + qi * t2r;
+                float wv[3] = {w0, w1, w2};
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    float t1r = sh[8 + 2 * d][j], t1i = sh[9 + 2 * d][j];
+                    float tr = (ikr_re * t1r - ikr_im * t1i) - wv[d] * qtr;
+                    float ti = (ikr_re * t1i + ikr_im * t1r) - wv[d] * qti;
+                    Ar[d] += er * tr - ei * ti;
+                    Ai[d] += er * ti + ei * tr;
+                }
+            }
+        }
+        // multiply the tile sum by e^{ik r0} (Laplace: kr = ki = 0 gives 1)
+        double pr = ph_c * damp0, pi = ph_s * damp0;
+        for (int d = 0; d < KD; ++d) {
+            accr[d] += pr * Ar[d] - pi * Ai[d];
+            acci[d] += pr * Ai[d] + pi * Ar[d];
+        }
+    }
+    if (m < a.npts) {
+        for (int d = 0; d < KD; ++d) {
+            double *p = a.partial + 2 * (((int64_t)chunk * KD + d) * a.npts + m);
+            p[0] = accr[d];
+            p[1] = acci[d];
+        }
+    }
+}
+
+template <typename OutT>
+__global__ void potential_reduce_kernel(const PotArgs a, int kd) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kd * a.npts) return;
+    double sr = 0, si = 0;
+    for (int c = 0; c < a.nchunk; ++c) {
+        const double *p = a.partial + 2 * ((int64_t)c * kd * a.npts + i);
+        sr += p[0];
+        si += p[1];
+    }
+    OutT *o = reinterpret_cast<OutT *>(a.out);
+    o[2 * (int64_t)i] = (OutT)sr;
+    o[2 * (int64_t)i + 1] = (OutT)si;
+}
+
+static void plan(int npts, int nsrc, int *nchunk, int *chunk_len, int *nsrc_pad) {
+    *nsrc_pad = ((nsrc + kPotTile - 1) / kPotTile) * kPotTile;
+    int tblocks = (npts + kPotThreads - 1) / kPotThreads;
+    int ntiles = *nsrc_pad / kPotTile;
+    int want = (4096 + tblocks - 1) / tblocks;  // aim at >= 4096 workgroups
+    if (want < 1) want = 1;
+    if (want > ntiles) want = ntiles;
+    if (want > 65535) want = 65535;
+    int tiles_per_chunk = (ntiles + want - 1) / want;
+    *chunk_len = tiles_per_chunk * kPotTile;
+    *nchunk = (*nsrc_pad + *chunk_len - 1) / *chunk_len;
+}
+
+static bool is_potential_op(int op) {
+    return op == BEM_LAPLACE_SINGLE_LAYER || op == BEM_LAPLACE_DOUBLE_LAYER || op == BEM_HELMHOLTZ_SINGLE_LAYER ||
+           op == BEM_HELMHOLTZ_DOUBLE_LAYER || op == BEM_MAXWELL_ELECTRIC_FIELD;
+}
+
+template <int OP>
+static int run_potential(PotArgs &a, int precision, int out_dtype, hipStream_t s) {
+    int kd = OP == kMaxwellEFIE ? 3 : 1;
+    hipLaunchKernelGGL((potential_prep_kernel<OP>), dim3((a.nsrc_pad + 255) / 256), dim3(256), 0, s, a);
+    int rc = check_launch("potential_prep_kernel");
+    if (rc) return rc;
+    dim3 grid((a.npts + kPotThreads - 1) / kPotThreads, a.nchunk);
+    if (precision == BEM_PRECISION_MIXED)
+        hipLaunchKernelGGL((potential_eval_mixed<OP>), grid, dim3(kPotThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL((potential_eval_fp64<OP>), grid, dim3(kPotThreads), 0, s, a);
+    if ((rc = check_launch("potential_eval"))) return rc;
+    int n = kd * a.npts;
+    if (out_dtype == BEM_C128)
+        hipLaunchKernelGGL((potential_reduce_kernel<double>), dim3((n + 255) / 256), dim3(256), 0, s, a, kd);
+    else
+        hipLaunchKernelGGL((potential_reduce_kernel<float>), dim3((n + 255) / 256), dim3(256), 0, s, a, kd);
+    return check_launch("potential_reduce_kernel");
+}
+
+}  // namespace bem
+
+using namespace bem;
+
+extern "C" size_t bem_potential_workspace_size(int op, const bem_space_t *space, const bem_quad_rule_t *rule,
+                                               int n_points, int precision) {
+    (void)precision;
+    if (!space || !rule || n_points < 0) return 0;
+    int nsrc = space->n_elements * rule->n_points, nchunk, chunk_len, nsrc_pad;
+    plan(n_points, nsrc, &nchunk, &chunk_len, &nsrc_pad);
+    int kd = op == BEM_MAXWELL_ELECTRIC_FIELD ? 3 : 1;
+    return sizeof(double) * ((size_t)kSrcFields * nsrc_pad + (size_t)2 * nchunk * kd * n_points) + 256;
+}
+
+extern "C" int bem_potential_evaluate(int op, const bem_space_t *space, const void *coefficients, int n_points,
+                                      const double *points, const bem_quad_rule_t *rule, double k_re, double k_im,
+                                      int precision, void *out, int out_dtype, void *workspace, size_t workspace_bytes,
+                                      void *stream) {
+    clear_error();
+    if (!space || !rule || !out || !coefficients || (n_points > 0 && !points) || !workspace) {
+        set_error("null argument");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (!is_potential_op(op)) {
+        set_error("operator %d has no potential", op);
+        return BEM_ERR_UNSUPPORTED;
+    }
+    int rc;
+    if ((rc = check_space(space, "potential"))) return rc;
+    if ((op == BEM_MAXWELL_ELECTRIC_FIELD) != (space->shapeset == BEM_SHAPE_RWG)) {
+        set_error("the electric field potential needs an RWG space, scalar potentials a scalar space");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (out_dtype != BEM_C128 && out_dtype != BEM_C64) {
+        set_error("potential output must be complex (BEM_C128 or BEM_C64)");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (rule->n_points < 1 || rule->n_points > 16) {
+        set_error("quadrature rule must have 1..16 points");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    size_t need = bem_potential_workspace_size(op, space, rule, n_points, precision);
+    if (workspace_bytes < need) {
+        set_error("workspace too small: %zu < %zu bytes", workspace_bytes, need);
+        return BEM_ERR_WORKSPACE;
+    }
+    if (n_points == 0) return BEM_OK;
+    PotArgs a;
+    a.sp = to_args(space);
+    a.rule = to_args(rule);
+    a.coeffs = (const double *)coefficients;
+    a.npts = n_points;
+    a.pts = points;
+    a.kr = (op == BEM_LAPLACE_SINGLE_LAYER || op == BEM_LAPLACE_DOUBLE_LAYER) ? 0.0 : k_re;
+    a.ki = (op == BEM_LAPLACE_SINGLE_LAYER || op == BEM_LAPLACE_DOUBLE_LAYER) ? 0.0 : k_im;
+    a.nsrc = space->n_elements * rule->n_points;
+    plan(n_points, a.nsrc, &a.nchunk, &a.chunk_len, &a.nsrc_pad);
+    uintptr_t base = ((uintptr_t)workspace + 255) & ~(uintptr_t)255;
+    a.src = (double *)base;
+    a.partial = a.src + (size_t)kSrcFields * a.nsrc_pad;
+    a.out = out;
+    hipStream_t s = (hipStream_t)stream;
+    switch (op) {
+    case BEM_LAPLACE_SINGLE_LAYER: return run_potential<kLaplaceSL>(a, precision, out_dtype, s);
+    case BEM_LAPLACE_DOUBLE_LAYER: return run_potential<kLaplaceDL>(a, precision, out_dtype, s);
+    case BEM_HELMHOLTZ_SINGLE_LAYER: return run_potential<kHelmholtzSL>(a, precision, out_dtype, s);
+    case BEM_HELMHOLTZ_DOUBLE_LAYER: return run_potential<kHelmholtzDL>(a, precision, out_dtype, s);
+    default: return run_potential<kMaxwellEFIE>(a, precision, out_dtype, s);
+    }
+}

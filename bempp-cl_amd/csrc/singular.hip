@@ -1,0 +1,175 @@
+// Singular (Sauter-Schwab) pairs, FP64, for gfx950.
+//
+// One 64-lane wavefront per (test element, trial element) pair that shares a vertex,
+// an edge or the whole triangle (bempp_cl/core/numba_kernels.py:1471-1526 scalar,
+// 1973-2061 EFIE). The pair's 512..1536 Duffy points are strided over the lanes; each
+// lane accumulates the 3x3 moment block S[a][b] = sum_k G_k w_k u_a(xi_k) u_b(eta_k)
+// with u = (1, xi_0, xi_1), the block is reduced across the wave with cross-lane
+// shuffles, scaled by J_test J_trial and turned into the local matrix block exactly as
+// for regular pairs. Lanes 0..8 add the multiplied values to the dense output with
+// hardware atomics, which replaces the reference's host-side np.add.at scatter
+// (bempp_cl/core/dense_assembler.py:78-98).
+#include "bem_device.h"
+#include "bem_internal.h"
+
+namespace bem {
+
+struct SingularArgs {
+    SpaceArgs test, trial;
+    int n_pairs;
+    const int32_t *pairs;
+    const double *tpts, *spts, *wts;
+    double kr, ki;
+    void *out;
+    int64_t ld;
+};
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int OP, int NTS, int NSS, typename OutT>
+__global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
+    constexpr bool CPLX = OpTraits<OP>::is_complex;
+    const int lane = threadIdx.x & 63;
+    const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pair >= a.n_pairs) return;  // uniform per wave
+    const int32_t *pd = a.pairs + 6 * (int64_t)pair;
+    const int e = pd[0], f = pd[1], toff = pd[2], soff = pd[3], woff = pd[4], np = pd[5];
+    const double *gt = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
+    const double *gf = a.trial.geom + (int64_t)f * BEM_GEOM_STRIDE;
+    double tnm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
+    double snm = a.trial.nmult ? (double)a.trial.nmult[f] : 1.0;
+    double nx[3] = {gt[9] * tnm, gt[10] * tnm, gt[11] * tnm};
+    double ny[3] = {gf[9] * snm, gf[10] * snm, gf[11] * snm};
+
+    double Sr[NTS][NSS], Si[NTS][NSS];
+#pragma unroll
+    for (int i = 0; i < NTS; ++i)
+#pragma unroll
+        for (int j = 0; j < NSS; ++j) Sr[i][j] = Si[i][j] = 0.0;
+
+    for (int k = lane; k < np; k += 64) {
+        double xi0 = a.tpts[2 * (int64_t)(toff + k)], xi1 = a.tpts[2 * (int64_t)(toff + k) + 1];
+        double et0 = a.spts[2 * (int64_t)(soff + k)], et1 = a.spts[2 * (int64_t)(soff + k) + 1];
+        double w = a.wts[woff + k];
+        double dx = (gf[0] + gf[3] * et0 + gf[6] * et1) - (gt[0] + gt[3] * xi0 + gt[6] * xi1);
+        double dy = (gf[1] + gf[4] * et0 + gf[7] * et1) - (gt[1] + gt[4] * xi0 + gt[7] * xi1);
+        double dz = (gf[2] + gf[5] * et0 + gf[8] * et1) - (gt[2] + gt[5] * xi0 + gt[8] * xi1);
+        double r2 = dx * dx + dy * dy + dz * dz;
+        double gre, gim;
+        green<OP, double>(dx, dy, dz, r2, nx, ny, a.kr, a.ki, gre, gim);
+        gre *= w;
+        gim *= w;
+        double ut[3] = {1.0, xi0, xi1}, us[3] = {1.0, et0, et1};
+#pragma unroll
+        for (int i = 0; i < NTS; ++i)
+#pragma unroll
+            for (int j = 0; j < NSS; ++j) {
+                double u = ut[i] * us[j];
+                Sr[i][j] += gre * u;
+                Si[i][j] += gim * u;
+            }
+    }
+    const double jj = gt[12] * gf[12];
+    C2d S[NTS][NSS];
+#pragma unroll
+    for (int i = 0; i < NTS; ++i)
+#pragma unroll
+        for (int j = 0; j < NSS; ++j) {
+            S[i][j].re = wave_sum(Sr[i][j]) * jj;
+            S[i][j].im = CPLX ? wave_sum(Si[i][j]) * jj : 0.0;
+        }
+    // lanes 0 .. NTS*NSS-1 each add one entry
+    if (lane < NTS * NSS) {
+        int i = lane / NSS, j = lane % NSS;
+        double m = a.test.mult[(int64_t)e * NTS + i] * a.trial.mult[(int64_t)f * NSS + j];
+        if (m != 0.0) {
+            C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
+            int row = a.test.l2g[(int64_t)e * NTS + i];
+            int col = a.trial.l2g[(int64_t)f * NSS + j];
+            OutT *o = reinterpret_cast<OutT *>(a.out);
+            int64_t idx = (int64_t)row * a.ld + col;
+            if constexpr (CPLX) {
+                atomic_add(o + 2 * idx, (OutT)(v.re * m));
+                atomic_add(o + 2 * idx + 1, (OutT)(v.im * m));
+            } else {
+                atomic_add(o + idx, (OutT)(v.re * m));
+            }
+        }
+    }
+}
+
+template <int OP, typename OutT>
+static int launch_sing(const SingularArgs &a, hipStream_t s) {
+    dim3 grid((a.n_pairs + 3) / 4);
+    if (a.n_pairs == 0) return BEM_OK;
+    if constexpr (OP == kMaxwellEFIE) {
+        hipLaunchKernelGGL((singular_kernel<OP, 3, 3, OutT>), grid, dim3(256), 0, s, a);
+    } else {
+        int nt = nshape_of(a.test.shape), ns = nshape_of(a.trial.shape);
+        if (nt == 1 && ns == 1) hipLaunchKernelGGL((singular_kernel<OP, 1, 1, OutT>), grid, dim3(256), 0, s, a);
+        else if (nt == 1 && ns == 3) hipLaunchKernelGGL((singular_kernel<OP, 1, 3, OutT>), grid, dim3(256), 0, s, a);
+        else if (nt == 3 && ns == 1) hipLaunchKernelGGL((singular_kernel<OP, 3, 1, OutT>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((singular_kernel<OP, 3, 3, OutT>), grid, dim3(256), 0, s, a);
+    }
+    return check_launch("singular_kernel");
+}
+
+template <int OP>
+static int launch_sing_op(const SingularArgs &a, int out_dtype, hipStream_t s) {
+    constexpr bool CPLX = OpTraits<OP>::is_complex;
+    if (CPLX && out_dtype == BEM_C128) return launch_sing<OP, double>(a, s);
+    if (CPLX && out_dtype == BEM_C64) return launch_sing<OP, float>(a, s);
+    if (!CPLX && out_dtype == BEM_F64) return launch_sing<OP, double>(a, s);
+    if (!CPLX && out_dtype == BEM_F32) return launch_sing<OP, float>(a, s);
+    set_error("output dtype %d does not match operator %d", out_dtype, OP);
+    return BEM_ERR_INVALID_ARGUMENT;
+}
+
+}  // namespace bem
+
+using namespace bem;
+
+extern "C" int bem_singular_assemble(int op, const bem_space_t *test, const bem_space_t *trial, int n_pairs,
+                                     const int32_t *pair_data, const double *test_points, const double *trial_points,
+                                     const double *weights, double k_re, double k_im, void *out, int64_t ld,
+                                     int out_dtype, void *stream) {
+    clear_error();
+    if (!test || !trial || !out || (n_pairs > 0 && (!pair_data || !test_points || !trial_points || !weights))) {
+        set_error("null argument");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (n_pairs < 0) {
+        set_error("negative pair count");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    int rc;
+    if ((rc = check_space(test, "test")) || (rc = check_space(trial, "trial"))) return rc;
+    SingularArgs a;
+    a.test = to_args(test);
+    a.trial = to_args(trial);
+    a.n_pairs = n_pairs;
+    a.pairs = pair_data;
+    a.tpts = test_points;
+    a.spts = trial_points;
+    a.wts = weights;
+    a.kr = k_re;
+    a.ki = k_im;
+    a.out = out;
+    a.ld = ld;
+    hipStream_t s = (hipStream_t)stream;
+    switch (op) {
+    case BEM_LAPLACE_SINGLE_LAYER: return launch_sing_op<kLaplaceSL>(a, out_dtype, s);
+    case BEM_LAPLACE_DOUBLE_LAYER: return launch_sing_op<kLaplaceDL>(a, out_dtype, s);
+    case BEM_LAPLACE_ADJOINT_DOUBLE_LAYER: return launch_sing_op<kLaplaceADL>(a, out_dtype, s);
+    case BEM_HELMHOLTZ_SINGLE_LAYER: return launch_sing_op<kHelmholtzSL>(a, out_dtype, s);
+    case BEM_HELMHOLTZ_DOUBLE_LAYER: return launch_sing_op<kHelmholtzDL>(a, out_dtype, s);
+    case BEM_HELMHOLTZ_ADJOINT_DOUBLE_LAYER: return launch_sing_op<kHelmholtzADL>(a, out_dtype, s);
+    case BEM_MAXWELL_ELECTRIC_FIELD: return launch_sing_op<kMaxwellEFIE>(a, out_dtype, s);
+    default: set_error("unsupported operator %d", op); return BEM_ERR_UNSUPPORTED;
+    }
+}

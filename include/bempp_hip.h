@@ -1,0 +1,156 @@
+/*
+ * bempp_hip.h -- C ABI of the MI355X (gfx950) dense BEM library libbempp_hip.so.
+ *
+ * The entry points replace the three backend functions the reference dispatches to
+ * (bempp_cl/core/dispatcher.py:4-55):
+ *   bem_regular_assemble   <- dense_assembler(...)     bempp_cl/core/numba_assemblers.py:58-112,
+ *                                                      bempp_cl/core/opencl_assemblers.py:108-284
+ *   bem_singular_assemble  <- singular_assembler(...)  bempp_cl/core/numba_assemblers.py:6-55,
+ *                                                      bempp_cl/core/opencl_assemblers.py:12-105
+ *                             plus the np.add.at scatter of bempp_cl/core/dense_assembler.py:78-98
+ *   bem_potential_evaluate <- potential_assembler(...) bempp_cl/core/numba_assemblers.py:115-164,
+ *                                                      bempp_cl/core/opencl_assemblers.py:287-465
+ * Extra entry points with no reference counterpart: bem_kbatch_assemble (many Helmholtz
+ * wavenumbers in one persistent launch), bem_adjoint_* (gradients), bem_fill_zero,
+ * bem_potential_workspace_size, bem_last_error, bem_version.
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers (hipMalloc'ed or torch device memory) unless
+ *    the comment says host. The library allocates nothing; callers pass workspaces.
+ *  - Work is enqueued on `stream` (a hipStream_t, NULL = default stream) and the call
+ *    returns without synchronising. Results are complete once the stream is synchronised.
+ *  - Return value 0 = success; otherwise one of the BEM_ERR_* codes, with a message
+ *    available from bem_last_error() (thread-local).
+ *  - Dense outputs are row-major: entry (row, col) at out[row * ld + col]; rows index
+ *    dual_to_range (test) dofs, columns domain (trial) dofs, as the reference's
+ *    np.zeros((rows, cols)) result. Complex outputs are interleaved (re, im).
+ */
+#ifndef BEMPP_HIP_H
+#define BEMPP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* operators (kernel_type / assembly_type of the reference OperatorDescriptor) */
+enum bem_operator {
+    BEM_LAPLACE_SINGLE_LAYER = 0,
+    BEM_LAPLACE_DOUBLE_LAYER = 1,
+    BEM_LAPLACE_ADJOINT_DOUBLE_LAYER = 2,
+    BEM_HELMHOLTZ_SINGLE_LAYER = 3,
+    BEM_HELMHOLTZ_DOUBLE_LAYER = 4,
+    BEM_HELMHOLTZ_ADJOINT_DOUBLE_LAYER = 5,
+    BEM_MAXWELL_ELECTRIC_FIELD = 6
+};
+
+/* reference shapesets (bempp_cl/api/space/shapesets.py:46-158) */
+enum bem_shapeset { BEM_SHAPE_P0 = 0, BEM_SHAPE_P1 = 1, BEM_SHAPE_RWG = 2 };
+
+/* output element types */
+enum bem_dtype { BEM_F32 = 0, BEM_F64 = 1, BEM_C64 = 2, BEM_C128 = 3 };
+
+/* arithmetic modes: FP64 everywhere, or FP32 for well-separated regular pairs with FP64
+ * for near-singular regular pairs and all singular pairs */
+enum bem_precision { BEM_PRECISION_FP64 = 0, BEM_PRECISION_MIXED = 1 };
+
+enum bem_error {
+    BEM_OK = 0,
+    BEM_ERR_INVALID_ARGUMENT = 1, /* reference raises ValueError */
+    BEM_ERR_UNSUPPORTED = 2,      /* reference raises KeyError (unsupported kernel) */
+    BEM_ERR_DEVICE = 3,           /* reference raises RuntimeError (device problem) */
+    BEM_ERR_WORKSPACE = 4         /* workspace too small */
+};
+
+/* Number of doubles per element in the geometry table */
+#define BEM_GEOM_STRIDE 16
+/* Geometry table row of element e (all float64):
+ *  [0..2] v0, [3..5] v1 - v0, [6..8] v2 - v0, [9..11] unit normal, [12] integration element
+ *  |(v1-v0) x (v2-v0)|, [13] diameter, [14..15] unused.  (bempp_cl/api/grid/grid.py:552-590) */
+
+/* One side (test or trial) of an assembly: a function space on a grid, device arrays. */
+typedef struct bem_space {
+    int32_t n_elements;         /* number of entries in `elements` (the support elements)  */
+    int32_t shapeset;           /* enum bem_shapeset                                        */
+    const int32_t *elements;    /* [n_elements] grid element indices                        */
+    const double *geometry;     /* [n_grid_elements * BEM_GEOM_STRIDE]                      */
+    const int32_t *vertex_ids;  /* [n_grid_elements * 4] vertex indices (4th unused)       */
+    const int32_t *local2global;/* [n_grid_elements * nshape] global dof of shape function */
+    const double *multipliers;  /* [n_grid_elements * nshape] local multipliers            */
+    const int32_t *normal_multipliers; /* [n_grid_elements] +1/-1, or NULL for all +1      */
+} bem_space_t;
+
+/* A triangle quadrature rule, host memory, at most 16 points on the reference triangle. */
+typedef struct bem_quad_rule {
+    int32_t n_points;
+    double x[16], y[16], w[16];
+} bem_quad_rule_t;
+
+/*
+ * Regular part of a dense Galerkin operator: all (test, trial) element pairs, except,
+ * when grids_identical != 0, pairs sharing a vertex (bempp_cl/core/numba_kernels.py:980-1074,
+ * 1853-1969; pairs sharing vertices are left to bem_singular_assemble).
+ *
+ * scatter: 0 = direct. Every pair owns its output entries (both spaces localised: DP0/DP1,
+ *              one dof per (element, shape function)); entries are written once, added to
+ *              the existing value when accumulate != 0, stored otherwise (no prior zeroing).
+ *          1 = atomic. Dofs are shared between elements (P1, RWG/SNC); contributions are
+ *              added atomically, so `out` must be initialised (e.g. bem_fill_zero).
+ * k_re, k_im: wavenumber for Helmholtz/Maxwell (ignored for Laplace).
+ * precision: enum bem_precision. mixed_near_factor: in MIXED mode, pairs whose centroid
+ *   distance is below factor * (diameter_test + diameter_trial) / 2 use FP64.
+ * kahan: compensated FP32 summation in MIXED mode.
+ * out_dtype must be BEM_F64/BEM_F32 for Laplace and BEM_C128/BEM_C64 otherwise.
+ */
+int bem_regular_assemble(int op, const bem_space_t *test, const bem_space_t *trial, int grids_identical,
+                         const bem_quad_rule_t *rule, double k_re, double k_im, int precision,
+                         double mixed_near_factor, int kahan, void *out, int64_t ld, int out_dtype,
+                         int scatter, int accumulate, void *stream);
+
+/*
+ * Singular (Sauter-Schwab) part, always FP64: for each of n_pairs pairs p,
+ *   pair_data[6 p + 0..5] = test element, trial element, test point offset, trial point offset,
+ *                           weight offset, number of points
+ * with reference coordinates test_points[2 * (offset + k) + 0/1], trial_points likewise and
+ * weights[weight offset + k] (device arrays, bempp_cl/core/singular_assembler.py:225-448).
+ * The local block is multiplied by the local multipliers (and RWG edge lengths) and added
+ * atomically to out at (local2global_test, local2global_trial).
+ */
+int bem_singular_assemble(int op, const bem_space_t *test, const bem_space_t *trial, int n_pairs,
+                          const int32_t *pair_data, const double *test_points, const double *trial_points,
+                          const double *weights, double k_re, double k_im, void *out, int64_t ld, int out_dtype,
+                          void *stream);
+
+/*
+ * Dense potential u(x_m) (bempp_cl/core/numba_kernels.py:1788-1849, 2281-2359):
+ * coefficients: complex128 [n_grid_elements * nshape] per (element, shape function), already
+ *   multiplied by the local multipliers (space.map_to_full_grid @ x).
+ * points: float64 [3 * n_points] (x0 y0 z0 x1 ...). out: [kdim * n_points] of out_dtype
+ *   (BEM_C128 or BEM_C64), row d holds component d (kdim = 3 for Maxwell, else 1).
+ * Only BEM_LAPLACE_SINGLE_LAYER, BEM_LAPLACE_DOUBLE_LAYER, BEM_HELMHOLTZ_SINGLE_LAYER,
+ * BEM_HELMHOLTZ_DOUBLE_LAYER and BEM_MAXWELL_ELECTRIC_FIELD are potentials.
+ * workspace: device memory of at least bem_potential_workspace_size(...) bytes.
+ */
+size_t bem_potential_workspace_size(int op, const bem_space_t *space, const bem_quad_rule_t *rule, int n_points,
+                                    int precision);
+int bem_potential_evaluate(int op, const bem_space_t *space, const void *coefficients, int n_points,
+                           const double *points, const bem_quad_rule_t *rule, double k_re, double k_im,
+                           int precision, void *out, int out_dtype, void *workspace, size_t workspace_bytes,
+                           void *stream);
+
+/* Set `bytes` bytes at `ptr` to zero with a fill kernel (safe inside graph capture). */
+int bem_fill_zero(void *ptr, size_t bytes, void *stream);
+
+/* Message of the last error on this thread ("" if none). */
+const char *bem_last_error(void);
+
+/* Library version string. */
+const char *bem_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEMPP_HIP_H */

@@ -1,0 +1,128 @@
+"""Quadrature rules: exactness, point counts and agreement with the oracle's independent generators.
+
+Pins: symmetric triangle rules integrate monomials x^a y^b exactly (a! b! / (a+b+2)!) up to
+their degree (triangle_gauss.py:2928-2940); 1-D Gauss-Legendre equals numpy's leggauss on
+[0, 1] (gauss.py:1042-1055); Sauter-Schwab rules have 6/5/2 n^4 points
+(test/unit/integration/test_duffy.py:8-32), weights summing to 1/4 = area(T)^2, and integrate
+smooth polynomials over T x T exactly.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+
+def exact_monomial(a, b):
+    return math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
+def test_triangle_rule_exactness(order):
+    from bempp_amd.api.integration.triangle_gauss import rule
+
+    pts, w = rule(order)
+    assert abs(w.sum() - 0.5) < 1e-15
+    for a in range(order + 1):
+        for b in range(order + 1 - a):
+            approx = np.sum(w * pts[0] ** a * pts[1] ** b)
+            assert abs(approx - exact_monomial(a, b)) < 1e-15, (order, a, b)
+
+
+def test_triangle_rule_not_exact_beyond_degree():
+    """A dropped or wrong weight would still integrate constants; degree 5 must fail for order 4."""
+    from bempp_amd.api.integration.triangle_gauss import rule
+
+    pts, w = rule(4)
+    errs = [abs(np.sum(w * pts[0] ** a * pts[1] ** (5 - a)) - exact_monomial(a, 5 - a)) for a in range(6)]
+    assert max(errs) > 1e-8
+
+
+def test_triangle_rule_point_counts():
+    from bempp_amd.api.integration.triangle_gauss import get_number_of_quad_points
+
+    assert [get_number_of_quad_points(o) for o in range(1, 6)] == [1, 3, 4, 6, 7]
+    from bempp_amd.api.integration.triangle_gauss import rule
+
+    with pytest.raises(ValueError):
+        rule(6)
+
+
+@pytest.mark.parametrize("n", [1, 2, 4, 7])
+def test_gauss_matches_leggauss_and_oracle(n):
+    from bempp_amd.api.integration.gauss import rule
+    from oracle import oracle as O
+
+    x, w = rule(n)
+    xr, wr = np.polynomial.legendre.leggauss(n)
+    np.testing.assert_allclose(np.sort(x), 0.5 * (1 + xr), atol=1e-15)
+    np.testing.assert_allclose(w.sum(), 1.0, atol=1e-15)
+    xo, wo = O.gauss01(n)
+    np.testing.assert_allclose(x, xo, atol=1e-15)
+    np.testing.assert_allclose(w, wo, atol=1e-14)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("kind,factor", [("coincident", 6), ("edge_adjacent", 5), ("vertex_adjacent", 2)])
+def test_duffy_rule(order, kind, factor):
+    from bempp_amd.api.integration.duffy_galerkin import number_of_quadrature_points, rule
+    from oracle import oracle as O
+
+    t, s, w = rule(order, kind)
+    assert len(w) == factor * order**4 == number_of_quadrature_points(order, kind)
+    if order >= 2:  # the Duffy Jacobians are polynomials of degree <= 3 per variable
+        assert abs(w.sum() - 0.25) < 1e-14
+    # all points inside the reference triangle
+    for p in (t, s):
+        assert np.all(p >= -1e-15) and np.all(p[0] + p[1] <= 1 + 1e-15)
+    tx, ty, sx, sy, wo = O.duffy(order, kind)
+    np.testing.assert_allclose(t, np.vstack([tx, ty]), atol=1e-15)
+    np.testing.assert_allclose(s, np.vstack([sx, sy]), atol=1e-15)
+    np.testing.assert_allclose(w, wo, rtol=1e-13)
+
+
+def _pair_geometry(kind):
+    """Two triangles in the plane sharing the configuration the canonical rule assumes."""
+    t = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    if kind == "coincident":
+        s = t.copy()
+    elif kind == "edge_adjacent":
+        s = np.array([[0.0, 0.0], [1.0, 0.0], [0.4, -0.9]])
+    else:
+        s = np.array([[0.0, 0.0], [-1.0, -0.2], [-0.3, -1.1]])
+    return t, s
+
+
+@pytest.mark.parametrize("kind", ["coincident", "edge_adjacent", "vertex_adjacent"])
+def test_duffy_integrates_smooth_function(kind):
+    """int_T int_S f(x, y) for a polynomial f, against a tensor product of regular rules."""
+    from bempp_amd.api.integration.duffy_galerkin import rule
+    from bempp_amd.api.integration.triangle_gauss import rule as trule
+
+    t, s = _pair_geometry(kind)
+    Jt = abs(np.linalg.det(np.array([t[1] - t[0], t[2] - t[0]])))
+    Js = abs(np.linalg.det(np.array([s[1] - s[0], s[2] - s[0]])))
+
+    def f(x, y):
+        return 1.0 + x[0] * y[1] + (x[1] - y[0]) ** 2 + x[0] ** 2 * y[0]
+
+    def mapping(tri, p):
+        return tri[0][:, None] + np.outer(tri[1] - tri[0], p[0]) + np.outer(tri[2] - tri[0], p[1])
+
+    tp, sp, w = rule(5, kind)
+    val = np.sum(w * f(mapping(t, tp), mapping(s, sp))) * Jt * Js
+    qp, qw = trule(5)
+    X = mapping(t, qp)
+    Y = mapping(s, qp)
+    ref = sum(qw[i] * qw[j] * f(X[:, i : i + 1], Y[:, j : j + 1])[0] for i in range(len(qw)) for j in range(len(qw))) * Jt * Js
+    assert abs(val - ref) < 1e-12 * abs(ref)
+
+
+def test_remap_shared_edge_maps_edge_to_edge():
+    from bempp_amd.api.integration.duffy_galerkin import remap_points_shared_edge
+
+    ref = np.array([[0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    for a, b in [(0, 1), (1, 0), (1, 2), (2, 1), (0, 2), (2, 0)]:
+        out = remap_points_shared_edge(ref, a, b)
+        np.testing.assert_allclose(out[:, 0], ref[:, a])
+        np.testing.assert_allclose(out[:, 1], ref[:, b])
