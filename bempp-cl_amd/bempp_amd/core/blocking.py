@@ -1,0 +1,157 @@
+"""Locality-ordered element blocks with per-block dof slots for the tiled regular kernel.
+
+For spaces whose dofs are shared between elements (P1, RWG/SNC), the regular kernel sums
+a workgroup's contributions in an LDS tile indexed by per-block dof slots and adds the
+tile to the dense matrix once per entry. This module builds the host side of that
+scheme: a space-filling-curve order of the support elements (so that consecutive elements
+share dofs), greedy blocks of at most `max_elements` elements and `max_dofs` distinct
+dofs, the distinct dof list of each block and the slot of every (element, shape function).
+"""
+
+import numpy as _np
+
+TEST_TILE_ELEMENTS = 16
+TEST_TILE_DOFS = 24
+TRIAL_BLOCK_ELEMENTS = 256
+TRIAL_BLOCK_DOFS = {8: 192, 16: 160}  # by output element size (complex64 / complex128)
+
+_CACHE = {}
+
+
+def _morton_keys(points):
+    """63-bit Morton codes of points (n, 3) in their bounding box."""
+    lo = points.min(axis=0)
+    span = points.max(axis=0) - lo
+    span[span == 0] = 1.0
+    q = ((points - lo) / span * (2**21 - 1)).astype(_np.uint64)
+    key = _np.zeros(len(points), dtype=_np.uint64)
+    for bit in range(21):
+        for d in range(3):
+            key |= ((q[:, d] >> _np.uint64(bit)) & _np.uint64(1)) << _np.uint64(3 * bit + d)
+    return key
+
+
+class Blocks(object):
+    """Host arrays of one blocking."""
+
+    def __init__(self, order, start, dofs, ndofs, slots, max_dofs):
+        self.order = order  # element indices in list order
+        self.start = start
+        self.dofs = dofs
+        self.ndofs = ndofs
+        self.slots = slots
+        self.max_dofs = max_dofs
+
+    @property
+    def n_blocks(self):
+        return len(self.ndofs)
+
+    @property
+    def total_slots(self):
+        return int(self.ndofs.sum())
+
+
+def _greedy_blocks(order, l2g, mult, max_elements, max_dofs):
+    nshape = l2g.shape[1]
+    starts = [0]
+    block_dofs = []
+    slots = _np.zeros((len(order), 3), dtype=_np.int32)
+    current = {}
+    count = 0
+    for pos, e in enumerate(order):
+        dofs = [int(l2g[e, i]) for i in range(nshape) if mult[e, i] != 0]
+        new = len({d for d in dofs if d not in current})
+        if count == max_elements or len(current) + new > max_dofs:
+            block_dofs.append(current)
+            starts.append(pos)
+            current = {}
+            count = 0
+        for i in range(nshape):
+            if mult[e, i] != 0:
+                d = int(l2g[e, i])
+                if d not in current:
+                    current[d] = len(current)
+                slots[pos, i] = current[d]
+        count += 1
+    block_dofs.append(current)
+    starts.append(len(order))
+    nb = len(block_dofs)
+    dofs = _np.zeros((nb, max_dofs), dtype=_np.int32)
+    ndofs = _np.zeros(nb, dtype=_np.int32)
+    for b, cur in enumerate(block_dofs):
+        # slots in ascending dof order: consecutive lanes of the flush then write
+        # neighbouring columns of a matrix row, which share cache lines
+        sorted_dofs = sorted(cur)
+        rank = {d: s for s, d in enumerate(sorted_dofs)}
+        ndofs[b] = len(cur)
+        dofs[b, : len(cur)] = sorted_dofs
+        old_to_new = _np.zeros(max(len(cur), 1), dtype=_np.int32)
+        for d, s in cur.items():
+            old_to_new[s] = rank[d]
+        seg = slice(starts[b], starts[b + 1])
+        slots[seg] = old_to_new[slots[seg]]
+    return Blocks(_np.asarray(order, dtype=_np.int32), _np.asarray(starts, dtype=_np.int32), dofs, ndofs, slots,
+                  max_dofs)
+
+
+def make_blocks(space, max_elements, max_dofs):
+    """Best of the grid order and the Morton order of centroids, by total slot count."""
+    key = (space.id, max_elements, max_dofs)
+    if key in _CACHE:
+        return _CACHE[key]
+    elements = space.support_elements.astype(_np.int64)
+    l2g = space.local2global
+    mult = space.local_multipliers
+    natural = _greedy_blocks(elements, l2g, mult, max_elements, max_dofs)
+    morton_order = elements[_np.argsort(_morton_keys(space.grid.centroids[elements]), kind="stable")]
+    morton = _greedy_blocks(morton_order, l2g, mult, max_elements, max_dofs)
+    best = natural if natural.total_slots <= morton.total_slots else morton
+    _CACHE[key] = best
+    return best
+
+
+class DeviceBlocks(object):
+    """A blocking in HBM with the space descriptor over its element order."""
+
+    def __init__(self, space, blocks):
+        import torch
+        from bempp_amd.core import hip_backend as hb
+
+        dev = hb.device()
+        base = hb.device_space(space)
+        self.blocks = blocks
+        self.elements = torch.from_numpy(blocks.order).to(dev)
+        self.start = torch.from_numpy(blocks.start).to(dev)
+        self.dofs = torch.from_numpy(blocks.dofs).to(dev)
+        self.ndofs = torch.from_numpy(blocks.ndofs).to(dev)
+        self.slots = torch.from_numpy(blocks.slots).to(dev)
+        self.base = base
+        self.space_struct = hb.BemSpace(
+            int(len(blocks.order)), base.shapeset, self.elements.data_ptr(), base.grid.geometry.data_ptr(),
+            base.grid.vertex_ids.data_ptr(), base.local2global.data_ptr(), base.multipliers.data_ptr(),
+            base.normal_multipliers.data_ptr(),
+        )
+        self.struct = hb.BemDofBlocks(
+            blocks.n_blocks, blocks.max_dofs, self.start.data_ptr(), self.dofs.data_ptr(), self.ndofs.data_ptr(),
+            self.slots.data_ptr(),
+        )
+
+
+_DEVICE_CACHE = {}
+
+
+def device_blocks(space, role, out_itemsize):
+    """Device blocking of a space as test ('test') or trial ('trial') side."""
+    if role == "test":
+        me, md = TEST_TILE_ELEMENTS, TEST_TILE_DOFS
+    else:
+        me, md = TRIAL_BLOCK_ELEMENTS, TRIAL_BLOCK_DOFS[out_itemsize]
+    key = (space.id, me, md)
+    if key not in _DEVICE_CACHE:
+        _DEVICE_CACHE[key] = DeviceBlocks(space, make_blocks(space, me, md))
+    return _DEVICE_CACHE[key]
+
+
+def clear_cache():
+    _CACHE.clear()
+    _DEVICE_CACHE.clear()

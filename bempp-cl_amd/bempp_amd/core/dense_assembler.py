@@ -76,8 +76,15 @@ def _is_localised_injective(space):
     return space.is_localised
 
 
-def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, out=None, singular=True):
-    """Assemble the dense matrix into a new (or the given) device tensor and return it."""
+def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, out=None, singular=True,
+                       accumulate=True, events=None):
+    """Assemble the dense matrix into a new (or the given) device tensor and return it.
+
+    out: optional (rows, cols) device tensor; the operator is added to it when accumulate is
+    true, otherwise it is overwritten. events: optional dict of torch.cuda.Event objects with
+    keys "regular_start", "regular_end", "singular_end", recorded around the launches on the
+    current stream (used for kernel timing).
+    """
     import torch
     from bempp_amd.api.integration.triangle_gauss import rule as triangle_rule
     from bempp_amd.core import hip_backend as hb
@@ -108,25 +115,50 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
 
     if out is None:
         out = torch.empty((rows, cols), dtype=tdtype, device=dev)
-        if not full_cover:
-            hb.fill_zero(out)
-        accumulate = 0
+        accumulate = False
     else:
-        accumulate = 1
+        if tuple(out.shape) != (rows, cols) or out.dtype != tdtype or not out.is_contiguous():
+            raise ValueError(f"out must be a contiguous ({rows}, {cols}) {dtype} tensor")
+    if not accumulate and not full_cover:
+        hb.fill_zero(out)
     hip_opts = getattr(parameters, "hip", None)
     near = getattr(hip_opts, "mixed_near_field_factor", 2.0) if hip_opts is not None else 2.0
     kahan = getattr(hip_opts, "kahan", True) if hip_opts is not None else True
-
-    hb.regular_assemble(
-        op, test, trial, grids_identical, rule, _wavenumber(operator_descriptor), mode, out,
-        scatter=0 if direct else 1, accumulate=accumulate if direct else 0, near_factor=near, kahan=kahan,
-    )
+    srule = None
     if grids_identical and singular:
         srule = singular_rule(dual_to_range.grid, parameters.quadrature.singular, dual_to_range.support, domain.support)
+
+    if events is not None:
+        events["regular_start"].record()
+    if direct:
+        hb.regular_assemble(
+            op, test, trial, grids_identical, rule, _wavenumber(operator_descriptor), mode, out,
+            accumulate=int(accumulate), near_factor=near, kahan=kahan,
+        )
+    else:
+        from bempp_amd.core.blocking import device_blocks
+
+        itemsize = out.element_size() * (1 if out.is_complex() else 2)
+        hb.regular_assemble_blocked(
+            op, device_blocks(dual_to_range, "test", itemsize), device_blocks(domain, "trial", itemsize),
+            grids_identical, rule, _wavenumber(operator_descriptor), mode, out, near_factor=near, kahan=kahan,
+        )
+    if events is not None:
+        events["regular_end"].record()
+    if mode == hb.PRECISION_MIXED:
+        from bempp_amd.core.near_field import device_near_candidates
+
+        cand = device_near_candidates(dual_to_range, domain, near)
+        hb.near_assemble(op, test, trial, cand, grids_identical, rule, _wavenumber(operator_descriptor), near, out)
+    if events is not None and "near_end" in events:
+        events["near_end"].record()
+    if srule is not None:
         hb.singular_assemble(
             op, test, trial, srule.pairs, srule.test_points, srule.trial_points, srule.weights,
             _wavenumber(operator_descriptor), out,
         )
+    if events is not None:
+        events["singular_end"].record()
     return out
 
 

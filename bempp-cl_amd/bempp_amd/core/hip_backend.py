@@ -46,6 +46,17 @@ class BemSpace(ctypes.Structure):
     ]
 
 
+class BemDofBlocks(ctypes.Structure):
+    _fields_ = [
+        ("n_blocks", ctypes.c_int32),
+        ("max_dofs", ctypes.c_int32),
+        ("start", ctypes.c_void_p),
+        ("dofs", ctypes.c_void_p),
+        ("ndofs", ctypes.c_void_p),
+        ("slots", ctypes.c_void_p),
+    ]
+
+
 class BemQuadRule(ctypes.Structure):
     _fields_ = [
         ("n_points", ctypes.c_int32),
@@ -75,8 +86,13 @@ def lib():
         sz = ctypes.c_size_t
         SP = ctypes.POINTER(BemSpace)
         QP = ctypes.POINTER(BemQuadRule)
-        L.bem_regular_assemble.argtypes = [i, SP, SP, i, QP, d, d, i, d, i, P, i64, i, i, i, P]
+        BP = ctypes.POINTER(BemDofBlocks)
+        L.bem_regular_assemble.argtypes = [i, SP, SP, i, QP, d, d, i, d, i, P, i64, i, i, P]
         L.bem_regular_assemble.restype = i
+        L.bem_regular_assemble_blocked.argtypes = [i, SP, SP, BP, BP, i, QP, d, d, i, d, i, P, i64, i, P]
+        L.bem_regular_assemble_blocked.restype = i
+        L.bem_near_assemble.argtypes = [i, SP, SP, i, P, i, QP, d, d, d, P, i64, i, P]
+        L.bem_near_assemble.restype = i
         L.bem_singular_assemble.argtypes = [i, SP, SP, i, P, P, P, P, d, d, P, i64, i, P]
         L.bem_singular_assemble.restype = i
         L.bem_potential_workspace_size.argtypes = [i, SP, QP, i, i]
@@ -217,15 +233,45 @@ def fill_zero(tensor):
     check(lib().bem_fill_zero(ptr(tensor), tensor.numel() * tensor.element_size(), stream_handle()))
 
 
-def regular_assemble(op, test, trial, grids_identical, rule, wavenumber, precision, out, scatter, accumulate=0,
+def _dtype_code(t):
+    return DTYPES[str(t.dtype).replace("torch.", "")]
+
+
+def regular_assemble(op, test, trial, grids_identical, rule, wavenumber, precision, out, accumulate=0,
                      near_factor=2.0, kahan=True):
+    """Regular pairs of localised spaces (DP0/DP1): direct stores into out."""
     k = complex(wavenumber)
     check(
         lib().bem_regular_assemble(
             OPERATORS[op], ctypes.byref(test.struct), ctypes.byref(trial.struct), int(grids_identical),
             ctypes.byref(rule), k.real, k.imag, precision, float(near_factor), int(bool(kahan)), ptr(out),
-            out.shape[-1], DTYPES[str(out.dtype).replace("torch.", "")], int(scatter), int(accumulate),
-            stream_handle(),
+            out.shape[-1], _dtype_code(out), int(accumulate), stream_handle(),
+        )
+    )
+
+
+def regular_assemble_blocked(op, test_blocks, trial_blocks, grids_identical, rule, wavenumber, precision, out,
+                             near_factor=2.0, kahan=True):
+    """Regular pairs of spaces with shared dofs (P1, RWG/SNC): LDS-tiled sums added to out."""
+    k = complex(wavenumber)
+    check(
+        lib().bem_regular_assemble_blocked(
+            OPERATORS[op], ctypes.byref(test_blocks.space_struct), ctypes.byref(trial_blocks.space_struct),
+            ctypes.byref(test_blocks.struct), ctypes.byref(trial_blocks.struct), int(grids_identical),
+            ctypes.byref(rule), k.real, k.imag, precision, float(near_factor), int(bool(kahan)), ptr(out),
+            out.shape[-1], _dtype_code(out), stream_handle(),
+        )
+    )
+
+
+def near_assemble(op, test, trial, pairs, grids_identical, rule, wavenumber, near_factor, out):
+    """FP64 near-field pairs of the mixed mode (pairs: (n, 2) int32 device tensor of candidates)."""
+    k = complex(wavenumber)
+    check(
+        lib().bem_near_assemble(
+            OPERATORS[op], ctypes.byref(test.struct), ctypes.byref(trial.struct), int(pairs.shape[0]), ptr(pairs),
+            int(grids_identical), ctypes.byref(rule), k.real, k.imag, float(near_factor), ptr(out), out.shape[-1],
+            _dtype_code(out), stream_handle(),
         )
     )
 

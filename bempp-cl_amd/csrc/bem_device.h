@@ -108,6 +108,25 @@ struct C2d {
     double re, im;
 };
 
+// Element centroid v0 + (e1 + e2) / 3 from a geometry row, with explicitly rounded
+// operations so that every kernel evaluating it gets bit-identical results.
+__device__ __forceinline__ void centroid_rn(const double *g, double *c) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) c[d] = __dadd_rn(g[d], __dmul_rn(__dadd_rn(g[3 + d], g[6 + d]), 1.0 / 3.0));
+}
+
+// Near-field test of the mixed-precision mode: |c_t - c_f| < factor * (diam_t + diam_f) / 2.
+// The FP32 regular kernel skips exactly the pairs for which this returns true and the FP64
+// near-field kernel computes exactly those, so it is evaluated without contraction.
+__device__ __forceinline__ bool pair_is_near(const double *ct, const double *cf, double diam_t, double diam_f,
+                                             double factor, double *dist2_out) {
+    double d0 = __dadd_rn(cf[0], -ct[0]), d1 = __dadd_rn(cf[1], -ct[1]), d2 = __dadd_rn(cf[2], -ct[2]);
+    double dist2 = __dadd_rn(__dadd_rn(__dmul_rn(d0, d0), __dmul_rn(d1, d1)), __dmul_rn(d2, d2));
+    double lim = __dmul_rn(__dmul_rn(factor, 0.5), __dadd_rn(diam_t, diam_f));
+    *dist2_out = dist2;
+    return dist2 < __dmul_rn(lim, lim);
+}
+
 // ------------------------------------------------------------------ epilogue
 // Local block value (i, j) from the moment block S, for scalar shapesets or EFIE.
 template <int OP, int NTS, int NSS>

@@ -1,0 +1,650 @@
+// Regular-pair dense Galerkin assembly for gfx950.
+//
+// Computes, for every (test element e, trial element f) pair that shares no vertex (or
+// every pair when the grids differ),
+//   A[g_e(i), g_f(j)] += m_e(i) m_f(j) sum_p sum_q w_p J_e w_q J_f phi_i(xi_p) psi_j(eta_q) G(x_p, y_q)
+// (bempp_cl/core/numba_kernels.py:980-1074), and the Maxwell EFIE block
+//   sum_pq G w w J J [ -ik phi_i.psi_j - 4 / (ik J_e J_f) ] l_i l_j m_i m_j
+// (numba_kernels.py:1853-1969).
+//
+// Work decomposition: a 256-thread workgroup owns a tile of up to TM test elements and a
+// block of up to 256 trial elements. Each lane keeps one trial element (quadrature point
+// offsets, weights, normal) in registers and walks the test elements staged in LDS, so
+// every LDS read is a broadcast.
+//
+// Per pair the kernel forms the 3x3 moment block
+//   S[a][b] = sum_pq G'_pq u_a(xi_p) u_b(eta_q),  u = (1, xi_0, xi_1),  G' = G w_p J_e w_q J_f
+// from which every local block follows (bem_device.h: local_value).
+//
+// Output scatter:
+//  * direct (both spaces localised, DP0/DP1): every pair owns its entries; lanes hold
+//    consecutive trial elements, so a DP0 row segment is one coalesced store.
+//  * tile (P1, RWG/SNC: dofs shared between elements): the host splits the (locality-
+//    ordered) element lists into test tiles and trial blocks and gives every block its
+//    list of distinct dofs and every (element, shape function) its slot in that list.
+//    The workgroup sums its contributions in an LDS tile [test slots x trial slots] with
+//    LDS atomics, then adds the tile to HBM once per entry. This removes the ~20-fold
+//    repetition of global atomics on the same entry (each P1 dof pair is touched by up to
+//    36 element pairs).
+//
+// MIXED precision: pairs with centroid distance >= factor * mean diameter run in FP32.
+// Per pair, the centroid offset d = c_f - c_e and the reference distance r0 = |d| (rounded
+// to FP32) are formed in FP64, together with the phase k r0 reduced mod 2 pi. Per point
+// pair only the small difference r - r0 = t / (r + r0) with t = |d + b_q - a_p|^2 - r0^2
+// (a, b: offsets of the quadrature points from the centroids) is needed in FP32, so the
+// FP32 phase error is ~k * eps * h instead of k * eps * r and the hardware v_sin/v_cos
+// (argument in revolutions) see arguments of size k h. The pair's moments are multiplied
+// by e^{ik r0} once. Sums over test points use Kahan compensation. Near pairs use FP64.
+#pragma once
+
+#include "bem_device.h"
+#include "bem_internal.h"
+
+namespace bem {
+
+constexpr int kRegThreads = 256;
+constexpr int kMaxQ = 16;
+constexpr double kInv2Pi = 0.15915494309189533577;
+
+struct BlockArgs {
+    const int32_t *start;  // [nblocks + 1] offsets into the element list (NULL: fixed blocks)
+    const int32_t *dofs;   // [nblocks * maxu] distinct global dofs per block
+    const int32_t *ndofs;  // [nblocks]
+    const int32_t *slots;  // [n_list * 3] slot of (list position, shape function) in its block
+    int nblocks;
+    int maxu;
+};
+
+struct RegularArgs {
+    SpaceArgs test, trial;
+    BlockArgs tb, sb;
+    RuleArgs rule;
+    double kr, ki;
+    double near_factor;
+    void *out;
+    int64_t ld;
+    int identical;
+    int accumulate;
+};
+
+template <int TM>
+struct TestTile {
+    int elem[TM];
+    int vid[TM][3];
+    int slot[TM][3];
+    double mult[TM][3];
+    double c[TM][3];
+    double diam[TM];
+    double xp[TM][kMaxQ][3];  // absolute test points (FP64 paths)
+    float a[TM][kMaxQ][3];    // offsets from the centroid (FP32 path)
+    float aa[TM][kMaxQ];      // |a|^2
+    float wp[TM][kMaxQ];      // w_p J_e
+    float nx[TM][3];          // test normal (multiplied)
+};
+
+// ------------------------------------------------------------------ FP64 pair
+template <int OP, int NTS, int NSS, int QT>
+__device__ __forceinline__ void pair_fp64_geom(const double *__restrict__ gt, const double *__restrict__ gf, double tnm,
+                                               double snm, const RuleArgs &rule, double kr, double ki,
+                                               C2d (&S)[NTS][NSS]) {
+    const int nq = QT > 0 ? QT : rule.n;
+    double nx[3] = {gt[9] * tnm, gt[10] * tnm, gt[11] * tnm};
+    double ny[3] = {gf[9] * snm, gf[10] * snm, gf[11] * snm};
+    double jt = gt[12], jf = gf[12];
+#pragma unroll
+    for (int a = 0; a < NTS; ++a)
+#pragma unroll
+        for (int b = 0; b < NSS; ++b) S[a][b].re = S[a][b].im = 0.0;
+#pragma unroll 1
+    for (int p = 0; p < nq; ++p) {
+        double xi = rule.x[p], eta = rule.y[p];
+        double x0 = gt[0] + gt[3] * xi + gt[6] * eta;
+        double x1 = gt[1] + gt[4] * xi + gt[7] * eta;
+        double x2 = gt[2] + gt[5] * xi + gt[8] * eta;
+        C2d T[NSS];
+#pragma unroll
+        for (int b = 0; b < NSS; ++b) T[b].re = T[b].im = 0.0;
+#pragma unroll(QT > 0 ? QT : 1)
+        for (int q = 0; q < nq; ++q) {
+            double s0 = rule.x[q], s1 = rule.y[q];
+            double dx = gf[0] + gf[3] * s0 + gf[6] * s1 - x0;
+            double dy = gf[1] + gf[4] * s0 + gf[7] * s1 - x1;
+            double dz = gf[2] + gf[5] * s0 + gf[8] * s1 - x2;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            double gre, gim;
+            green<OP, double>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim);
+            double wq = rule.w[q] * jf;
+            gre *= wq;
+            gim *= wq;
+            T[0].re += gre;
+            T[0].im += gim;
+            if constexpr (NSS == 3) {
+                T[1].re += gre * s0;
+                T[1].im += gim * s0;
+                T[2].re += gre * s1;
+                T[2].im += gim * s1;
+            }
+        }
+        double wp = rule.w[p] * jt;
+        double u[3] = {wp, wp * xi, wp * eta};
+#pragma unroll
+        for (int a = 0; a < NTS; ++a)
+#pragma unroll
+            for (int b = 0; b < NSS; ++b) {
+                S[a][b].re += u[a] * T[b].re;
+                S[a][b].im += u[a] * T[b].im;
+            }
+    }
+}
+
+// FP64 pair with test points from LDS and trial points from registers
+template <int OP, int NTS, int NSS, int QT, int TM>
+__device__ __forceinline__ void pair_fp64_regs(const TestTile<TM> &T_, int t, const double *gt, double tnm,
+                                               const double (&yq)[QT > 0 ? QT : 1][3], const double (&wq)[QT > 0 ? QT : 1],
+                                               const double *ny, const RuleArgs &rule, double kr, double ki,
+                                               C2d (&S)[NTS][NSS]) {
+    double nx[3] = {gt[9] * tnm, gt[10] * tnm, gt[11] * tnm};
+    double jt = gt[12];
+#pragma unroll
+    for (int a = 0; a < NTS; ++a)
+#pragma unroll
+        for (int b = 0; b < NSS; ++b) S[a][b].re = S[a][b].im = 0.0;
+#pragma unroll 1
+    for (int p = 0; p < QT; ++p) {
+        double x0 = T_.xp[t][p][0], x1 = T_.xp[t][p][1], x2 = T_.xp[t][p][2];
+        C2d T[NSS];
+#pragma unroll
+        for (int b = 0; b < NSS; ++b) T[b].re = T[b].im = 0.0;
+#pragma unroll
+        for (int q = 0; q < QT; ++q) {
+            double dx = yq[q][0] - x0, dy = yq[q][1] - x1, dz = yq[q][2] - x2;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            double gre, gim;
+            green<OP, double>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim);
+            gre *= wq[q];
+            gim *= wq[q];
+            T[0].re += gre;
+            T[0].im += gim;
+            if constexpr (NSS == 3) {
+                double s0 = rule.x[q], s1 = rule.y[q];
+                T[1].re += gre * s0;
+                T[1].im += gim * s0;
+                T[2].re += gre * s1;
+                T[2].im += gim * s1;
+            }
+        }
+        double wp = rule.w[p] * jt;
+        double u[3] = {wp, wp * rule.x[p], wp * rule.y[p]};
+#pragma unroll
+        for (int a = 0; a < NTS; ++a)
+#pragma unroll
+            for (int b = 0; b < NSS; ++b) {
+                S[a][b].re += u[a] * T[b].re;
+                S[a][b].im += u[a] * T[b].im;
+            }
+    }
+}
+
+// ------------------------------------------------------------------ FP32 far pair
+// Trial element state of one lane (FP32 path)
+template <int QT>
+struct TrialF32 {
+    static constexpr int Q = QT > 0 ? QT : kMaxQ;
+    float b[Q][3];  // offsets of the quadrature points from the centroid
+    float bb[Q];    // |b|^2
+    float wq[Q];    // w_q J_f / (4 pi)
+    float bn[Q];    // b . n_y
+    float ny[3];
+};
+
+__device__ __forceinline__ float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
+__device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
+
+template <int OP, int NTS, int NSS, int QT, int TM, bool KAHAN>
+__device__ __forceinline__ void pair_fp32(const TestTile<TM> &T_, int t, const TrialF32<QT> &F, double d0, double d1,
+                                          double d2, double dist2, const RuleArgs &rule, double kr, double ki,
+                                          C2d (&S)[NTS][NSS]) {
+    constexpr bool HELM = OP >= kHelmholtzSL;
+    const int nq = QT > 0 ? QT : rule.n;
+    const float fd0 = (float)d0, fd1 = (float)d1, fd2 = (float)d2;
+    // reference distance r0 (FP32 value) and the FP64 residual |d|^2 - r0^2
+    const float r0 = sqrtf((float)dist2);
+    const float r0sq = r0 * r0;
+    const float resid = (float)(dist2 - (double)r0 * (double)r0);
+    const float kr_rev = (float)(kr * kInv2Pi);
+    const float kr_f = (float)kr, ki_f = (float)ki;
+    const float ki_log2e = (float)(ki * 1.4426950408889634);
+
+    // test normal projections of the trial offsets (ADL)
+    float bnx[TrialF32<QT>::Q];
+    if constexpr (OP == kLaplaceADL || OP == kHelmholtzADL) {
+#pragma unroll(QT > 0 ? QT : 1)
+        for (int q = 0; q < nq; ++q)
+            bnx[q] = F.b[q][0] * T_.nx[t][0] + F.b[q][1] * T_.nx[t][1] + F.b[q][2] * T_.nx[t][2];
+    }
+
+    Acc<float, KAHAN> Sr[NTS][NSS], Si[NTS][NSS];
+#pragma unroll
+    for (int i = 0; i < NTS; ++i)
+#pragma unroll
+        for (int j = 0; j < NSS; ++j) {
+            Sr[i][j].zero();
+            Si[i][j].zero();
+        }
+    // the test-point loop stays rolled: unrolling it lets the scheduler keep all 36
+    // evaluations in flight, which exhausts the VGPR file and drops occupancy to 1 wave
+#pragma unroll 1
+    for (int p = 0; p < nq; ++p) {
+        // w = d - a_p: y - x = w + b_q
+        const float w0 = fd0 - T_.a[t][p][0], w1 = fd1 - T_.a[t][p][1], w2 = fd2 - T_.a[t][p][2];
+        // |w|^2 - r0^2 = resid - 2 d.a_p + |a_p|^2
+        const float Ap = resid - 2.f * (fd0 * T_.a[t][p][0] + fd1 * T_.a[t][p][1] + fd2 * T_.a[t][p][2]) + T_.aa[t][p];
+        const float tw0 = 2.f * w0, tw1 = 2.f * w1, tw2 = 2.f * w2;
+        float wn = 0.f, wnx = 0.f;
+        if constexpr (OP == kLaplaceDL || OP == kHelmholtzDL) wn = w0 * F.ny[0] + w1 * F.ny[1] + w2 * F.ny[2];
+        if constexpr (OP == kLaplaceADL || OP == kHelmholtzADL)
+            wnx = -(w0 * T_.nx[t][0] + w1 * T_.nx[t][1] + w2 * T_.nx[t][2]);
+        float Tr[NSS], Ti[NSS];
+#pragma unroll
+        for (int b = 0; b < NSS; ++b) Tr[b] = Ti[b] = 0.f;
+#pragma unroll(QT > 0 ? QT : 1)
+        for (int q = 0; q < nq; ++q) {
+            // t = |y - x|^2 - r0^2
+            const float tt = Ap + F.bb[q] + tw0 * F.b[q][0] + tw1 * F.b[q][1] + tw2 * F.b[q][2];
+            const float r2 = r0sq + tt;
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            float gre, gim;
+            if constexpr (!HELM) {
+                float g = rinv * F.wq[q];
+                if constexpr (OP == kLaplaceSL) {
+                    gre = g;
+                } else if constexpr (OP == kLaplaceDL) {
+                    gre = -(wn + F.bn[q]) * g * rinv * rinv;
+                } else {  // (y - x).n_x = bnx - wnx
+                    gre = (bnx[q] - wnx) * g * rinv * rinv;
+                }
+                gim = 0.f;
+            } else {
+                const float r = r2 * rinv;
+                const float delta = tt * __builtin_amdgcn_rcpf(r + r0);
+                const float ph = kr_rev * delta;
+                const float s = sin_rev(ph), c = cos_rev(ph);
+                float g = rinv * F.wq[q];
+                if (ki_f != 0.f) g *= __builtin_amdgcn_exp2f(-ki_log2e * delta);
+                if constexpr (OP == kHelmholtzSL || OP == kMaxwellEFIE) {
+                    gre = c * g;
+                    gim = s * g;
+                } else {
+                    float dot;
+                    if constexpr (OP == kHelmholtzDL) dot = wn + F.bn[q];
+                    else dot = wnx - bnx[q];
+                    g *= dot * rinv * rinv;
+                    const float ar = -1.f - ki_f * r, ai = kr_f * r;
+                    gre = (ar * c - ai * s) * g;
+                    gim = (ai * c + ar * s) * g;
+                }
+            }
+            Tr[0] += gre;
+            if constexpr (HELM) Ti[0] += gim;
+            if constexpr (NSS == 3) {
+                const float s0 = (float)rule.x[q], s1 = (float)rule.y[q];
+                Tr[1] += gre * s0;
+                Tr[2] += gre * s1;
+                if constexpr (HELM) {
+                    Ti[1] += gim * s0;
+                    Ti[2] += gim * s1;
+                }
+            }
+        }
+        const float wp = T_.wp[t][p];
+        const float u[3] = {wp, wp * (float)rule.x[p], wp * (float)rule.y[p]};
+#pragma unroll
+        for (int i = 0; i < NTS; ++i)
+#pragma unroll
+            for (int j = 0; j < NSS; ++j) {
+                Sr[i][j].add(u[i] * Tr[j]);
+                if constexpr (HELM) Si[i][j].add(u[i] * Ti[j]);
+            }
+    }
+    // multiply by e^{i k r0} e^{-ki r0} (FP64 phase reduced to one revolution)
+    double pr = 1.0, pi = 0.0;
+    if constexpr (HELM) {
+        double rev = kr * kInv2Pi * (double)r0;
+        rev -= rint(rev);
+        float sf = sin_rev((float)rev), cf = cos_rev((float)rev);
+        double damp = ki != 0.0 ? exp(-ki * (double)r0) : 1.0;
+        pr = (double)cf * damp;
+        pi = (double)sf * damp;
+    }
+#pragma unroll
+    for (int i = 0; i < NTS; ++i)
+#pragma unroll
+        for (int j = 0; j < NSS; ++j) {
+            double sr = (double)Sr[i][j].s - (double)Sr[i][j].c;
+            double si = HELM ? (double)Si[i][j].s - (double)Si[i][j].c : 0.0;
+            S[i][j].re = pr * sr - pi * si;
+            S[i][j].im = pr * si + pi * sr;
+        }
+}
+
+template <typename OutT, bool CPLX>
+__device__ __forceinline__ void store_direct(void *out, int64_t ld, int row, int col, C2d v, int accumulate) {
+    OutT *o = reinterpret_cast<OutT *>(out);
+    int64_t idx = (int64_t)row * ld + col;
+    if constexpr (CPLX) {
+        using V2 = typename std::conditional<sizeof(OutT) == 8, double2, float2>::type;
+        V2 *p = reinterpret_cast<V2 *>(o + 2 * idx);
+        V2 val;
+        if (accumulate) {
+            val = *p;
+            val.x += (OutT)v.re;
+            val.y += (OutT)v.im;
+        } else {
+            val.x = (OutT)v.re;
+            val.y = (OutT)v.im;
+        }
+        *p = val;
+    } else {
+        OutT *p = o + idx;
+        if (accumulate) p[0] += (OutT)v.re;
+        else p[0] = (OutT)v.re;
+    }
+}
+
+// ------------------------------------------------------------------ kernel
+template <int OP, int NTS, int NSS, int QT, int TM, bool MIXED, bool KAHAN, bool TILE, int MAXU_T, int MAXU_S,
+          typename OutT>
+__global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs a) {
+    constexpr bool CPLX = OpTraits<OP>::is_complex;
+    constexpr int NT = NTS, NS = NSS;
+    using AccT = OutT;  // LDS accumulation type: float in MIXED (complex64), double in FP64 mode
+    const int nq = QT > 0 ? QT : a.rule.n;
+
+    __shared__ TestTile<TM> T_;
+    __shared__ AccT acc[TILE ? MAXU_T * MAXU_S * (CPLX ? 2 : 1) : 1];
+
+    const int tid = threadIdx.x;
+    int t0, t1, s0, s1;
+    if constexpr (TILE) {
+        t0 = a.tb.start[blockIdx.y];
+        t1 = a.tb.start[blockIdx.y + 1];
+        s0 = a.sb.start[blockIdx.x];
+        s1 = a.sb.start[blockIdx.x + 1];
+    } else {
+        t0 = blockIdx.y * TM;
+        t1 = min(t0 + TM, a.test.n);
+        s0 = blockIdx.x * kRegThreads;
+        s1 = min(s0 + kRegThreads, a.trial.n);
+    }
+    // guards against malformed block descriptions (the host never produces them)
+    s1 = min(s1, s0 + kRegThreads);
+    const int jf = s0 + tid;
+    const bool valid = jf < s1;
+    const int ntile = max(0, min(t1 - t0, TM));
+
+    if constexpr (TILE) {
+        for (int i = tid; i < MAXU_T * MAXU_S * (CPLX ? 2 : 1); i += kRegThreads) acc[i] = (AccT)0;
+    }
+    if (tid < ntile) {
+        const int pos = t0 + tid;
+        const int e = a.test.elements[pos];
+        const double *g = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
+        T_.elem[tid] = e;
+        for (int i = 0; i < 3; ++i) T_.vid[tid][i] = a.test.vids[4 * e + i];
+        for (int i = 0; i < NT; ++i) {
+            T_.mult[tid][i] = a.test.mult[(int64_t)e * NT + i];
+            if constexpr (TILE) T_.slot[tid][i] = min(max(a.tb.slots[3 * pos + i], 0), MAXU_T - 1);
+        }
+        centroid_rn(g, T_.c[tid]);
+        T_.diam[tid] = g[13];
+        double nm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
+        T_.nx[tid][0] = (float)(g[9] * nm);
+        T_.nx[tid][1] = (float)(g[10] * nm);
+        T_.nx[tid][2] = (float)(g[11] * nm);
+        for (int p = 0; p < nq; ++p) {
+            double xi = a.rule.x[p], eta = a.rule.y[p];
+            for (int d = 0; d < 3; ++d) {
+                double off = g[3 + d] * (xi - 1.0 / 3.0) + g[6 + d] * (eta - 1.0 / 3.0);
+                T_.xp[tid][p][d] = g[d] + g[3 + d] * xi + g[6 + d] * eta;
+                T_.a[tid][p][d] = (float)off;
+            }
+            T_.aa[tid][p] = T_.a[tid][p][0] * T_.a[tid][p][0] + T_.a[tid][p][1] * T_.a[tid][p][1] +
+                            T_.a[tid][p][2] * T_.a[tid][p][2];
+            T_.wp[tid][p] = (float)(a.rule.w[p] * g[12]);
+        }
+    }
+    __syncthreads();
+
+    // trial element of this lane
+    int f = 0;
+    int fv0 = -1, fv1 = -1, fv2 = -1;
+    double snm = 1.0, cf0 = 0, cf1 = 0, cf2 = 0, diam_f = 0;
+    const double *gf = a.trial.geom;
+    double mf[NS];
+    int slot_f[NS];
+    TrialF32<QT> F32;
+    double yq[QT > 0 ? QT : 1][3], wq64[QT > 0 ? QT : 1], ny64[3] = {0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        mf[j] = 0.0;
+        slot_f[j] = 0;
+    }
+    if (valid) {
+        f = a.trial.elements[jf];
+        gf = a.trial.geom + (int64_t)f * BEM_GEOM_STRIDE;
+        fv0 = a.trial.vids[4 * f];
+        fv1 = a.trial.vids[4 * f + 1];
+        fv2 = a.trial.vids[4 * f + 2];
+        snm = a.trial.nmult ? (double)a.trial.nmult[f] : 1.0;
+        double cfv[3];
+        centroid_rn(gf, cfv);
+        cf0 = cfv[0];
+        cf1 = cfv[1];
+        cf2 = cfv[2];
+        diam_f = gf[13];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            mf[j] = a.trial.mult[(int64_t)f * NS + j];
+            if constexpr (TILE) slot_f[j] = min(max(a.sb.slots[3 * jf + j], 0), MAXU_S - 1);
+        }
+        ny64[0] = gf[9] * snm;
+        ny64[1] = gf[10] * snm;
+        ny64[2] = gf[11] * snm;
+        if constexpr (MIXED) {
+            F32.ny[0] = (float)ny64[0];
+            F32.ny[1] = (float)ny64[1];
+            F32.ny[2] = (float)ny64[2];
+#pragma unroll(QT > 0 ? QT : 1)
+            for (int q = 0; q < nq; ++q) {
+                double s0q = a.rule.x[q] - 1.0 / 3.0, s1q = a.rule.y[q] - 1.0 / 3.0;
+                float b0 = (float)(gf[3] * s0q + gf[6] * s1q);
+                float b1 = (float)(gf[4] * s0q + gf[7] * s1q);
+                float b2 = (float)(gf[5] * s0q + gf[8] * s1q);
+                F32.b[q][0] = b0;
+                F32.b[q][1] = b1;
+                F32.b[q][2] = b2;
+                F32.bb[q] = b0 * b0 + b1 * b1 + b2 * b2;
+                F32.wq[q] = (float)(a.rule.w[q] * gf[12] * kInv4Pi);
+                F32.bn[q] = b0 * F32.ny[0] + b1 * F32.ny[1] + b2 * F32.ny[2];
+            }
+        } else if constexpr (QT > 0) {
+#pragma unroll
+            for (int q = 0; q < QT; ++q) {
+                double s0q = a.rule.x[q], s1q = a.rule.y[q];
+                for (int d = 0; d < 3; ++d) yq[q][d] = gf[d] + gf[3 + d] * s0q + gf[6 + d] * s1q;
+                wq64[q] = a.rule.w[q] * gf[12];
+            }
+        }
+    }
+
+    for (int t = 0; t < ntile; ++t) {
+        const int e = T_.elem[t];
+        const double *gt = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
+        bool adj = false;
+        if (a.identical) {
+            int v0 = T_.vid[t][0], v1 = T_.vid[t][1], v2 = T_.vid[t][2];
+            adj = v0 == fv0 || v0 == fv1 || v0 == fv2 || v1 == fv0 || v1 == fv1 || v1 == fv2 || v2 == fv0 ||
+                  v2 == fv1 || v2 == fv2;
+        }
+        C2d S[NTS][NSS];
+#pragma unroll
+        for (int i = 0; i < NTS; ++i)
+#pragma unroll
+            for (int j = 0; j < NSS; ++j) S[i][j].re = S[i][j].im = 0.0;
+
+        // MIXED: near pairs are left to the FP64 near-field kernel (bem_near_assemble)
+        bool skip = adj;
+        if (valid && !adj) {
+            if constexpr (!MIXED) {
+                double tnm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
+                if constexpr (QT > 0)
+                    pair_fp64_regs<OP, NTS, NSS, QT, TM>(T_, t, gt, tnm, yq, wq64, ny64, a.rule, a.kr, a.ki, S);
+                else
+                    pair_fp64_geom<OP, NTS, NSS, QT>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S);
+            } else {
+                const double cfv[3] = {cf0, cf1, cf2};
+                double dist2;
+                skip = pair_is_near(T_.c[t], cfv, T_.diam[t], diam_f, a.near_factor, &dist2);
+                if (!skip) {
+                    double d0 = cf0 - T_.c[t][0], d1 = cf1 - T_.c[t][1], d2 = cf2 - T_.c[t][2];
+                    pair_fp32<OP, NTS, NSS, QT, TM, KAHAN>(T_, t, F32, d0, d1, d2, dist2, a.rule, a.kr, a.ki, S);
+                }
+            }
+        }
+
+        if (valid) {
+#pragma unroll
+            for (int i = 0; i < NT; ++i) {
+                const double mi = T_.mult[t][i];
+#pragma unroll
+                for (int j = 0; j < NS; ++j) {
+                    const double m = mi * mf[j];
+                    if constexpr (TILE) {
+                        if (m != 0.0 && !skip) {
+                            C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
+                            AccT *slot = acc + (CPLX ? 2 : 1) * (T_.slot[t][i] * MAXU_S + slot_f[j]);
+                            unsafeAtomicAdd(slot, (AccT)(v.re * m));
+                            if constexpr (CPLX) unsafeAtomicAdd(slot + 1, (AccT)(v.im * m));
+                        }
+                    } else {
+                        C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
+                        v.re *= m;
+                        v.im *= m;
+                        int row = a.test.l2g[(int64_t)e * NT + i];
+                        int col = a.trial.l2g[(int64_t)f * NS + j];
+                        store_direct<OutT, CPLX>(a.out, a.ld, row, col, v, a.accumulate);
+                    }
+                }
+            }
+        }
+    }
+
+    if constexpr (TILE) {
+        __syncthreads();
+        const int ndt = min(a.tb.ndofs[blockIdx.y], MAXU_T), nds = min(a.sb.ndofs[blockIdx.x], MAXU_S);
+        const int32_t *dt = a.tb.dofs + (int64_t)blockIdx.y * a.tb.maxu;
+        const int32_t *ds = a.sb.dofs + (int64_t)blockIdx.x * a.sb.maxu;
+        OutT *o = reinterpret_cast<OutT *>(a.out);
+        for (int idx = tid; idx < ndt * nds; idx += kRegThreads) {
+            const int ia = idx / nds, is = idx - ia * nds;
+            const AccT *src = acc + (CPLX ? 2 : 1) * (ia * MAXU_S + is);
+            const int64_t pos = (int64_t)dt[ia] * a.ld + ds[is];
+            if constexpr (CPLX) {
+                if (src[0] != (AccT)0) unsafeAtomicAdd(o + 2 * pos, src[0]);
+                if (src[1] != (AccT)0) unsafeAtomicAdd(o + 2 * pos + 1, src[1]);
+            } else {
+                if (src[0] != (AccT)0) unsafeAtomicAdd(o + pos, src[0]);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ dispatch
+template <int OP, int NTS, int NSS, int QT, bool MIXED, typename OutT>
+static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
+    constexpr int TM = 16;
+    constexpr bool CPLX = OpTraits<OP>::is_complex;
+    // LDS tile of distinct dofs: test slots x trial slots
+    constexpr int MT = 24;
+    constexpr int MS = (CPLX && sizeof(OutT) == 8) ? 160 : 192;
+    dim3 grid;
+    if (tile) {
+        if (a.tb.maxu > MT || a.sb.maxu > MS) {
+            set_error("dof blocks larger than the LDS tile (%d x %d > %d x %d)", a.tb.maxu, a.sb.maxu, MT, MS);
+            return BEM_ERR_INVALID_ARGUMENT;
+        }
+        grid = dim3(a.sb.nblocks, a.tb.nblocks);
+    } else {
+        grid = dim3((a.trial.n + kRegThreads - 1) / kRegThreads, (a.test.n + TM - 1) / TM);
+    }
+    if (grid.x == 0 || grid.y == 0) return BEM_OK;
+    if (grid.y > 65535u) {
+        set_error("too many test tiles for one launch (%u)", grid.y);
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+#define BEM_LAUNCH(K, T)                                                                                         \
+    hipLaunchKernelGGL((regular_kernel<OP, NTS, NSS, QT, TM, MIXED, K, T, (T ? MT : 1), (T ? MS : 1), OutT>), grid, \
+                       dim3(kRegThreads), 0, stream, a)
+    if (tile) {
+        if (MIXED && kahan) BEM_LAUNCH(true, true);
+        else BEM_LAUNCH(false, true);
+    } else {
+        if (MIXED && kahan) BEM_LAUNCH(true, false);
+        else BEM_LAUNCH(false, false);
+    }
+#undef BEM_LAUNCH
+    return check_launch("regular_kernel");
+}
+
+template <int OP, int NTS, int NSS, bool MIXED, typename OutT>
+static int launch2(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
+    if (a.rule.n == 6) return launch3<OP, NTS, NSS, 6, MIXED, OutT>(a, kahan, tile, stream);
+    return launch3<OP, NTS, NSS, 0, MIXED, OutT>(a, kahan, tile, stream);
+}
+
+template <int OP, bool MIXED, typename OutT>
+static int launch1(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
+    if constexpr (OP == kMaxwellEFIE) {
+        return launch2<OP, 3, 3, MIXED, OutT>(a, kahan, tile, stream);
+    } else {
+        int nt = nshape_of(a.test.shape), ns = nshape_of(a.trial.shape);
+        if (nt == 1 && ns == 1) return launch2<OP, 1, 1, MIXED, OutT>(a, kahan, tile, stream);
+        if (nt == 1 && ns == 3) return launch2<OP, 1, 3, MIXED, OutT>(a, kahan, tile, stream);
+        if (nt == 3 && ns == 1) return launch2<OP, 3, 1, MIXED, OutT>(a, kahan, tile, stream);
+        return launch2<OP, 3, 3, MIXED, OutT>(a, kahan, tile, stream);
+    }
+}
+
+template <int OP>
+static int launch_op(const RegularArgs &a, int precision, int kahan, bool tile, int out_dtype, hipStream_t stream) {
+    constexpr bool CPLX = OpTraits<OP>::is_complex;
+    const bool mixed = precision == BEM_PRECISION_MIXED;
+    if (CPLX) {
+        if (out_dtype == BEM_C128)
+            return mixed ? launch1<OP, true, double>(a, kahan, tile, stream) : launch1<OP, false, double>(a, kahan, tile, stream);
+        if (out_dtype == BEM_C64)
+            return mixed ? launch1<OP, true, float>(a, kahan, tile, stream) : launch1<OP, false, float>(a, kahan, tile, stream);
+    } else {
+        if (out_dtype == BEM_F64)
+            return mixed ? launch1<OP, true, double>(a, kahan, tile, stream) : launch1<OP, false, double>(a, kahan, tile, stream);
+        if (out_dtype == BEM_F32)
+            return mixed ? launch1<OP, true, float>(a, kahan, tile, stream) : launch1<OP, false, float>(a, kahan, tile, stream);
+    }
+    set_error("output dtype %d does not match operator %d", out_dtype, OP);
+    return BEM_ERR_INVALID_ARGUMENT;
+}
+
+// One translation unit per operator instantiates launch_op<OP> (regular_op<k>.hip).
+int regular_launch_op(int op, const RegularArgs &a, int precision, int kahan, bool tile, int out_dtype, hipStream_t s);
+#define BEM_DECLARE_REGULAR_OP(N) \
+    int regular_launch_##N(const RegularArgs &a, int precision, int kahan, bool tile, int out_dtype, hipStream_t s);
+BEM_DECLARE_REGULAR_OP(0)
+BEM_DECLARE_REGULAR_OP(1)
+BEM_DECLARE_REGULAR_OP(2)
+BEM_DECLARE_REGULAR_OP(3)
+BEM_DECLARE_REGULAR_OP(4)
+BEM_DECLARE_REGULAR_OP(5)
+BEM_DECLARE_REGULAR_OP(6)
+#undef BEM_DECLARE_REGULAR_OP
+
+}  // namespace bem

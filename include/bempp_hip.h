@@ -88,16 +88,32 @@ typedef struct bem_quad_rule {
     double x[16], y[16], w[16];
 } bem_quad_rule_t;
 
+/* Blocking of one side for bem_regular_assemble_blocked (device arrays). The element list of
+ * the bem_space_t is split into n_blocks consecutive blocks; block b holds list positions
+ * [start[b], start[b+1]) (at most 256 elements for the trial side, at most 16 for the test side)
+ * and the ndofs[b] <= max_dofs distinct global dofs dofs[b * max_dofs + s]; slots[3 * pos + i]
+ * is the index s of the dof of shape function i of the element at list position pos. */
+typedef struct bem_dof_blocks {
+    int32_t n_blocks;
+    int32_t max_dofs;
+    const int32_t *start;  /* [n_blocks + 1] */
+    const int32_t *dofs;   /* [n_blocks * max_dofs] */
+    const int32_t *ndofs;  /* [n_blocks] */
+    const int32_t *slots;  /* [n_elements * 3] */
+} bem_dof_blocks_t;
+
 /*
  * Regular part of a dense Galerkin operator: all (test, trial) element pairs, except,
  * when grids_identical != 0, pairs sharing a vertex (bempp_cl/core/numba_kernels.py:980-1074,
  * 1853-1969; pairs sharing vertices are left to bem_singular_assemble).
  *
- * scatter: 0 = direct. Every pair owns its output entries (both spaces localised: DP0/DP1,
- *              one dof per (element, shape function)); entries are written once, added to
- *              the existing value when accumulate != 0, stored otherwise (no prior zeroing).
- *          1 = atomic. Dofs are shared between elements (P1, RWG/SNC); contributions are
- *              added atomically, so `out` must be initialised (e.g. bem_fill_zero).
+ * bem_regular_assemble is for spaces whose (element, shape function) pairs own distinct dofs
+ * (DP0, DP1): every entry is written once; it is added to the existing value when
+ * accumulate != 0 and stored otherwise (no prior zeroing needed).
+ * bem_regular_assemble_blocked is for spaces with dofs shared between elements (P1, RWG/SNC):
+ * contributions are summed per (test block, trial block) in LDS and added to `out`, which
+ * must be initialised (e.g. bem_fill_zero). At most 24 test dofs and 192 trial dofs per block
+ * (160 for complex128 output).
  * k_re, k_im: wavenumber for Helmholtz/Maxwell (ignored for Laplace).
  * precision: enum bem_precision. mixed_near_factor: in MIXED mode, pairs whose centroid
  *   distance is below factor * (diameter_test + diameter_trial) / 2 use FP64.
@@ -107,7 +123,24 @@ typedef struct bem_quad_rule {
 int bem_regular_assemble(int op, const bem_space_t *test, const bem_space_t *trial, int grids_identical,
                          const bem_quad_rule_t *rule, double k_re, double k_im, int precision,
                          double mixed_near_factor, int kahan, void *out, int64_t ld, int out_dtype,
-                         int scatter, int accumulate, void *stream);
+                         int accumulate, void *stream);
+int bem_regular_assemble_blocked(int op, const bem_space_t *test, const bem_space_t *trial,
+                                 const bem_dof_blocks_t *test_blocks, const bem_dof_blocks_t *trial_blocks,
+                                 int grids_identical, const bem_quad_rule_t *rule, double k_re, double k_im,
+                                 int precision, double mixed_near_factor, int kahan, void *out, int64_t ld,
+                                 int out_dtype, void *stream);
+
+/*
+ * Near-field regular pairs of the MIXED mode, in FP64 (no reference counterpart: the
+ * reference computes every regular pair in one precision). The MIXED regular kernels skip
+ * every non-adjacent pair with |c_test - c_trial| < factor * (diam_test + diam_trial) / 2;
+ * this entry point computes exactly those among the n_pairs candidate pairs
+ * pairs[2 p] (test element), pairs[2 p + 1] (trial element) and adds them to `out` with
+ * atomics. The candidate list must contain every such pair; other candidates are ignored.
+ */
+int bem_near_assemble(int op, const bem_space_t *test, const bem_space_t *trial, int n_pairs, const int32_t *pairs,
+                      int grids_identical, const bem_quad_rule_t *rule, double k_re, double k_im,
+                      double near_factor, void *out, int64_t ld, int out_dtype, void *stream);
 
 /*
  * Singular (Sauter-Schwab) part, always FP64: for each of n_pairs pairs p,
