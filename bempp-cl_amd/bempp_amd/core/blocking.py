@@ -10,10 +10,12 @@ dofs, the distinct dof list of each block and the slot of every (element, shape 
 
 import numpy as _np
 
+# limits of the device tile (BEM_TILE_* in include/bempp_hip.h)
 TEST_TILE_ELEMENTS = 16
-TEST_TILE_DOFS = 24
+TEST_TILE_DOFS = 20
 TRIAL_BLOCK_ELEMENTS = 256
-TRIAL_BLOCK_DOFS = {8: 192, 16: 160}  # by output element size (complex64 / complex128)
+TRIAL_BLOCK_DOFS = 160
+MAX_INCIDENCE = 12
 
 _CACHE = {}
 
@@ -34,13 +36,34 @@ def _morton_keys(points):
 class Blocks(object):
     """Host arrays of one blocking."""
 
-    def __init__(self, order, start, dofs, ndofs, slots, max_dofs):
+    def __init__(self, order, start, dofs, ndofs, slots, max_dofs, nshape):
         self.order = order  # element indices in list order
         self.start = start
         self.dofs = dofs
         self.ndofs = ndofs
         self.slots = slots
         self.max_dofs = max_dofs
+        self.nshape = nshape
+        self.incidence = None
+
+    def build_incidence(self, multipliers, max_incidence):
+        """int16 [n_blocks, max_dofs, max_incidence]: codes 3 * (pos - start) + i of each slot, -1 padded."""
+        inc = _np.full((self.n_blocks, self.max_dofs, max_incidence), -1, dtype=_np.int16)
+        fill = _np.zeros((self.n_blocks, self.max_dofs), dtype=_np.int64)
+        for b in range(self.n_blocks):
+            for pos in range(self.start[b], self.start[b + 1]):
+                e = self.order[pos]
+                for i in range(self.nshape):
+                    if multipliers[e, i] == 0:
+                        continue
+                    s = self.slots[pos, i]
+                    k = fill[b, s]
+                    if k >= max_incidence:
+                        raise ValueError("a dof has more element incidences in one block than the device supports")
+                    inc[b, s, k] = 3 * (pos - self.start[b]) + i
+                    fill[b, s] = k + 1
+        self.incidence = inc
+        return inc
 
     @property
     def n_blocks(self):
@@ -51,26 +74,30 @@ class Blocks(object):
         return int(self.ndofs.sum())
 
 
-def _greedy_blocks(order, l2g, mult, max_elements, max_dofs):
+def _greedy_blocks(order, l2g, mult, max_elements, max_dofs, max_incidence=MAX_INCIDENCE):
     nshape = l2g.shape[1]
     starts = [0]
     block_dofs = []
     slots = _np.zeros((len(order), 3), dtype=_np.int32)
     current = {}
+    uses = {}
     count = 0
     for pos, e in enumerate(order):
         dofs = [int(l2g[e, i]) for i in range(nshape) if mult[e, i] != 0]
         new = len({d for d in dofs if d not in current})
-        if count == max_elements or len(current) + new > max_dofs:
+        crowded = any(uses.get(d, 0) + dofs.count(d) > max_incidence for d in dofs)
+        if count == max_elements or len(current) + new > max_dofs or crowded:
             block_dofs.append(current)
             starts.append(pos)
             current = {}
+            uses = {}
             count = 0
         for i in range(nshape):
             if mult[e, i] != 0:
                 d = int(l2g[e, i])
                 if d not in current:
                     current[d] = len(current)
+                uses[d] = uses.get(d, 0) + 1
                 slots[pos, i] = current[d]
         count += 1
     block_dofs.append(current)
@@ -91,7 +118,7 @@ def _greedy_blocks(order, l2g, mult, max_elements, max_dofs):
         seg = slice(starts[b], starts[b + 1])
         slots[seg] = old_to_new[slots[seg]]
     return Blocks(_np.asarray(order, dtype=_np.int32), _np.asarray(starts, dtype=_np.int32), dofs, ndofs, slots,
-                  max_dofs)
+                  max_dofs, nshape)
 
 
 def make_blocks(space, max_elements, max_dofs):
@@ -125,6 +152,9 @@ class DeviceBlocks(object):
         self.dofs = torch.from_numpy(blocks.dofs).to(dev)
         self.ndofs = torch.from_numpy(blocks.ndofs).to(dev)
         self.slots = torch.from_numpy(blocks.slots).to(dev)
+        if blocks.incidence is None:
+            blocks.build_incidence(space.local_multipliers, MAX_INCIDENCE)
+        self.incidence = torch.from_numpy(_np.ascontiguousarray(blocks.incidence)).to(dev)
         self.base = base
         self.space_struct = hb.BemSpace(
             int(len(blocks.order)), base.shapeset, self.elements.data_ptr(), base.grid.geometry.data_ptr(),
@@ -133,19 +163,19 @@ class DeviceBlocks(object):
         )
         self.struct = hb.BemDofBlocks(
             blocks.n_blocks, blocks.max_dofs, self.start.data_ptr(), self.dofs.data_ptr(), self.ndofs.data_ptr(),
-            self.slots.data_ptr(),
+            self.slots.data_ptr(), self.incidence.data_ptr(), MAX_INCIDENCE,
         )
 
 
 _DEVICE_CACHE = {}
 
 
-def device_blocks(space, role, out_itemsize):
+def device_blocks(space, role):
     """Device blocking of a space as test ('test') or trial ('trial') side."""
     if role == "test":
         me, md = TEST_TILE_ELEMENTS, TEST_TILE_DOFS
     else:
-        me, md = TRIAL_BLOCK_ELEMENTS, TRIAL_BLOCK_DOFS[out_itemsize]
+        me, md = TRIAL_BLOCK_ELEMENTS, TRIAL_BLOCK_DOFS
     key = (space.id, me, md)
     if key not in _DEVICE_CACHE:
         _DEVICE_CACHE[key] = DeviceBlocks(space, make_blocks(space, me, md))

@@ -138,9 +138,8 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     else:
         from bempp_amd.core.blocking import device_blocks
 
-        itemsize = out.element_size() * (1 if out.is_complex() else 2)
         hb.regular_assemble_blocked(
-            op, device_blocks(dual_to_range, "test", itemsize), device_blocks(domain, "trial", itemsize),
+            op, device_blocks(dual_to_range, "test"), device_blocks(domain, "trial"),
             grids_identical, rule, _wavenumber(operator_descriptor), mode, out, near_factor=near, kahan=kahan,
         )
     if events is not None:

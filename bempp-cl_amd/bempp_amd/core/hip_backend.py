@@ -54,6 +54,8 @@ class BemDofBlocks(ctypes.Structure):
         ("dofs", ctypes.c_void_p),
         ("ndofs", ctypes.c_void_p),
         ("slots", ctypes.c_void_p),
+        ("incidence", ctypes.c_void_p),
+        ("max_incidence", ctypes.c_int32),
     ]
 
 

@@ -108,6 +108,10 @@ struct C2d {
     double re, im;
 };
 
+struct C2f {
+    float re, im;
+};
+
 // Element centroid v0 + (e1 + e2) / 3 from a geometry row, with explicitly rounded
 // operations so that every kernel evaluating it gets bit-identical results.
 __device__ __forceinline__ void centroid_rn(const double *g, double *c) {

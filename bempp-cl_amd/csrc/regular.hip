@@ -52,11 +52,13 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
     a.ld = ld;
     a.identical = grids_identical;
     a.accumulate = accumulate;
-    a.tb = BlockArgs{nullptr, nullptr, nullptr, nullptr, 0, 0};
+    a.tb = BlockArgs{nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, 0};
     a.sb = a.tb;
     if (tile) {
-        a.tb = BlockArgs{tblocks->start, tblocks->dofs, tblocks->ndofs, tblocks->slots, tblocks->n_blocks, tblocks->max_dofs};
-        a.sb = BlockArgs{sblocks->start, sblocks->dofs, sblocks->ndofs, sblocks->slots, sblocks->n_blocks, sblocks->max_dofs};
+        a.tb = BlockArgs{tblocks->start, tblocks->dofs, tblocks->ndofs, tblocks->slots, tblocks->incidence,
+                         tblocks->n_blocks, tblocks->max_dofs, tblocks->max_incidence};
+        a.sb = BlockArgs{sblocks->start, sblocks->dofs, sblocks->ndofs, sblocks->slots, sblocks->incidence,
+                         sblocks->n_blocks, sblocks->max_dofs, sblocks->max_incidence};
     }
     hipStream_t s = (hipStream_t)stream;
     switch (op) {

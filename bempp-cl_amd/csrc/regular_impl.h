@@ -51,9 +51,13 @@ struct BlockArgs {
     const int32_t *dofs;   // [nblocks * maxu] distinct global dofs per block
     const int32_t *ndofs;  // [nblocks]
     const int32_t *slots;  // [n_list * 3] slot of (list position, shape function) in its block
+    const int16_t *inc;    // trial side: [nblocks * maxu * maxinc] incidences 3 * (pos - start) + j, -1 padded
     int nblocks;
     int maxu;
+    int maxinc;
 };
+
+constexpr int kMaxInc = 12;  // incidences per dof and block held in registers by the owner lane
 
 struct RegularArgs {
     SpaceArgs test, trial;
@@ -67,7 +71,8 @@ struct RegularArgs {
     int accumulate;
 };
 
-template <int TM>
+// NXP: 1 in MIXED mode, where the absolute FP64 test points are not needed
+template <int TM, int NXP = TM>
 struct TestTile {
     int elem[TM];
     int vid[TM][3];
@@ -75,7 +80,7 @@ struct TestTile {
     double mult[TM][3];
     double c[TM][3];
     double diam[TM];
-    double xp[TM][kMaxQ][3];  // absolute test points (FP64 paths)
+    double xp[NXP][kMaxQ][3];  // absolute test points (FP64 paths)
     float a[TM][kMaxQ][3];    // offsets from the centroid (FP32 path)
     float aa[TM][kMaxQ];      // |a|^2
     float wp[TM][kMaxQ];      // w_p J_e
@@ -138,8 +143,8 @@ __device__ __forceinline__ void pair_fp64_geom(const double *__restrict__ gt, co
 }
 
 // FP64 pair with test points from LDS and trial points from registers
-template <int OP, int NTS, int NSS, int QT, int TM>
-__device__ __forceinline__ void pair_fp64_regs(const TestTile<TM> &T_, int t, const double *gt, double tnm,
+template <int OP, int NTS, int NSS, int QT, int TM, int NXP>
+__device__ __forceinline__ void pair_fp64_regs(const TestTile<TM, NXP> &T_, int t, const double *gt, double tnm,
                                                const double (&yq)[QT > 0 ? QT : 1][3], const double (&wq)[QT > 0 ? QT : 1],
                                                const double *ny, const RuleArgs &rule, double kr, double ki,
                                                C2d (&S)[NTS][NSS]) {
@@ -200,10 +205,10 @@ struct TrialF32 {
 __device__ __forceinline__ float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
 __device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
 
-template <int OP, int NTS, int NSS, int QT, int TM, bool KAHAN>
-__device__ __forceinline__ void pair_fp32(const TestTile<TM> &T_, int t, const TrialF32<QT> &F, double d0, double d1,
+template <int OP, int NTS, int NSS, int QT, int TM, bool KAHAN, int NXP>
+__device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, const TrialF32<QT> &F, double d0, double d1,
                                           double d2, double dist2, const RuleArgs &rule, double kr, double ki,
-                                          C2d (&S)[NTS][NSS]) {
+                                          C2f (&S)[NTS][NSS]) {
     constexpr bool HELM = OP >= kHelmholtzSL;
     const int nq = QT > 0 ? QT : rule.n;
     const float fd0 = (float)d0, fd1 = (float)d1, fd2 = (float)d2;
@@ -307,24 +312,52 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM> &T_, int t, const T
             }
     }
     // multiply by e^{i k r0} e^{-ki r0} (FP64 phase reduced to one revolution)
-    double pr = 1.0, pi = 0.0;
+    float pr = 1.f, pi = 0.f;
     if constexpr (HELM) {
         double rev = kr * kInv2Pi * (double)r0;
         rev -= rint(rev);
-        float sf = sin_rev((float)rev), cf = cos_rev((float)rev);
-        double damp = ki != 0.0 ? exp(-ki * (double)r0) : 1.0;
-        pr = (double)cf * damp;
-        pi = (double)sf * damp;
+        float damp = ki != 0.0 ? __expf((float)(-ki * (double)r0)) : 1.f;
+        pr = cos_rev((float)rev) * damp;
+        pi = sin_rev((float)rev) * damp;
     }
 #pragma unroll
     for (int i = 0; i < NTS; ++i)
 #pragma unroll
         for (int j = 0; j < NSS; ++j) {
-            double sr = (double)Sr[i][j].s - (double)Sr[i][j].c;
-            double si = HELM ? (double)Si[i][j].s - (double)Si[i][j].c : 0.0;
+            float sr = Sr[i][j].s - Sr[i][j].c;
+            float si = HELM ? Si[i][j].s - Si[i][j].c : 0.f;
             S[i][j].re = pr * sr - pi * si;
             S[i][j].im = pr * si + pi * sr;
         }
+}
+
+// Scalar local block from FP32 moments: phi = (u0 - u1 - u2, u1, u2) on P1 sides, u0 on P0
+// sides, i.e. V = C^T S C with the sparse 3x3 change of basis C (see local_value).
+template <int NTS, int NSS>
+__device__ __forceinline__ void scalar_block_f32(const C2f (&S)[NTS][NSS], C2f (&V)[NTS][NSS]) {
+    C2f Y[NTS][NSS];
+#pragma unroll
+    for (int a = 0; a < NTS; ++a) {
+        if constexpr (NSS == 3) {
+            Y[a][0].re = S[a][0].re - S[a][1].re - S[a][2].re;
+            Y[a][0].im = S[a][0].im - S[a][1].im - S[a][2].im;
+            Y[a][1] = S[a][1];
+            Y[a][2] = S[a][2];
+        } else {
+            Y[a][0] = S[a][0];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NSS; ++j) {
+        if constexpr (NTS == 3) {
+            V[0][j].re = Y[0][j].re - Y[1][j].re - Y[2][j].re;
+            V[0][j].im = Y[0][j].im - Y[1][j].im - Y[2][j].im;
+            V[1][j] = Y[1][j];
+            V[2][j] = Y[2][j];
+        } else {
+            V[0][j] = Y[0][j];
+        }
+    }
 }
 
 template <typename OutT, bool CPLX>
@@ -357,11 +390,18 @@ template <int OP, int NTS, int NSS, int QT, int TM, bool MIXED, bool KAHAN, bool
 __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs a) {
     constexpr bool CPLX = OpTraits<OP>::is_complex;
     constexpr int NT = NTS, NS = NSS;
+    constexpr int NC = CPLX ? 2 : 1;
+    // OWNER: the trial-side reduction of the tile runs without atomics. For every test
+    // element each lane stores its local block to LDS; after a barrier the lane owning trial
+    // slot s sums the block entries of the (element, j) incidences of s and updates its own
+    // column of the tile. The FP64 mode, which is compute bound, adds with LDS atomics.
+    constexpr bool OWNER = TILE && MIXED;
     using AccT = OutT;  // LDS accumulation type: float in MIXED (complex64), double in FP64 mode
     const int nq = QT > 0 ? QT : a.rule.n;
 
-    __shared__ TestTile<TM> T_;
-    __shared__ AccT acc[TILE ? MAXU_T * MAXU_S * (CPLX ? 2 : 1) : 1];
+    __shared__ TestTile<TM, MIXED ? 1 : TM> T_;
+    __shared__ AccT acc[TILE ? MAXU_T * MAXU_S * NC : 1];
+    __shared__ float stage[OWNER ? NT * NS * NC : 1][OWNER ? kRegThreads : 1];
 
     const int tid = threadIdx.x;
     int t0, t1, s0, s1;
@@ -405,7 +445,7 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
             double xi = a.rule.x[p], eta = a.rule.y[p];
             for (int d = 0; d < 3; ++d) {
                 double off = g[3 + d] * (xi - 1.0 / 3.0) + g[6 + d] * (eta - 1.0 / 3.0);
-                T_.xp[tid][p][d] = g[d] + g[3 + d] * xi + g[6 + d] * eta;
+                if constexpr (!MIXED) T_.xp[tid][p][d] = g[d] + g[3 + d] * xi + g[6 + d] * eta;
                 T_.a[tid][p][d] = (float)off;
             }
             T_.aa[tid][p] = T_.a[tid][p][0] * T_.a[tid][p][0] + T_.a[tid][p][1] * T_.a[tid][p][1] +
@@ -477,6 +517,19 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
         }
     }
 
+    // owner lane state: the incidences (3 * lane + j) of trial slot `tid` in this block
+    int own_inc[OWNER ? kMaxInc : 1];
+    int own_nds = 0;
+    if constexpr (OWNER) {
+        own_nds = min(a.sb.ndofs[blockIdx.x], MAXU_S);
+        const int16_t *src = a.sb.inc + ((int64_t)blockIdx.x * a.sb.maxu + min(tid, a.sb.maxu - 1)) * a.sb.maxinc;
+#pragma unroll
+        for (int k = 0; k < kMaxInc; ++k) {
+            int code = (tid < own_nds && k < a.sb.maxinc) ? (int)src[k] : -1;
+            own_inc[k] = code < 3 * kRegThreads ? code : -1;
+        }
+    }
+
     for (int t = 0; t < ntile; ++t) {
         const int e = T_.elem[t];
         const double *gt = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
@@ -486,11 +539,15 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
             adj = v0 == fv0 || v0 == fv1 || v0 == fv2 || v1 == fv0 || v1 == fv1 || v1 == fv2 || v2 == fv0 ||
                   v2 == fv1 || v2 == fv2;
         }
-        C2d S[NTS][NSS];
+        C2d S[NTS][NSS];   // FP64 moments (FP64 mode; MIXED paths that need the FP64 epilogue)
+        C2f Sf[NTS][NSS];  // FP32 moments (MIXED)
 #pragma unroll
         for (int i = 0; i < NTS; ++i)
 #pragma unroll
-            for (int j = 0; j < NSS; ++j) S[i][j].re = S[i][j].im = 0.0;
+            for (int j = 0; j < NSS; ++j) {
+                S[i][j].re = S[i][j].im = 0.0;
+                Sf[i][j].re = Sf[i][j].im = 0.f;
+            }
 
         // MIXED: near pairs are left to the FP64 near-field kernel (bem_near_assemble)
         bool skip = adj;
@@ -507,12 +564,65 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
                 skip = pair_is_near(T_.c[t], cfv, T_.diam[t], diam_f, a.near_factor, &dist2);
                 if (!skip) {
                     double d0 = cf0 - T_.c[t][0], d1 = cf1 - T_.c[t][1], d2 = cf2 - T_.c[t][2];
-                    pair_fp32<OP, NTS, NSS, QT, TM, KAHAN>(T_, t, F32, d0, d1, d2, dist2, a.rule, a.kr, a.ki, S);
+                    pair_fp32<OP, NTS, NSS, QT, TM, KAHAN>(T_, t, F32, d0, d1, d2, dist2, a.rule, a.kr, a.ki, Sf);
                 }
             }
         }
+        if constexpr (MIXED && (!OWNER || OP == kMaxwellEFIE)) {
+#pragma unroll
+            for (int i = 0; i < NTS; ++i)
+#pragma unroll
+                for (int j = 0; j < NSS; ++j) {
+                    S[i][j].re = (double)Sf[i][j].re;
+                    S[i][j].im = (double)Sf[i][j].im;
+                }
+        }
 
-        if (valid) {
+        if constexpr (OWNER) {
+            // stage this lane's block (zero for idle lanes, skipped pairs, zero multipliers)
+            C2f V[NTS][NSS];
+            if constexpr (OP != kMaxwellEFIE) scalar_block_f32<NTS, NSS>(Sf, V);
+#pragma unroll
+            for (int i = 0; i < NT; ++i)
+#pragma unroll
+                for (int j = 0; j < NS; ++j) {
+                    float vr = 0.f, vi = 0.f;
+                    const float m = (float)T_.mult[t][i] * (float)mf[j];
+                    if (valid && !skip && m != 0.f) {
+                        if constexpr (OP == kMaxwellEFIE) {
+                            C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
+                            vr = (float)v.re * m;
+                            vi = (float)v.im * m;
+                        } else {
+                            vr = V[i][j].re * m;
+                            vi = V[i][j].im * m;
+                        }
+                    }
+                    stage[(j * NT + i) * NC][tid] = vr;
+                    if constexpr (CPLX) stage[(j * NT + i) * NC + 1][tid] = vi;
+                }
+            __syncthreads();
+            if (tid < own_nds) {
+#pragma unroll
+                for (int i = 0; i < NT; ++i) {
+                    if (T_.mult[t][i] == 0.0) continue;
+                    float sr = 0.f, si = 0.f;
+#pragma unroll
+                    for (int k = 0; k < kMaxInc; ++k) {
+                        const int code = own_inc[k];
+                        if (code >= 0) {
+                            const int p = code / 3, j = code - 3 * p;
+                            sr += stage[(j * NT + i) * NC][p];
+                            if constexpr (CPLX) si += stage[(j * NT + i) * NC + 1][p];
+                        }
+                    }
+                    AccT *dst = acc + NC * (T_.slot[t][i] * MAXU_S + tid);
+                    dst[0] += (AccT)sr;
+                    if constexpr (CPLX) dst[1] += (AccT)si;
+                }
+            }
+            __syncthreads();
+        } else if (valid) {
 #pragma unroll
             for (int i = 0; i < NT; ++i) {
                 const double mi = T_.mult[t][i];
@@ -564,13 +674,18 @@ template <int OP, int NTS, int NSS, int QT, bool MIXED, typename OutT>
 static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
     constexpr int TM = 16;
     constexpr bool CPLX = OpTraits<OP>::is_complex;
-    // LDS tile of distinct dofs: test slots x trial slots
-    constexpr int MT = 24;
-    constexpr int MS = (CPLX && sizeof(OutT) == 8) ? 160 : 192;
+    // LDS tile of distinct dofs: test slots x trial slots (BEM_TILE_TEST_DOFS x BEM_TILE_TRIAL_DOFS)
+    constexpr int MT = BEM_TILE_TEST_DOFS;
+    constexpr int MS = BEM_TILE_TRIAL_DOFS;
+    (void)CPLX;
     dim3 grid;
     if (tile) {
-        if (a.tb.maxu > MT || a.sb.maxu > MS) {
+        if (a.tb.maxu > MT || a.sb.maxu > MS || a.tb.maxu < 1 || a.sb.maxu < 1) {
             set_error("dof blocks larger than the LDS tile (%d x %d > %d x %d)", a.tb.maxu, a.sb.maxu, MT, MS);
+            return BEM_ERR_INVALID_ARGUMENT;
+        }
+        if (MIXED && (a.sb.inc == nullptr || a.sb.maxinc < 1 || a.sb.maxinc > kMaxInc)) {
+            set_error("mixed precision needs trial incidence lists with 1..%d entries per dof", kMaxInc);
             return BEM_ERR_INVALID_ARGUMENT;
         }
         grid = dim3(a.sb.nblocks, a.tb.nblocks);

@@ -100,7 +100,19 @@ typedef struct bem_dof_blocks {
     const int32_t *dofs;   /* [n_blocks * max_dofs] */
     const int32_t *ndofs;  /* [n_blocks] */
     const int32_t *slots;  /* [n_elements * 3] */
+    /* trial side, MIXED precision: for block b and slot s, up to max_incidence codes
+     * 3 * (pos - start[b]) + i of the (element, shape function) pairs carrying dof s, -1 padded:
+     * incidence[(b * max_dofs + s) * max_incidence + k]. May be NULL for the test side. */
+    const int16_t *incidence;
+    int32_t max_incidence;
 } bem_dof_blocks_t;
+
+/* LDS tile limits of bem_regular_assemble_blocked */
+#define BEM_TILE_TEST_ELEMENTS 16
+#define BEM_TILE_TEST_DOFS 20
+#define BEM_TILE_TRIAL_ELEMENTS 256
+#define BEM_TILE_TRIAL_DOFS 160
+#define BEM_TILE_MAX_INCIDENCE 12
 
 /*
  * Regular part of a dense Galerkin operator: all (test, trial) element pairs, except,
@@ -112,8 +124,7 @@ typedef struct bem_dof_blocks {
  * accumulate != 0 and stored otherwise (no prior zeroing needed).
  * bem_regular_assemble_blocked is for spaces with dofs shared between elements (P1, RWG/SNC):
  * contributions are summed per (test block, trial block) in LDS and added to `out`, which
- * must be initialised (e.g. bem_fill_zero). At most 24 test dofs and 192 trial dofs per block
- * (160 for complex128 output).
+ * must be initialised (e.g. bem_fill_zero). Block limits: BEM_TILE_* below.
  * k_re, k_im: wavenumber for Helmholtz/Maxwell (ignored for Laplace).
  * precision: enum bem_precision. mixed_near_factor: in MIXED mode, pairs whose centroid
  *   distance is below factor * (diameter_test + diameter_trial) / 2 use FP64.

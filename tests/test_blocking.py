@@ -19,7 +19,7 @@ def ico3():
 
 
 @pytest.mark.parametrize("kind", [("P", 1), ("RWG", 0), ("DP", 0)])
-@pytest.mark.parametrize("limits", [(16, 24), (256, 192)])
+@pytest.mark.parametrize("limits", [(16, 20), (256, 160)])
 def test_blocks_are_consistent(ico3, kind, limits):
     from bempp_amd.api import function_space
     from bempp_amd.core.blocking import make_blocks
@@ -60,12 +60,21 @@ def test_tiled_scatter_equals_direct_scatter():
     vals = L * m[:, None, :, None] * m[None, :, None, :]
     np.add.at(direct, (rows.ravel(), cols.ravel()), vals.ravel())
 
-    tb = make_blocks(sp, 16, 24)
-    sb = make_blocks(sp, 256, 192)
+    tb = make_blocks(sp, 16, 20)
+    sb = make_blocks(sp, 256, 160)
+    inc = sb.build_incidence(m, 12)
+    # incidence lists name exactly the (element, shape function) pairs of each slot
+    for bs in range(sb.n_blocks):
+        for s in range(sb.ndofs[bs]):
+            codes = [c for c in inc[bs, s] if c >= 0]
+            assert codes
+            for c in codes:
+                pos = sb.start[bs] + c // 3
+                assert sb.slots[pos, c % 3] == s and m[sb.order[pos], c % 3] != 0
     tiled = np.zeros((N, N))
     for bt in range(tb.n_blocks):
         for bs in range(sb.n_blocks):
-            tile = np.zeros((24, 192))
+            tile = np.zeros((20, 160))
             for pt in range(tb.start[bt], tb.start[bt + 1]):
                 e = tb.order[pt]
                 for ps in range(sb.start[bs], sb.start[bs + 1]):
