@@ -123,7 +123,7 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
         hb.fill_zero(out)
     hip_opts = getattr(parameters, "hip", None)
     near = getattr(hip_opts, "mixed_near_field_factor", 2.0) if hip_opts is not None else 2.0
-    kahan = getattr(hip_opts, "kahan", True) if hip_opts is not None else True
+    kahan = getattr(hip_opts, "kahan", False) if hip_opts is not None else False
     srule = None
     if grids_identical and singular:
         srule = singular_rule(dual_to_range.grid, parameters.quadrature.singular, dual_to_range.support, domain.support)

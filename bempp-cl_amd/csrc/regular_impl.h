@@ -205,7 +205,9 @@ struct TrialF32 {
 __device__ __forceinline__ float sin_rev(float x) { return __builtin_amdgcn_sinf(x); }
 __device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf(x); }
 
-template <int OP, int NTS, int NSS, int QT, int TM, bool KAHAN, int NXP>
+// DAMP: complex wavenumber (e^{-Im(k) r} factor); a template parameter so that real
+// wavenumbers do not pay for an if-converted exponential per evaluation.
+template <int OP, int NTS, int NSS, int QT, int TM, bool KAHAN, bool DAMP, int NXP>
 __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, const TrialF32<QT> &F, double d0, double d1,
                                           double d2, double dist2, const RuleArgs &rule, double kr, double ki,
                                           C2f (&S)[NTS][NSS]) {
@@ -275,7 +277,7 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
                 const float ph = kr_rev * delta;
                 const float s = sin_rev(ph), c = cos_rev(ph);
                 float g = rinv * F.wq[q];
-                if (ki_f != 0.f) g *= __builtin_amdgcn_exp2f(-ki_log2e * delta);
+                if constexpr (DAMP) g *= __builtin_amdgcn_exp2f(-ki_log2e * delta);
                 if constexpr (OP == kHelmholtzSL || OP == kMaxwellEFIE) {
                     gre = c * g;
                     gim = s * g;
@@ -284,7 +286,7 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
                     if constexpr (OP == kHelmholtzDL) dot = wn + F.bn[q];
                     else dot = wnx - bnx[q];
                     g *= dot * rinv * rinv;
-                    const float ar = -1.f - ki_f * r, ai = kr_f * r;
+                    const float ar = DAMP ? -1.f - ki_f * r : -1.f, ai = kr_f * r;
                     gre = (ar * c - ai * s) * g;
                     gim = (ai * c + ar * s) * g;
                 }
@@ -564,7 +566,10 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
                 skip = pair_is_near(T_.c[t], cfv, T_.diam[t], diam_f, a.near_factor, &dist2);
                 if (!skip) {
                     double d0 = cf0 - T_.c[t][0], d1 = cf1 - T_.c[t][1], d2 = cf2 - T_.c[t][2];
-                    pair_fp32<OP, NTS, NSS, QT, TM, KAHAN>(T_, t, F32, d0, d1, d2, dist2, a.rule, a.kr, a.ki, Sf);
+                    if (OpTraits<OP>::is_complex && a.ki != 0.0)
+                        pair_fp32<OP, NTS, NSS, QT, TM, KAHAN, true>(T_, t, F32, d0, d1, d2, dist2, a.rule, a.kr, a.ki, Sf);
+                    else
+                        pair_fp32<OP, NTS, NSS, QT, TM, KAHAN, false>(T_, t, F32, d0, d1, d2, dist2, a.rule, a.kr, a.ki, Sf);
                 }
             }
         }

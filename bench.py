@@ -48,7 +48,20 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU sample")
     ap.add_argument("--no-reference", action="store_true", help="skip the FP64 comparison (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB device to device (reference dispatch for HBM byte counters)")
     return ap.parse_args()
+
+
+def traffic_from_profiles(kernel_key):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    entry = table.get(kernel_key)
+    return None if entry is None else float(entry["hbm_bytes_per_launch"])
 
 
 def rel_fro(a, b):
@@ -63,32 +76,36 @@ def cpu_baseline(grid, space, k, n_entries_step, target_seconds):
 
     cores = len(os.sched_getaffinity(0))
     F = grid.number_of_elements
-    # calibrate on a small block, then time a block sized for ~target_seconds
-    n0 = 64
-    t = time.perf_counter()
-    O.assemble("helmholtz_single_layer", grid.vertices, grid.elements, space, space, k, parts=1,
-               test_range=(0, n0), nthreads=cores)
-    t0 = time.perf_counter() - t
-    n1 = int(min(F, max(n0, n0 * target_seconds / 2.0 / max(t0, 1e-3))))
-    times = {}
-    for op in ("helmholtz_single_layer", "helmholtz_double_layer"):
+
+    def timed(op, rows, parts):
         t = time.perf_counter()
-        O.assemble(op, grid.vertices, grid.elements, space, space, k, parts=1, test_range=(0, n1), nthreads=cores)
-        times[op] = time.perf_counter() - t
-    # singular pairs of the sampled rows, timed the same way
-    t = time.perf_counter()
-    O.assemble("helmholtz_single_layer", grid.vertices, grid.elements, space, space, k, parts=2,
-               test_range=(0, max(1, n1 // 8)), nthreads=cores)
-    ts = (time.perf_counter() - t) * 8
-    est = sum(times.values()) * F / n1 + 2 * ts * F / n1
+        O.assemble(op, grid.vertices, grid.elements, space, space, k, parts=parts, test_range=(0, rows),
+                   nthreads=cores)
+        return time.perf_counter() - t
+
+    # two-point calibration separates the fixed cost (result allocation, colouring) from the
+    # per-row cost, then one block sized for ~target_seconds is timed per operator
+    ta = timed("helmholtz_single_layer", 64, 1)
+    tb = timed("helmholtz_single_layer", 192, 1)
+    per_row = max((tb - ta) / 128, 1e-6)
+    fixed = max(ta - 64 * per_row, 0.0)
+    n1 = int(min(F, max(192, target_seconds / 2.0 / per_row)))
+    est, sampled = 0.0, 0.0
+    for op in ("helmholtz_single_layer", "helmholtz_double_layer"):
+        t_reg = timed(op, n1, 1)
+        sampled += t_reg
+        ns = max(1, n1 // 8)
+        t_sing = timed(op, ns, 2)
+        est += fixed + (t_reg - fixed) * F / n1 + max(t_sing - fixed, 0.0) * F / ns
     return {
         "value": n_entries_step / est,
         "unit": "matrix entries/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"FP64 oracle, Helmholtz SL+DL P1 regular pairs of test elements [0,{n1}) of {F} "
-                  f"({sum(times.values()):.1f} s) + singular pairs of [0,{max(1, n1 // 8)}), extrapolated "
-                  f"linearly in test rows to the full step",
+        "sample": f"FP64 oracle (C/OpenMP restatement of the reference Numba algorithm), Helmholtz SL+DL P1: "
+                  f"regular pairs of test elements [0,{n1}) of {F} ({sampled:.1f} s) and singular pairs of "
+                  f"[0,{max(1, n1 // 8)}), per-row cost extrapolated linearly to all rows; estimated full step "
+                  f"{est:.0f} s",
     }
 
 
@@ -110,6 +127,12 @@ def main():
     from bempp_amd.core.singular_assembler import singular_rule
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     k = args.k
     t = time.perf_counter()
     grid = bempp.shapes.icosphere(args.level)
@@ -190,7 +213,7 @@ def main():
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": sl["regular_tflops"] / PEAK_FP32_TFLOPS,
-        "traffic": None,
+        "traffic": traffic_from_profiles("regular_kernel_helmholtz_single_layer_p1_level5_mixed"),
         "algorithmic_flop_per_launch": n_evals * FLOP_PER_EVAL[name0],
         "flop_per_eval": FLOP_PER_EVAL[name0],
         "evals_per_launch": n_evals,
@@ -212,6 +235,26 @@ def main():
         result["fp64_entries_per_s"] = entries_per_step / t64
         result["speedup_vs_fp64"] = value / result["fp64_entries_per_s"]
         log(f"fp64 step {1e3 * t64:.1f} ms, rel err {errs}")
+
+        # the same step with Kahan-compensated FP32 sums (parameters.hip.kahan)
+        from bempp_amd.api.utils.parameters import DefaultParameters
+
+        pk = DefaultParameters()
+        pk.hip.kahan = True
+        kres = {}
+        for name, (f, _) in ops.items():
+            dk = f(space, space, space, k, precision="single", parameters=pk).descriptor
+            hip_dense_assemble(space, space, pk, dk, out=outs[name], accumulate=False)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            hip_dense_assemble(space, space, pk, dk, out=outs[name], accumulate=False)
+            torch.cuda.synchronize()
+            ms = 1e3 * (time.perf_counter() - t)
+            a64 = hip_dense_assemble(space, space, params, desc64[name])
+            kres[name] = {"ms": ms, "rel_err_vs_fp64": rel_fro(outs[name], a64)}
+            del a64
+        result["kahan_variant"] = kres
+        log(f"kahan variant {kres}")
 
         # potential: complex Gaussian P1 density at 16384 points on the radius-2 sphere
         rng = np.random.default_rng(0)
