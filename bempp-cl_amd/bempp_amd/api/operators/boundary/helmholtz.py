@@ -43,3 +43,25 @@ def adjoint_double_layer(domain, range_, dual_to_range, wavenumber, parameters=N
     """Helmholtz adjoint double-layer boundary operator."""
     return _make("helmholtz_adjoint_double_layer_boundary", "helmholtz_adjoint_double_layer", domain, range_,
                  dual_to_range, wavenumber, parameters, assembler, device_interface, precision)
+
+
+def single_layer_frequency_sweep(domain, range_, dual_to_range, wavenumbers, parameters=None, device_interface=None,
+                                 precision=None):
+    """Weak forms of the Helmholtz single layer at many real wavenumbers (DP0 x DP0 spaces).
+
+    Returns one DenseDiscreteBoundaryOperator per wavenumber, each equal to
+    `single_layer(domain, range_, dual_to_range, k, assembler="dense").weak_form()`, assembled
+    together by the frequency-batched device kernel. The matrices are views of one
+    (len(wavenumbers), rows, cols) device tensor.
+    """
+    import bempp_amd.api as api
+    from bempp_amd.api.assembly.discrete_boundary_operator import DenseDiscreteBoundaryOperator
+    from bempp_amd.core.dispatcher import check_device_interface
+    from bempp_amd.core.frequency_sweep import hip_kbatch_assemble
+
+    for k in _np.atleast_1d(wavenumbers):
+        _check_wavenumber(k)
+    check_device_interface(device_interface or api.DEFAULT_DEVICE_INTERFACE)
+    mats = hip_kbatch_assemble(domain, dual_to_range, wavenumbers, precision or api.DEFAULT_PRECISION,
+                               parameters=parameters)
+    return [DenseDiscreteBoundaryOperator(mats[m]) for m in range(mats.shape[0])]

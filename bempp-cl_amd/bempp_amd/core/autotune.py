@@ -1,0 +1,97 @@
+"""Launch-shape autotuner for the frequency-batched assembler.
+
+The persistent kbatch kernel has two free launch parameters: the number of test elements
+staged in LDS per tile (8, 16 or 32) and the number of resident workgroups per CU
+(the persistent grid is CUs x that number). `tune` times every candidate on the actual
+problem (same spaces, same wavenumbers, same output buffer) with HIP events and keeps the
+fastest. Results are cached in memory and in a JSON file keyed by
+(operator, shapesets, rows, cols, number of wavenumbers, precision, device name), so a
+configuration is tuned once per machine. The cache file is $BEMPP_AMD_TUNE_CACHE, else
+$XDG_CACHE_HOME/bempp_amd/tune.json, else ~/.cache/bempp_amd/tune.json.
+"""
+
+import json
+import os
+
+CANDIDATE_TILES = (8, 16, 32)
+CANDIDATE_WG_PER_CU = (1, 2, 4, 8)
+DEFAULT_CONFIG = (16, 4)
+
+_MEMORY = {}
+
+
+def cache_path():
+    p = os.environ.get("BEMPP_AMD_TUNE_CACHE")
+    if p:
+        return p
+    base = os.environ.get("XDG_CACHE_HOME") or os.path.join(os.path.expanduser("~"), ".cache")
+    return os.path.join(base, "bempp_amd", "tune.json")
+
+
+def _load():
+    try:
+        with open(cache_path()) as f:
+            data = json.load(f)
+        return data if isinstance(data, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(key, value):
+    data = _load()
+    data[key] = value
+    path = cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp%d" % os.getpid()
+        with open(tmp, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home only loses persistence, not correctness
+
+
+def make_key(op, test_shapeset, trial_shapeset, rows, cols, nk, precision, device_name):
+    return "|".join(str(x) for x in (op, test_shapeset, trial_shapeset, rows, cols, nk, precision, device_name))
+
+
+def lookup(key):
+    if key in _MEMORY:
+        return _MEMORY[key]
+    entry = _load().get(key)
+    if entry is not None:
+        cfg = (int(entry["config"][0]), int(entry["config"][1]))
+        _MEMORY[key] = cfg
+        return cfg
+    return None
+
+
+def tune(key, run, candidates=None, repeats=2):
+    """Time run(config) for every candidate config on the current stream; cache and return the best.
+
+    run: callable enqueuing one complete launch with the given (test_tile, workgroups_per_cu).
+    """
+    import torch
+
+    if candidates is None:
+        candidates = [(t, w) for t in CANDIDATE_TILES for w in CANDIDATE_WG_PER_CU]
+    timings = {}
+    for cfg in candidates:
+        run(cfg)  # warm-up (code object load, caches)
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(repeats):
+            run(cfg)
+        end.record()
+        end.synchronize()
+        timings[cfg] = start.elapsed_time(end) / repeats
+    best = min(timings, key=timings.get)
+    _MEMORY[key] = best
+    _store(key, {"config": list(best), "ms": timings[best],
+                 "all": {"%d,%d" % c: round(t, 4) for c, t in timings.items()}})
+    return best
+
+
+def clear_memory():
+    _MEMORY.clear()

@@ -5,7 +5,7 @@ The reference accepts "numba" and "opencl"; this library provides the "hip" inte
 """
 
 
-def _check(device_interface):
+def check_device_interface(device_interface):
     interface_type = str(device_interface).split("_")[0]
     if interface_type != "hip":
         raise ValueError("Device interface must be 'hip'.")
@@ -13,7 +13,7 @@ def _check(device_interface):
 
 def dense_assembler_dispatcher(device_interface, *args, **kwargs):
     """Regular + singular dense assembly on the device; returns a device tensor."""
-    _check(device_interface)
+    check_device_interface(device_interface)
     from bempp_amd.core.dense_assembler import hip_dense_assemble
 
     return hip_dense_assemble(*args, **kwargs)
@@ -21,7 +21,7 @@ def dense_assembler_dispatcher(device_interface, *args, **kwargs):
 
 def potential_dispatcher(device_interface, *args, **kwargs):
     """Return an evaluator closure for a dense potential."""
-    _check(device_interface)
+    check_device_interface(device_interface)
     from bempp_amd.core.dense_potential_assembler import hip_potential_assembler
 
     return hip_potential_assembler(*args, **kwargs)

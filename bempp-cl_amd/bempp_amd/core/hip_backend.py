@@ -68,6 +68,17 @@ class BemQuadRule(ctypes.Structure):
     ]
 
 
+class BemKBatchConfig(ctypes.Structure):
+    _fields_ = [("test_tile", ctypes.c_int32), ("workgroups_per_cu", ctypes.c_int32)]
+
+
+KBATCH_MAX_WAVENUMBERS = 64
+KBATCH_REGULAR = 1
+KBATCH_NEAR = 2
+KBATCH_SINGULAR = 4
+KBATCH_ALL = 7
+
+
 def library_path():
     return _LIB_PATH
 
@@ -105,9 +116,11 @@ def lib():
         L.bem_fill_zero.restype = i
         L.bem_last_error.restype = ctypes.c_char_p
         L.bem_version.restype = ctypes.c_char_p
-        if hasattr(L, "bem_kbatch_assemble"):
-            L.bem_kbatch_assemble.argtypes = [SP, SP, QP, P, i, d, i, P, i64, i64, P, i, P]
-            L.bem_kbatch_assemble.restype = i
+        L.bem_kbatch_assemble.argtypes = [
+            SP, SP, i, QP, i, ctypes.POINTER(ctypes.c_double), i, i, d, ctypes.POINTER(BemKBatchConfig), P, i64, i64,
+            i, i, P, i, P, P, P, P, P,
+        ]
+        L.bem_kbatch_assemble.restype = i
         _LIB = L
     return _LIB
 
@@ -305,3 +318,30 @@ def potential_evaluate(op, space, coefficients, points, rule, wavenumber, precis
         )
     )
     return ws
+
+
+def kbatch_assemble(test, trial, grids_identical, rule, wavenumbers, precision, near_factor, config, out,
+                    near_pairs=None, singular=None, parts=KBATCH_ALL):
+    """Helmholtz single layer (DP0 x DP0) for many real wavenumbers: out is (nk, rows, cols).
+
+    config: (test_tile, workgroups_per_cu). singular: DeviceSingularRule or None.
+    """
+    ks = np.ascontiguousarray(np.asarray(wavenumbers, dtype=np.float64))
+    nk = ks.shape[0]
+    if out.dim() != 3 or out.shape[0] != nk:
+        raise ValueError("out must have shape (number of wavenumbers, rows, cols)")
+    cfg = BemKBatchConfig(int(config[0]), int(config[1]))
+    n_near = 0 if near_pairs is None else int(near_pairs.shape[0])
+    if singular is None:
+        n_sing, sp, tp, spp, w = 0, None, None, None, None
+    else:
+        n_sing, sp, tp, spp, w = int(singular.pairs.shape[0]), singular.pairs, singular.test_points, singular.trial_points, singular.weights
+    check(
+        lib().bem_kbatch_assemble(
+            ctypes.byref(test.struct), ctypes.byref(trial.struct), int(grids_identical), ctypes.byref(rule), nk,
+            ks.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), precision, int(parts), float(near_factor),
+            ctypes.byref(cfg),
+            ptr(out), out.shape[-1], out.shape[-1] * out.shape[-2], _dtype_code(out), n_near, ptr(near_pairs),
+            n_sing, ptr(sp), ptr(tp), ptr(spp), ptr(w), stream_handle(),
+        )
+    )

@@ -184,6 +184,39 @@ int bem_potential_evaluate(int op, const bem_space_t *space, const void *coeffic
                            int precision, void *out, int out_dtype, void *workspace, size_t workspace_bytes,
                            void *stream);
 
+/* Launch shape of bem_kbatch_assemble, picked by the autotuner (bempp_amd/core/autotune.py). */
+typedef struct bem_kbatch_config {
+    int32_t test_tile;          /* test elements staged in LDS per tile: 8, 16 or 32 */
+    int32_t workgroups_per_cu;  /* persistent grid = CUs * this (1..16)              */
+} bem_kbatch_config_t;
+
+#define BEM_KBATCH_MAX_WAVENUMBERS 64
+/* `parts` bits of bem_kbatch_assemble */
+#define BEM_KBATCH_REGULAR 1
+#define BEM_KBATCH_NEAR 2
+#define BEM_KBATCH_SINGULAR 4
+
+/*
+ * Frequency sweep of the Helmholtz single layer, DP0 x DP0 (no single reference counterpart:
+ * the reference assembles one wavenumber per weak_form() call,
+ * bempp_cl/core/numba_kernels.py:980-1074 with helmholtz_single_layer at numba_kernels.py:298-320).
+ * out[m * matrix_stride + row * ld + col] receives the matrix of real wavenumber wavenumbers[m]
+ * (host array, 1 <= nk <= BEM_KBATCH_MAX_WAVENUMBERS). Regular pairs run in one persistent
+ * launch that evaluates the point geometry of a pair once for all wavenumbers; every entry is
+ * stored (no prior zeroing). Then, MIXED mode only, the FP64 near-field pairs among near_pairs
+ * (as bem_near_assemble), and the FP64 singular pairs (as bem_singular_assemble) are added.
+ * precision: BEM_PRECISION_MIXED with out_dtype BEM_C64, or BEM_PRECISION_FP64 with BEM_C128.
+ * The regular rule must have 6 points (order 4). `parts` (BEM_KBATCH_* bits) selects which of
+ * the three launches run, so that a caller can time them separately; the regular launch must
+ * precede the others on the stream.
+ */
+int bem_kbatch_assemble(const bem_space_t *test, const bem_space_t *trial, int grids_identical,
+                        const bem_quad_rule_t *rule, int nk, const double *wavenumbers, int precision, int parts,
+                        double mixed_near_factor, const bem_kbatch_config_t *config, void *out, int64_t ld,
+                        int64_t matrix_stride, int out_dtype, int n_near, const int32_t *near_pairs, int n_singular,
+                        const int32_t *singular_pairs, const double *test_points, const double *trial_points,
+                        const double *weights, void *stream);
+
 /* Set `bytes` bytes at `ptr` to zero with a fill kernel (safe inside graph capture). */
 int bem_fill_zero(void *ptr, size_t bytes, void *stream);
 

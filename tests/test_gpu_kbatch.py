@@ -1,0 +1,167 @@
+"""Frequency-batched Helmholtz single layer (DP0 x DP0) on the GPU.
+
+Every matrix of a sweep must equal the single-wavenumber assembly of the same operator:
+FP64 sweep vs FP64 single-k within 1e-12 and vs the FP64 oracle within 1e-12; the mixed
+sweep within 1e-6 of the FP64 single-k assembly (relative Frobenius error, per wavenumber).
+The launch shape must not change any value, and the autotuner must cache its choice.
+"""
+
+import json
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_fro(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def ico2():
+    from bempp_amd.api.grid.shapes import icosphere
+
+    return icosphere(2)
+
+
+@pytest.fixture(scope="module")
+def dp0(ico2):
+    from bempp_amd.api import function_space
+
+    return function_space(ico2, "DP", 0)
+
+
+KS = np.array([1.0, 3.7, 8.0, 16.0])
+
+
+def _single(space, k, precision, params):
+    from bempp_amd.api.operators.boundary.helmholtz import single_layer
+
+    return single_layer(space, space, space, k, parameters=params, precision=precision).weak_form().to_dense()
+
+
+def test_fp64_sweep_matches_single_wavenumber_and_oracle(ico2, dp0, default_parameters, hip_available):
+    from bempp_amd.core.frequency_sweep import hip_kbatch_assemble
+    from oracle import oracle as O
+
+    mats = hip_kbatch_assemble(dp0, dp0, KS, "double", parameters=default_parameters, config=(16, 4)).cpu().numpy()
+    assert mats.dtype == np.complex128 and mats.shape == (len(KS), 320, 320)
+    for m, k in enumerate(KS):
+        assert rel_fro(mats[m], _single(dp0, k, "double", default_parameters)) < 1e-12
+    ref = O.assemble("helmholtz_single_layer", ico2.vertices, ico2.elements, dp0, dp0, KS[-1])
+    assert rel_fro(mats[-1], ref) < 1e-12
+
+
+# Sweeps chosen so that the far, near and singular pairs use every expansion order bucket and
+# the direct fallback: the order grows with k_max times the element size (level 2: h ~ 0.3,
+# level 3: h ~ 0.15).
+SWEEPS = [
+    (2, np.linspace(0.1, 1.0, 8)),
+    (2, np.linspace(1.0, 4.0, 8)),
+    (2, np.linspace(1.0, 16.0, 32)),
+    (3, np.linspace(1.0, 16.0, 32)),
+]
+
+
+@pytest.mark.parametrize("level,ks", SWEEPS, ids=["L2_k0.1-1", "L2_k1-4", "L2_k1-16", "L3_k1-16"])
+def test_mixed_sweep_vs_fp64(level, ks, default_parameters, hip_available):
+    import torch
+    from bempp_amd.api import function_space
+    from bempp_amd.api.grid.shapes import icosphere
+    from bempp_amd.core.dense_assembler import hip_dense_assemble
+    from bempp_amd.api.operators.boundary.helmholtz import single_layer
+    from bempp_amd.core.frequency_sweep import hip_kbatch_assemble
+
+    sp = function_space(icosphere(level), "DP", 0)
+    mats = hip_kbatch_assemble(sp, sp, ks, "single", parameters=default_parameters, config=(16, 4))
+    assert mats.dtype == torch.complex64
+    errs = []
+    for m, k in enumerate(ks):
+        op = single_layer(sp, sp, sp, k, parameters=default_parameters, precision="double")
+        ref = hip_dense_assemble(sp, sp, default_parameters, op.descriptor)
+        errs.append((torch.linalg.norm(mats[m].to(torch.complex128) - ref) / torch.linalg.norm(ref)).item())
+    assert max(errs) < 1e-6, errs
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_launch_shape_does_not_change_values(dp0, default_parameters, precision, hip_available):
+    import torch
+    from bempp_amd.core.frequency_sweep import hip_kbatch_assemble
+
+    base = hip_kbatch_assemble(dp0, dp0, KS, precision, parameters=default_parameters, config=(16, 4))
+    for cfg in [(8, 1), (32, 8), (16, 16)]:
+        other = hip_kbatch_assemble(dp0, dp0, KS, precision, parameters=default_parameters, config=cfg)
+        torch.cuda.synchronize()
+        assert torch.equal(base, other), cfg
+
+
+def test_api_sweep_matches_weak_forms(dp0, default_parameters, hip_available):
+    from bempp_amd.api.operators.boundary.helmholtz import single_layer_frequency_sweep
+
+    ks = [2.0, 5.5]
+    ops = single_layer_frequency_sweep(dp0, dp0, dp0, ks, parameters=default_parameters, precision="double")
+    assert len(ops) == 2
+    for op, k in zip(ops, ks):
+        assert rel_fro(op.to_dense(), _single(dp0, k, "double", default_parameters)) < 1e-12
+
+
+def _dp0_brute_force(trial_grid, test_grid, k):
+    """DP0 x DP0 Helmholtz single layer of two disjoint grids: the plain 6 x 6 point sum."""
+    from bempp_amd.api.integration.triangle_gauss import rule
+
+    pts, w = rule(4)
+
+    def points(grid):
+        v, e = grid.vertices, grid.elements
+        v0, v1, v2 = v[:, e[0]].T, v[:, e[1]].T, v[:, e[2]].T
+        x = v0[:, None, :] + (v1 - v0)[:, None, :] * pts[0][None, :, None] + (v2 - v0)[:, None, :] * pts[1][None, :, None]
+        return x, grid.integration_elements[:, None] * w[None, :]
+
+    x, wx = points(test_grid)
+    y, wy = points(trial_grid)
+    r = np.linalg.norm(x[:, None, :, None, :] - y[None, :, None, :, :], axis=-1)
+    g = np.exp(1j * k * r) / (4 * np.pi * r)
+    return np.einsum("ep,fq,efpq->ef", wx, wy, g)
+
+
+def test_sweep_on_different_grids(ico2, default_parameters, hip_available):
+    """Test and trial spaces on different grids: every pair is regular."""
+    from bempp_amd.api import function_space
+    from bempp_amd.api.grid.grid import Grid
+    from bempp_amd.api.grid.shapes import icosphere
+    from bempp_amd.core.frequency_sweep import hip_kbatch_assemble
+
+    small = icosphere(1)
+    g2 = Grid(small.vertices * 1.5, small.elements)
+    dom = function_space(g2, "DP", 0)
+    dual = function_space(ico2, "DP", 0)
+    ks = [3.0, 7.5]
+    ref = [_dp0_brute_force(g2, ico2, k) for k in ks]
+    m64 = hip_kbatch_assemble(dom, dual, ks, "double", parameters=default_parameters, config=(8, 2)).cpu().numpy()
+    m32 = hip_kbatch_assemble(dom, dual, ks, "single", parameters=default_parameters, config=(8, 2)).cpu().numpy()
+    for m in range(len(ks)):
+        assert rel_fro(m64[m], ref[m]) < 1e-12
+        assert rel_fro(m32[m].astype(np.complex128), ref[m]) < 1e-6
+
+
+def test_autotuner_caches_choice(dp0, default_parameters, tmp_path, monkeypatch, hip_available):
+    from bempp_amd.core import autotune
+    from bempp_amd.core.frequency_sweep import hip_kbatch_assemble
+
+    cache = tmp_path / "tune.json"
+    monkeypatch.setenv("BEMPP_AMD_TUNE_CACHE", str(cache))
+    autotune.clear_memory()
+    ks = np.linspace(1.0, 16.0, 8)
+    tuned = hip_kbatch_assemble(dp0, dp0, ks, "single", parameters=default_parameters).cpu().numpy()
+    data = json.loads(cache.read_text())
+    assert len(data) == 1
+    entry = next(iter(data.values()))
+    assert len(entry["all"]) == len(autotune.CANDIDATE_TILES) * len(autotune.CANDIDATE_WG_PER_CU)
+    cfg = tuple(entry["config"])
+    autotune.clear_memory()
+    key = next(iter(data))
+    assert autotune.lookup(key) == cfg
+    fixed = hip_kbatch_assemble(dp0, dp0, ks, "single", parameters=default_parameters, config=cfg).cpu().numpy()
+    assert np.array_equal(tuned, fixed)
+    autotune.clear_memory()

@@ -18,7 +18,8 @@ def declared_functions():
 def test_header_declares_entry_points():
     names = declared_functions()
     for required in ["bem_regular_assemble", "bem_singular_assemble", "bem_potential_evaluate",
-                     "bem_potential_workspace_size", "bem_fill_zero", "bem_last_error", "bem_version"]:
+                     "bem_potential_workspace_size", "bem_fill_zero", "bem_last_error", "bem_version",
+                     "bem_kbatch_assemble"]:
         assert required in names
 
 
@@ -50,3 +51,12 @@ def test_error_path_without_gpu():
     rc = L.bem_potential_evaluate(2, ctypes.byref(sp), ctypes.c_void_p(16), 1, ctypes.c_void_p(16),
                                   ctypes.byref(rule), 0.0, 0.0, 0, ctypes.c_void_p(16), 3, ctypes.c_void_p(16), 0, None)
     assert rc == 2  # adjoint double layer has no potential: KeyError class
+    ks = (ctypes.c_double * 1)(1.0)
+    rc = L.bem_kbatch_assemble(ctypes.byref(sp), ctypes.byref(sp), 1, ctypes.byref(rule), 1, ks, 1, 7, 2.0, None,
+                               ctypes.c_void_p(16), 1, 1, 2, 0, None, 0, None, None, None, None, None)
+    assert rc == 1
+    assert b"null" in L.bem_last_error()
+    cfg = hip_backend.BemKBatchConfig(16, 4)
+    rc = L.bem_kbatch_assemble(ctypes.byref(sp), ctypes.byref(sp), 1, ctypes.byref(rule), 65, ks, 1, 7, 2.0,
+                               ctypes.byref(cfg), ctypes.c_void_p(16), 1, 1, 2, 0, None, 0, None, None, None, None, None)
+    assert rc != 0
