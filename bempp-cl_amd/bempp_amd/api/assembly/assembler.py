@@ -58,6 +58,7 @@ class PotentialAssembler(object):
     def __init__(self, space, points, operator_descriptor, device_interface, assembler, parameters):
         self.space = space
         self.points = points
+        self.descriptor = operator_descriptor
         self.kernel_dimension = operator_descriptor.kernel_dimension
         self._is_complex = operator_descriptor.is_complex
         self._implementation = select_potential_implementation(

@@ -27,3 +27,7 @@ class PotentialOperator(object):
     @property
     def kernel_dimension(self):
         return self._assembler.kernel_dimension
+
+    @property
+    def descriptor(self):
+        return self._assembler.descriptor

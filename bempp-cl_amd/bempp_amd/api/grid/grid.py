@@ -56,12 +56,30 @@ class Grid(object):
             domain_indices = _np.zeros(self._elements.shape[1], dtype=_np.uint32)
         self._domain_indices = _np.asarray(domain_indices, dtype=_np.uint32)
         self._id = grid_id if grid_id else f"grid{next(_ID_COUNTER)}"
+        self._topology_id = self._id
         self._data = {}
 
         self._enumerate_edges()
         self._compute_geometry()
         self._compute_adjacency()
         self._compute_boundary_information()
+
+    def with_vertices(self, vertices):
+        """A new grid with the same elements and moved vertices.
+
+        Edges, adjacency and boundary information depend on the topology only and are
+        shared; the geometry (jacobians, normals, diameters, ...) is recomputed.
+        """
+        v = _np.asfortranarray(_np.asarray(vertices, dtype=_np.float64))
+        if v.shape != self._vertices.shape:
+            raise ValueError(f"vertices must have shape {self._vertices.shape}")
+        g = object.__new__(Grid)
+        g.__dict__.update(self.__dict__)
+        g._vertices = v
+        g._id = f"grid{next(_ID_COUNTER)}"
+        g._data = {}
+        g._compute_geometry()
+        return g
 
     # ------------------------------------------------------------------ setup
     def _enumerate_edges(self):
@@ -175,6 +193,11 @@ class Grid(object):
         self._vertex_on_boundary[self._edges[:, self._edge_on_boundary].ravel()] = True
 
     # ------------------------------------------------------------- properties
+    @property
+    def topology_id(self):
+        """Identifier shared by grids with the same elements (see with_vertices)."""
+        return self._topology_id
+
     @property
     def id(self):
         return self._id

@@ -121,6 +121,12 @@ def lib():
             i, i, P, i, P, P, P, P, P,
         ]
         L.bem_kbatch_assemble.restype = i
+        L.bem_adjoint_single_layer.argtypes = [SP, QP, d, i, d, P, P, i, P, i, P, P, P, P, P, P]
+        L.bem_adjoint_single_layer.restype = i
+        L.bem_adjoint_potential_single_layer.argtypes = [SP, QP, d, P, i, P, P, P, P, P]
+        L.bem_adjoint_potential_single_layer.restype = i
+        L.bem_dense_matvec.argtypes = [i, P, i, i64, i64, i64, P, P, P]
+        L.bem_dense_matvec.restype = i
         _LIB = L
     return _LIB
 
@@ -345,3 +351,49 @@ def kbatch_assemble(test, trial, grids_identical, rule, wavenumbers, precision, 
             n_sing, ptr(sp), ptr(tp), ptr(spp), ptr(w), stream_handle(),
         )
     )
+
+
+def adjoint_single_layer(space, rule, wavenumber, precision, near_factor, x, y, grad, near_pairs=None,
+                         singular=None):
+    """Add the vertex gradient of Re(x^H V y) to grad ((n_vertices, 3) float64 device tensor)."""
+    n_near = 0 if near_pairs is None else int(near_pairs.shape[0])
+    if singular is None:
+        n_sing, sp, tp, spp, w = 0, None, None, None, None
+    else:
+        n_sing, sp, tp, spp, w = int(singular.pairs.shape[0]), singular.pairs, singular.test_points, \
+            singular.trial_points, singular.weights
+    check(
+        lib().bem_adjoint_single_layer(
+            ctypes.byref(space.struct), ctypes.byref(rule), float(wavenumber), precision, float(near_factor), ptr(x),
+            ptr(y), n_near, ptr(near_pairs), n_sing, ptr(sp), ptr(tp), ptr(spp), ptr(w), ptr(grad), stream_handle(),
+        )
+    )
+
+
+def adjoint_potential_single_layer(space, rule, wavenumber, coefficients, points, cotangent, grad_coefficients=None,
+                                   grad_vertices=None):
+    """Add P^H g to grad_coefficients and the vertex gradient of Re(g^H P c) to grad_vertices."""
+    check(
+        lib().bem_adjoint_potential_single_layer(
+            ctypes.byref(space.struct), ctypes.byref(rule), float(wavenumber), ptr(coefficients),
+            int(points.shape[0]), ptr(points), ptr(cotangent), ptr(grad_coefficients), ptr(grad_vertices),
+            stream_handle(),
+        )
+    )
+
+
+def dense_matvec(A, x, trans=0, out=None):
+    """op(A) x with FP64 accumulation; A: (rows, cols) device tensor, x: complex128 vector."""
+    torch = _torch()
+    if A.dim() != 2 or A.stride(1) != 1:
+        raise ValueError("A must be a row-major 2-d tensor")
+    rows, cols = A.shape
+    n_out = rows if trans == 0 else cols
+    n_in = cols if trans == 0 else rows
+    if x.shape != (n_in,) or x.dtype != torch.complex128 or not x.is_contiguous():
+        raise ValueError(f"x must be a contiguous complex128 vector of length {n_in}")
+    if out is None:
+        out = torch.empty(n_out, dtype=torch.complex128, device=A.device)
+    check(lib().bem_dense_matvec(int(trans), ptr(A), _dtype_code(A), rows, cols, A.stride(0), ptr(x), ptr(out),
+                                 stream_handle()))
+    return out

@@ -98,7 +98,8 @@ class DeviceSingularRule(object):
 
 
 def singular_rule(grid, order, test_support, trial_support, on_device=True):
-    key = (grid.id, int(order), test_support.tobytes(), trial_support.tobytes(), on_device)
+    # the pairs and the reference-coordinate points depend on the topology only
+    key = (grid.topology_id, int(order), test_support.tobytes(), trial_support.tobytes(), on_device)
     if key not in _CACHE:
         rule = SingularRule(grid, order, _np.asarray(test_support, bool), _np.asarray(trial_support, bool))
         _CACHE[key] = DeviceSingularRule(rule) if on_device else rule

@@ -23,6 +23,7 @@
 // Regular pairs run in one persistent launch: gridDim.x workgroups walk the
 // (test tile x trial block) space with a static stride. Each lane owns one trial element;
 // the test elements of the tile are staged in LDS and read as broadcasts.
+#include "far_split.h"
 #include "regular_impl.h"
 
 namespace bem {
@@ -58,20 +59,6 @@ __device__ __forceinline__ int taylor_order(T x, T tol, int nmax) {
         term *= x / (T)(n + 1);
     }
     return n;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
 }
 
 // sum_n (i k)^n c_n (c_n already divided by n!): even and odd parts by Horner in k^2
@@ -121,25 +108,6 @@ struct TrialF32Lane {
     float wq[kQ];  // w_q J_f / (4 pi)
 };
 
-// geometry of a far pair in the reference-distance split of the single-wavenumber kernel
-struct FarPair {
-    float fd0, fd1, fd2;  // centroid offset d = c_f - c_e (FP32)
-    float r0, r0sq, resid;
-};
-
-__device__ __forceinline__ FarPair far_pair_setup(const double *ce, const double *cf) {
-    FarPair P;
-    const double d0 = cf[0] - ce[0], d1 = cf[1] - ce[1], d2 = cf[2] - ce[2];
-    const double dist2 = d0 * d0 + d1 * d1 + d2 * d2;
-    P.r0 = sqrtf((float)dist2);
-    P.r0sq = P.r0 * P.r0;
-    P.resid = (float)(dist2 - (double)P.r0 * (double)P.r0);
-    P.fd0 = (float)d0;
-    P.fd1 = (float)d1;
-    P.fd2 = (float)d2;
-    return P;
-}
-
 // visit the 36 point pairs: f(amplitude w w J J / (4 pi r), delta = r - r0)
 template <int TM, typename F>
 __device__ __forceinline__ void far_points(const TileF32<TM> &T_, int t, const TrialF32Lane &L, const FarPair &P, F &&f) {
@@ -167,14 +135,6 @@ __device__ __forceinline__ void store_c64(void *out, int64_t idx, float re, floa
     *reinterpret_cast<float2 *>(reinterpret_cast<float *>(out) + 2 * idx) = v;
 }
 
-// e^{i k r0} with the phase reduced to one revolution in FP64
-__device__ __forceinline__ void phase_f32(double k, double r0rev, float &c, float &s) {
-    double rev = k * r0rev;
-    rev -= rint(rev);
-    c = __builtin_amdgcn_cosf((float)rev);
-    s = __builtin_amdgcn_sinf((float)rev);
-}
-
 template <int NM, int TM>
 __device__ __forceinline__ void far_pair_moments(const Args &a, const TileF32<TM> &T_, int t, const TrialF32Lane &L,
                                                  const FarPair &P, bool skip, float m, bool valid, int64_t base) {
@@ -197,7 +157,7 @@ __device__ __forceinline__ void far_pair_moments(const Args &a, const TileF32<TM
         const double kd = a.k[km];
         float re, im, cs, sn;
         taylor_eval<NM>(c, (float)kd, re, im);
-        phase_f32(kd, r0rev, cs, sn);
+        phase_rev_f32(kd, r0rev, cs, sn);
         if (valid) store_c64(a.out, km * a.kstride + base, cs * re - sn * im, cs * im + sn * re);
     }
 }
@@ -219,7 +179,7 @@ __device__ __forceinline__ void far_pair_direct(const Args &a, const TileF32<TM>
             });
         }
         float cs, sn;
-        phase_f32(kd, r0rev, cs, sn);
+        phase_rev_f32(kd, r0rev, cs, sn);
         if (valid) store_c64(a.out, km * a.kstride + base, m * (cs * sr - sn * si), m * (cs * si + sn * sr));
     }
 }
@@ -288,7 +248,7 @@ __global__ __launch_bounds__(kRegThreads) void kbatch_far_kernel(const Args a) {
                 L.wq[q] = 0.f;
             }
         }
-        const float rho_fw = wave_max(sqrtf(rho2_f));
+        const float rho_fw = wave_max64(sqrtf(rho2_f));
         const int64_t col = valid ? a.trial.l2g[f] : 0;
 
         for (int t = 0; t < ntile; ++t) {
@@ -540,7 +500,7 @@ __global__ __launch_bounds__(256) void kbatch_near_kernel(const Args a, int n_pa
             x = a.kmax * (quad_radius(P.gt, a.rule) + quad_radius(P.gf, a.rule)) * 1.0001;
         }
     }
-    const int N = wave_max(skip ? 0 : taylor_order(x, kTol64, 32));
+    const int N = wave_max64(skip ? 0 : taylor_order(x, kTol64, 32));
     const int bucket = __builtin_amdgcn_readfirstlane(N <= 8 ? 0 : N <= 16 ? 1 : N <= 24 ? 2 : N <= 32 ? 3 : 4);
     switch (bucket) {
     case 0: near_pair_moments<8>(a, P, skip); break;
@@ -591,7 +551,7 @@ __device__ __forceinline__ void singular_pair_moments(const Args &a, const Singu
         }
     });
 #pragma unroll
-    for (int n = 0; n <= NM; ++n) c[n] = wave_sum(c[n]);
+    for (int n = 0; n <= NM; ++n) c[n] = wave_sum64(c[n]);
     scale_factorials<NM>(c, P.scale);
     // lane m evaluates wavenumber m
     for (int km = lane; km < a.nk; km += 64) {
@@ -614,8 +574,8 @@ __device__ __forceinline__ void singular_pair_direct(const Args &a, const Singul
             sr += c * g;
             si += s * g;
         });
-        sr = wave_sum(sr);
-        si = wave_sum(si);
+        sr = wave_sum64(sr);
+        si = wave_sum64(si);
         if (lane == 0) add_c(a.out, km * a.kstride + P.base, (OutT)(sr * P.scale), (OutT)(si * P.scale));
     }
 }

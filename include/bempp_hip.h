@@ -11,8 +11,8 @@
  *   bem_potential_evaluate <- potential_assembler(...) bempp_cl/core/numba_assemblers.py:115-164,
  *                                                      bempp_cl/core/opencl_assemblers.py:287-465
  * Extra entry points with no reference counterpart: bem_kbatch_assemble (many Helmholtz
- * wavenumbers in one persistent launch), bem_adjoint_* (gradients), bem_fill_zero,
- * bem_potential_workspace_size, bem_last_error, bem_version.
+ * wavenumbers in one persistent launch), bem_adjoint_* (gradients), bem_dense_matvec,
+ * bem_fill_zero, bem_potential_workspace_size, bem_last_error, bem_version.
  *
  * Conventions
  *  - All array arguments are DEVICE pointers (hipMalloc'ed or torch device memory) unless
@@ -216,6 +216,45 @@ int bem_kbatch_assemble(const bem_space_t *test, const bem_space_t *trial, int g
                         int64_t matrix_stride, int out_dtype, int n_near, const int32_t *near_pairs, int n_singular,
                         const int32_t *singular_pairs, const double *test_points, const double *trial_points,
                         const double *weights, void *stream);
+
+/*
+ * Vertex gradient of the Helmholtz single layer (DP0 x DP0 on one space; no reference
+ * counterpart, the reference has no derivatives). Adds to grad_vertices ([n_vertices * 3],
+ * float64) the gradient with respect to the vertex coordinates of
+ *     Re sum_{e,f} conj(x_e) V_ef y_f,
+ * V the Galerkin matrix of bempp_cl/core/numba_kernels.py:298-320 (regular pairs, 980-1074)
+ * with the singular pairs of bem_singular_assemble. x, y: complex128 [global dofs]. With x the
+ * cotangent of V c and y = c this is the vertex part of the backward of c -> V c.
+ * precision: BEM_PRECISION_MIXED (FP32 far pairs, FP64 near pairs among near_pairs and FP64
+ * singular pairs, as the forward mixed mode) or BEM_PRECISION_FP64. k: real wavenumber
+ * (k = 0 is the Laplace single layer). The regular rule must have 6 points.
+ */
+int bem_adjoint_single_layer(const bem_space_t *space, const bem_quad_rule_t *rule, double k, int precision,
+                             double mixed_near_factor, const void *x, const void *y, int n_near,
+                             const int32_t *near_pairs, int n_singular, const int32_t *singular_pairs,
+                             const double *test_points, const double *trial_points, const double *weights,
+                             double *grad_vertices, void *stream);
+
+/*
+ * Backward of the DP0 single-layer potential u = P c at n_points points (FP64;
+ * bempp_cl/core/numba_kernels.py:1788-1849 is the forward): for the cotangent g of u
+ * (complex128 [n_points]) adds P^H g to grad_coefficients (complex128 [global dofs], may be
+ * NULL) and the vertex gradient of Re sum_p conj(g_p) u_p to grad_vertices (may be NULL).
+ * coefficients: complex128 [global dofs]; points: float64 [3 * n_points].
+ */
+int bem_adjoint_potential_single_layer(const bem_space_t *space, const bem_quad_rule_t *rule, double k,
+                                       const void *coefficients, int n_points, const double *points,
+                                       const void *cotangent, void *grad_coefficients, double *grad_vertices,
+                                       void *stream);
+
+/*
+ * y = op(A) x for a dense row-major matrix in HBM (the product of the reference's
+ * DenseDiscreteBoundaryOperator, bempp_cl/api/assembly/discrete_boundary_operator.py).
+ * trans: 0 = A x, 1 = A^T x, 2 = A^H x. a_dtype: enum bem_dtype. x, y: complex128 device
+ * vectors (y is overwritten). Sums are accumulated in FP64.
+ */
+int bem_dense_matvec(int trans, const void *A, int a_dtype, int64_t rows, int64_t cols, int64_t ld, const void *x,
+                     void *y, void *stream);
 
 /* Set `bytes` bytes at `ptr` to zero with a fill kernel (safe inside graph capture). */
 int bem_fill_zero(void *ptr, size_t bytes, void *stream);
