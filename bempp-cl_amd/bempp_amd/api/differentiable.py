@@ -44,6 +44,7 @@ class SingleLayerProblem(object):
         self._topology = None
         self._Grid = Grid
         self._last = None
+        self._space = None
 
     def grid(self, vertices):
         """Grid of this topology at the given (n_vertices, 3) positions (tensor or array)."""
@@ -62,6 +63,12 @@ class SingleLayerProblem(object):
             g = self._topology.with_vertices(v.T)
         self._last = (v.copy(), g)
         return g
+
+    def space(self, grid):
+        """The DP0 space of a grid from `grid` (kept for the last grid, so device copies are reused)."""
+        if self._space is None or self._space.grid is not grid:
+            self._space = _dp0_space(grid)
+        return self._space
 
     def operator(self, space):
         from bempp_amd.api.operators.boundary import helmholtz, laplace
@@ -123,7 +130,7 @@ def _make_functions():
             from bempp_amd.core.dense_assembler import hip_dense_assemble
 
             grid = problem.grid(vertices)
-            space = _dp0_space(grid)
+            space = problem.space(grid)
             dev = hb.device()
             c = coefficients.detach().to(dev, torch.complex128).contiguous()
             if c.shape != (space.global_dof_count,):
@@ -165,7 +172,7 @@ def _make_functions():
             from bempp_amd.core.dense_potential_assembler import hip_potential_assembler
 
             grid = problem.grid(vertices)
-            space = _dp0_space(grid)
+            space = problem.space(grid)
             dev = hb.device()
             c = coefficients.detach().to(dev, torch.complex128).contiguous()
             if c.shape != (space.global_dof_count,):

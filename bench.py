@@ -51,9 +51,192 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device to device (reference dispatch for HBM byte counters)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the frequency-sweep measurement")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the loss/gradient measurement")
+    ap.add_argument("--adjoint-level", type=int, default=6)
+    ap.add_argument("--adjoint-k", type=float, default=8.0)
+    ap.add_argument("--fd-vertices", type=int, default=32)
     ap.add_argument("--sweep-level", type=int, default=4)
     ap.add_argument("--sweep-reps", type=int, default=3)
     return ap.parse_args()
+
+
+def _ev():
+    import torch
+
+    return torch.cuda.Event(enable_timing=True)
+
+
+def adjoint_bench(level, k, fd_vertices):
+    """Loss L = ||V c - b||^2 + sum_p |u(p)|^2 on a perturbed sphere and its gradient.
+
+    V: DP0 Helmholtz single layer (mixed FP32/FP64, complex64 in HBM), u: its potential at 4096
+    points on the radius-2 sphere. Forward and backward (torch.autograd; backward = adjoint
+    kernels + V^H g) are timed on the device after one warm-up pass. The mixed loss and
+    gradients are compared with an FP64 evaluation, and the FP64 adjoint gradient with central
+    finite differences (h = 1e-6) of the FP64 loss on `fd_vertices` random vertices (all three
+    coordinates), each by re-assembling only the rows and columns of the elements at the
+    vertex (dL from the changed entries, so that the large constant part of L cancels exactly).
+    """
+    import numpy as np
+    import torch
+
+    import bempp_amd.api as bempp
+    from bempp_amd.api.differentiable import (SingleLayerProblem, _device_rule, single_layer_apply,
+                                              single_layer_potential)
+    from bempp_amd.api.grid.shapes import perturbed_icosphere
+    from bempp_amd.api.space.space import function_space
+    from bempp_amd.core import clear_caches
+    from bempp_amd.core import hip_backend as hb
+    from bempp_amd.core.dense_assembler import hip_dense_assemble
+    from bempp_amd.core.singular_assembler import singular_rule
+
+    grid = perturbed_icosphere(level)
+    F, NV = grid.number_of_elements, grid.number_of_vertices
+    rng5, rng6, rng7 = np.random.default_rng(5), np.random.default_rng(6), np.random.default_rng(7)
+    pts = rng5.standard_normal((4096, 3))
+    pts = 2.0 * pts / np.linalg.norm(pts, axis=1)[:, None]
+    c0 = (rng6.standard_normal(F) + 1j * rng6.standard_normal(F)) / np.sqrt(2)
+    b0 = (rng6.standard_normal(F) + 1j * rng6.standard_normal(F)) / np.sqrt(2)
+    dev = hb.device()
+    pts_t = torch.from_numpy(pts).to(dev)
+    b = torch.from_numpy(b0).to(dev)
+    v_host = np.ascontiguousarray(grid.vertices.T)
+    params = bempp.GLOBAL_PARAMETERS
+    prob = SingleLayerProblem(grid.elements, k, precision="single", parameters=params)
+
+    def run():
+        v = torch.from_numpy(v_host).to(dev).requires_grad_(True)
+        c = torch.from_numpy(c0).to(dev).requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        y = single_layer_apply(v, c, prob)
+        u = single_layer_potential(v, c, pts_t, prob)
+        L = (y - b).abs().pow(2).sum() + u.abs().pow(2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return float(L.detach()), v.grad.detach(), c.grad.detach(), t1 - t0, t2 - t1
+
+    run()  # warm-up: host set-up (near-field candidates, singular pairs), code objects
+    L32, gv32, gc32, t_fwd, t_bwd = run()
+    log(f"adjoint L{level}: forward {t_fwd:.3f} s, backward {t_bwd:.3f} s")
+
+    # kernel-level timing of the regular adjoint launch (events on the launch stream)
+    space = prob.space(prob.grid(v_host))
+    dsp = hb.device_space(space)
+    rule = _device_rule(params)
+    g_y = torch.from_numpy(c0).to(dev)  # any cotangent: the kernel work does not depend on it
+    cvec = torch.from_numpy(c0).to(dev)
+    grad = torch.zeros((NV, 3), dtype=torch.float64, device=dev)
+    e0, e1 = _ev(), _ev()
+    e0.record()
+    hb.adjoint_single_layer(dsp, rule, k, hb.PRECISION_MIXED, 2.0, g_y, cvec, grad)
+    e1.record()
+    torch.cuda.synchronize()
+    adj_ms = e0.elapsed_time(e1)
+    n_sing = singular_rule(grid, params.quadrature.singular, space.support, space.support).host.number_of_pairs
+    n_evals = (F * F - n_sing) * 36
+    del grad
+
+    # FP64 reference: loss and gradients (manual forward/backward, keeping V for the FD check)
+    sp64 = function_space(grid, "DP", 0)
+    from bempp_amd.api.operators.boundary import helmholtz
+    from bempp_amd.api.operators.potential import helmholtz as hpot
+    from bempp_amd.core.dense_potential_assembler import hip_potential_assembler
+
+    A64 = hip_dense_assemble(sp64, sp64, params, helmholtz.single_layer(sp64, sp64, sp64, k, precision="double").descriptor)
+    c = torch.from_numpy(c0).to(dev)
+    r = hb.dense_matvec(A64, c) - b
+    pot_desc = hpot.single_layer(sp64, pts.T, k, precision="double").descriptor
+    u64, _ = hip_potential_assembler(sp64, pot_desc, pts.T, params)(c, as_tensor=True)
+    u64 = u64[0]
+    L64 = float((r.abs().pow(2).sum() + u64.abs().pow(2).sum()).item())
+    gc64 = hb.dense_matvec(A64, (2 * r).contiguous(), trans=2)
+    gv64 = torch.zeros((NV, 3), dtype=torch.float64, device=dev)
+    cand = None
+    srule = singular_rule(grid, params.quadrature.singular, sp64.support, sp64.support)
+    dsp64 = hb.device_space(sp64)
+    hb.adjoint_single_layer(dsp64, rule, k, hb.PRECISION_FP64, 2.0, (2 * r).contiguous(), c, gv64, singular=srule)
+    hb.adjoint_potential_single_layer(dsp64, rule, k, c, pts_t, (2 * u64).contiguous(), grad_coefficients=gc64,
+                                      grad_vertices=gv64)
+    torch.cuda.synchronize()
+    res = {
+        "workload": f"Helmholtz single layer DP0, perturbed icosphere level {level} ({F} triangles, {NV} vertices), "
+                    f"k={k}, loss ||Vc-b||^2 + sum|u|^2 (u: potential at 4096 points, radius 2), mixed FP32/FP64",
+        "forward_s": t_fwd,
+        "backward_s": t_bwd,
+        "backward_over_forward": t_bwd / t_fwd,
+        "adjoint_regular_kernel_ms": adj_ms,
+        "adjoint_algorithmic_tflops": n_evals * 51 / (adj_ms * 1e-3) / 1e12,
+        "adjoint_evals": n_evals,
+        "loss_rel_err_vs_fp64": abs(L32 - L64) / abs(L64),
+        "grad_vertices_rel_err_vs_fp64": float(torch.linalg.norm(gv32.to(dev) - gv64) / torch.linalg.norm(gv64)),
+        "grad_c_rel_err_vs_fp64": float(torch.linalg.norm(gc32.to(dev) - gc64) / torch.linalg.norm(gc64)),
+    }
+
+    # central differences of the FP64 loss by partial re-assembly
+    if fd_vertices > 0:
+        t = time.perf_counter()
+        topo = prob.grid(v_host)
+        picks = rng7.choice(NV, fd_vertices, replace=False)
+        r_np = r
+        fd, ana_64, ana_32 = [], [], []
+        el = grid.elements
+        for vi in picks:
+            E = np.nonzero((el == vi).any(axis=0))[0]
+            Et = torch.from_numpy(E).to(dev)
+            cE = c[Et]
+            colsV = A64[:, Et]
+            # potential of the elements at vi, unperturbed
+            spE0 = function_space(grid, "DP", 0, support_elements=E)
+            ev0 = hip_potential_assembler(spE0, hpot.single_layer(spE0, pts.T, k, precision="double").descriptor,
+                                          pts.T, params)
+            uE0 = ev0(_full_dp0(spE0, cE), as_tensor=True)[0][0]
+            for d in range(3):
+                dl = []
+                for sgn in (1.0, -1.0):
+                    vv = v_host.copy()
+                    vv[vi, d] += sgn * 1e-6
+                    g2 = topo.with_vertices(vv.T)
+                    sp_full = function_space(g2, "DP", 0)
+                    spE = function_space(g2, "DP", 0, support_elements=E)
+                    dE = helmholtz.single_layer(sp_full, sp_full, spE, k, precision="double").descriptor
+                    R = hip_dense_assemble(sp_full, spE, params, dE)  # rows of E
+                    C = hip_dense_assemble(spE, sp_full, params, dE)  # columns of E
+                    delta = (C.to(torch.complex128) - colsV) @ cE
+                    delta[Et] = R @ c - b[Et] - r[Et]
+                    dr = (2 * (torch.conj(r) * delta).real + delta.abs().pow(2)).sum()
+                    evE = hip_potential_assembler(spE, hpot.single_layer(spE, pts.T, k, precision="double").descriptor,
+                                                  pts.T, params)
+                    du = evE(_full_dp0(spE, cE), as_tensor=True)[0][0] - uE0
+                    dpot = (2 * (torch.conj(u64) * du).real + du.abs().pow(2)).sum()
+                    dl.append(float((dr + dpot).item()))
+                    clear_caches()
+                fd.append((dl[0] - dl[1]) / 2e-6)
+                ana_64.append(float(gv64[vi, d]))
+                ana_32.append(float(gv32[vi, d]))
+        fd, ana_64, ana_32 = map(np.array, (fd, ana_64, ana_32))
+        res["fd_check"] = {
+            "vertices": int(fd_vertices),
+            "h": 1e-6,
+            "fp64_adjoint_rel_err_vs_fd": float(np.linalg.norm(ana_64 - fd) / np.linalg.norm(fd)),
+            "mixed_adjoint_rel_err_vs_fd": float(np.linalg.norm(ana_32 - fd) / np.linalg.norm(fd)),
+            "seconds": time.perf_counter() - t,
+        }
+    del A64
+    return res
+
+
+def _full_dp0(space, coeffs):
+    """Per-element coefficients of a DP0 space (device) from its dof vector."""
+    import numpy as np
+    import torch
+
+    l2g = torch.from_numpy(space.local2global[:, 0].astype(np.int64)).to(coeffs.device)
+    mult = torch.from_numpy(np.ascontiguousarray(space.local_multipliers[:, 0])).to(coeffs.device)
+    return (coeffs[l2g] * mult).contiguous()
 
 
 def frequency_sweep_bench(level, reps):
@@ -355,6 +538,13 @@ def main():
     if not args.no_sweep:
         result["frequency_sweep"] = frequency_sweep_bench(args.sweep_level, args.sweep_reps)
         log(f"frequency sweep {result['frequency_sweep']}")
+
+    if not args.no_adjoint:
+        del outs
+        torch.cuda.empty_cache()
+        result["adjoint"] = adjoint_bench(args.adjoint_level, args.adjoint_k, args.fd_vertices)
+        log(f"adjoint {result['adjoint']}")
+        torch.cuda.empty_cache()
 
     if not args.no_cpu_baseline:
         log("timing the CPU baseline ...")
