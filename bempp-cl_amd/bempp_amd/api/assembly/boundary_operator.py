@@ -104,3 +104,66 @@ class _ScaledBoundaryOperator(BoundaryOperator):
 
     def _assemble(self):
         return self._op.weak_form() * self._alpha
+
+
+def _fusion_key(op):
+    """Settings that must agree for two operators to be assembled in one pass."""
+    p = op.assembler.parameters
+    hip = getattr(p, "hip", None)
+    return (
+        p.quadrature.regular,
+        p.quadrature.singular,
+        p.assembly.always_promote_to_double,
+        getattr(hip, "mixed_near_field_factor", 2.0),
+        getattr(hip, "kahan", False),
+        op.assembler._device_interface,
+    )
+
+
+def assemble_weak_forms(operators):
+    """weak_form() of every operator in the list, in order.
+
+    A single layer and the double layer of the same family (Laplace or Helmholtz) with the same
+    spaces, wavenumber, precision and quadrature, both with the dense assembler, are assembled
+    together in one pass that computes every point pair's distance and e^{ikr} once (the
+    matrices are those weak_form() returns). The results are cached on the operators as
+    weak_form() caches them.
+    """
+    from bempp_amd.core.dense_assembler import DenseAssembler, can_fuse_sl_dl, hip_dense_assemble_sl_dl
+    from bempp_amd.core.dispatcher import check_device_interface
+
+    ops = list(operators)
+
+    def dense(op):
+        return (isinstance(op, BoundaryOperatorWithAssembler) and op._cached is None
+                and isinstance(op.assembler._implementation, DenseAssembler))
+
+    for i, a in enumerate(ops):
+        if not dense(a):
+            continue
+        for b in ops[i + 1:]:
+            if not dense(b) or _fusion_key(a) != _fusion_key(b):
+                continue
+            for sl, dl in ((a, b), (b, a)):
+                if can_fuse_sl_dl((sl.descriptor, sl.domain, sl.dual_to_range), (dl.descriptor, dl.domain, dl.dual_to_range)):
+                    check_device_interface(sl.assembler._device_interface)
+                    m1, m2 = hip_dense_assemble_sl_dl(sl.domain, sl.dual_to_range, sl.assembler.parameters,
+                                                      sl.descriptor, dl.descriptor)
+                    sl._cached = _wrap(m1, sl.assembler.parameters)
+                    dl._cached = _wrap(m2, dl.assembler.parameters)
+                    break
+            if a._cached is not None:
+                break
+    return [op.weak_form() for op in ops]
+
+
+def _wrap(mat, parameters):
+    import torch
+    from bempp_amd.api.assembly.discrete_boundary_operator import DenseDiscreteBoundaryOperator
+
+    if parameters.assembly.always_promote_to_double:
+        if mat.dtype == torch.float32:
+            mat = mat.double()
+        elif mat.dtype == torch.complex64:
+            mat = mat.to(torch.complex128)
+    return DenseDiscreteBoundaryOperator(mat)

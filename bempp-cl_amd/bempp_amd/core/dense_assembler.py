@@ -161,6 +161,101 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     return out
 
 
+_SL_DL = {
+    ("laplace_single_layer", "laplace_double_layer"),
+    ("helmholtz_single_layer", "helmholtz_double_layer"),
+}
+
+
+def can_fuse_sl_dl(first, second):
+    """True if the two (descriptor, domain, dual_to_range) triples form a fusable single/double pair."""
+    d1, dom1, dual1 = first
+    d2, dom2, dual2 = second
+    return (
+        (d1.assembly_type, d2.assembly_type) == ("default_scalar", "default_scalar")
+        and (d1.kernel_type, d2.kernel_type) in _SL_DL
+        and tuple(d1.options) == tuple(d2.options)
+        and d1.precision == d2.precision
+        and dom1 == dom2
+        and dual1 == dual2
+    )
+
+
+def hip_dense_assemble_sl_dl(domain, dual_to_range, parameters, sl_descriptor, dl_descriptor, out_single=None,
+                             out_double=None, events=None):
+    """Assemble a single layer and the double layer of the same family in one pass.
+
+    Both matrices equal those of two hip_dense_assemble calls (the same quadrature and
+    precision); every point pair's distance, 1/r and e^{ikr} are computed once for both.
+    Returns (single layer, double layer) device tensors. events: optional dict of
+    torch.cuda.Event with keys "regular_start", "regular_end", "singular_end".
+    """
+    import torch
+    from bempp_amd.api.integration.triangle_gauss import rule as triangle_rule
+    from bempp_amd.core import hip_backend as hb
+    from bempp_amd.core.singular_assembler import singular_rule
+
+    if not can_fuse_sl_dl((sl_descriptor, domain, dual_to_range), (dl_descriptor, domain, dual_to_range)):
+        raise ValueError("expected a single layer and the double layer of the same family, wavenumber and precision")
+    op = _kernel_name(sl_descriptor)
+    precision = sl_descriptor.precision
+    types = get_type(precision)
+    dtype = types.complex if sl_descriptor.is_complex else types.real
+    tdtype = getattr(torch, dtype)
+    rows, cols = dual_to_range.global_dof_count, domain.global_dof_count
+    dev = hb.device()
+    grids_identical = domain.grid == dual_to_range.grid
+    test = hb.device_space(dual_to_range)
+    trial = hb.device_space(domain)
+    points, weights = triangle_rule(parameters.quadrature.regular)
+    rule = hb.quad_rule(points, weights)
+    mode = hb.PRECISION_MIXED if precision == "single" else hb.PRECISION_FP64
+    direct = _is_localised_injective(domain) and _is_localised_injective(dual_to_range)
+    full_cover = (
+        direct
+        and dual_to_range.number_of_support_elements * dual_to_range.number_of_shape_functions == rows
+        and domain.number_of_support_elements * domain.number_of_shape_functions == cols
+    )
+    outs = []
+    for o in (out_single, out_double):
+        if o is None:
+            o = torch.empty((rows, cols), dtype=tdtype, device=dev)
+        elif tuple(o.shape) != (rows, cols) or o.dtype != tdtype or not o.is_contiguous():
+            raise ValueError(f"outputs must be contiguous ({rows}, {cols}) {dtype} tensors")
+        if not full_cover:
+            hb.fill_zero(o)
+        outs.append(o)
+    hip_opts = getattr(parameters, "hip", None)
+    near = getattr(hip_opts, "mixed_near_field_factor", 2.0) if hip_opts is not None else 2.0
+    kahan = getattr(hip_opts, "kahan", False) if hip_opts is not None else False
+    k = _wavenumber(sl_descriptor)
+
+    if events is not None:
+        events["regular_start"].record()
+    if direct:
+        hb.regular_assemble_sl_dl(op, test, trial, None, None, grids_identical, rule, k, mode, outs[0], outs[1],
+                                  accumulate=0, near_factor=near, kahan=kahan)
+    else:
+        from bempp_amd.core.blocking import device_blocks
+
+        hb.regular_assemble_sl_dl(op, test, trial, device_blocks(dual_to_range, "test"), device_blocks(domain, "trial"),
+                                  grids_identical, rule, k, mode, outs[0], outs[1], near_factor=near, kahan=kahan)
+    if events is not None:
+        events["regular_end"].record()
+    if mode == hb.PRECISION_MIXED:
+        from bempp_amd.core.near_field import device_near_candidates
+
+        cand = device_near_candidates(dual_to_range, domain, near)
+        hb.near_assemble_sl_dl(op, test, trial, cand, grids_identical, rule, k, near, outs[0], outs[1])
+    if grids_identical:
+        srule = singular_rule(dual_to_range.grid, parameters.quadrature.singular, dual_to_range.support, domain.support)
+        hb.singular_assemble_sl_dl(op, test, trial, srule.pairs, srule.test_points, srule.trial_points, srule.weights,
+                                   k, outs[0], outs[1])
+    if events is not None:
+        events["singular_end"].record()
+    return outs[0], outs[1]
+
+
 def assemble_dense(domain, dual_to_range, parameters, operator_descriptor, device_interface):
     """Assemble and return the dense matrix as a numpy array (dense_assembler.py:49-100)."""
     from bempp_amd.core.dispatcher import dense_assembler_dispatcher

@@ -127,6 +127,12 @@ def lib():
         L.bem_adjoint_potential_single_layer.restype = i
         L.bem_dense_matvec.argtypes = [i, P, i, i64, i64, i64, P, P, P]
         L.bem_dense_matvec.restype = i
+        L.bem_regular_assemble_sl_dl.argtypes = [i, SP, SP, BP, BP, i, QP, d, d, i, d, i, P, P, i64, i, i, P]
+        L.bem_regular_assemble_sl_dl.restype = i
+        L.bem_near_assemble_sl_dl.argtypes = [i, SP, SP, i, P, i, QP, d, d, d, P, P, i64, i, P]
+        L.bem_near_assemble_sl_dl.restype = i
+        L.bem_singular_assemble_sl_dl.argtypes = [i, SP, SP, i, P, P, P, P, d, d, P, P, i64, i, P]
+        L.bem_singular_assemble_sl_dl.restype = i
         _LIB = L
     return _LIB
 
@@ -397,3 +403,48 @@ def dense_matvec(A, x, trans=0, out=None):
     check(lib().bem_dense_matvec(int(trans), ptr(A), _dtype_code(A), rows, cols, A.stride(0), ptr(x), ptr(out),
                                  stream_handle()))
     return out
+
+
+def regular_assemble_sl_dl(op, test, trial, test_blocks, trial_blocks, grids_identical, rule, wavenumber, precision,
+                           out_single, out_double, accumulate=0, near_factor=2.0, kahan=False):
+    """Single layer (op) and its double layer in one pass; blocks None for localised spaces.
+
+    With blocks, the element lists of the blocks (their space_struct) replace those of test/trial.
+    """
+    k = complex(wavenumber)
+    tb = None if test_blocks is None else ctypes.byref(test_blocks.struct)
+    sb = None if trial_blocks is None else ctypes.byref(trial_blocks.struct)
+    ts = test.struct if test_blocks is None else test_blocks.space_struct
+    ss = trial.struct if trial_blocks is None else trial_blocks.space_struct
+    check(
+        lib().bem_regular_assemble_sl_dl(
+            OPERATORS[op], ctypes.byref(ts), ctypes.byref(ss), tb, sb, int(grids_identical),
+            ctypes.byref(rule), k.real, k.imag, precision, float(near_factor), int(bool(kahan)), ptr(out_single),
+            ptr(out_double), out_single.shape[-1], _dtype_code(out_single), int(accumulate), stream_handle(),
+        )
+    )
+
+
+def near_assemble_sl_dl(op, test, trial, pairs, grids_identical, rule, wavenumber, near_factor, out_single,
+                        out_double):
+    k = complex(wavenumber)
+    check(
+        lib().bem_near_assemble_sl_dl(
+            OPERATORS[op], ctypes.byref(test.struct), ctypes.byref(trial.struct), int(pairs.shape[0]), ptr(pairs),
+            int(grids_identical), ctypes.byref(rule), k.real, k.imag, float(near_factor), ptr(out_single),
+            ptr(out_double), out_single.shape[-1], _dtype_code(out_single), stream_handle(),
+        )
+    )
+
+
+def singular_assemble_sl_dl(op, test, trial, pairs, test_points, trial_points, weights, wavenumber, out_single,
+                            out_double):
+    k = complex(wavenumber)
+    n = 0 if pairs is None else pairs.shape[0]
+    check(
+        lib().bem_singular_assemble_sl_dl(
+            OPERATORS[op], ctypes.byref(test.struct), ctypes.byref(trial.struct), n, ptr(pairs), ptr(test_points),
+            ptr(trial_points), ptr(weights), k.real, k.imag, ptr(out_single), ptr(out_double), out_single.shape[-1],
+            _dtype_code(out_single), stream_handle(),
+        )
+    )

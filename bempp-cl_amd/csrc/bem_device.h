@@ -83,6 +83,47 @@ __device__ __forceinline__ void green(T dx, T dy, T dz, T r2, const T *nx, const
     }
 }
 
+// Single and double layer of one family at the same point pair, sharing r, 1/r and e^{ikr}:
+// (g1re, g1im) = SL, (g2re, g2im) = DL with the trial normal ny.
+template <bool HELM, typename T>
+__device__ __forceinline__ void green_sl_dl(T dx, T dy, T dz, T r2, const T *ny, T kr, T ki, T &g1re, T &g1im,
+                                            T &g2re, T &g2im) {
+    const T rinv = rsqrt_t(r2);
+    const T dot = dx * ny[0] + dy * ny[1] + dz * ny[2];
+    if constexpr (!HELM) {
+        g1re = T(kInv4Pi) * rinv;
+        g1im = T(0);
+        g2re = -g1re * dot * rinv * rinv;
+        g2im = T(0);
+    } else {
+        const T r = r2 * rinv;
+        T s, c;
+        sincos_t(kr * r, &s, &c);
+        T g = T(kInv4Pi) * rinv;
+        if (ki != T(0)) g *= exp_t(-ki * r);
+        g1re = c * g;
+        g1im = s * g;
+        const T gd = g * dot * rinv * rinv;
+        const T fr = c * gd, fi = s * gd;
+        const T a = T(-1) - ki * r, b = kr * r;
+        g2re = a * fr - b * fi;
+        g2im = b * fr + a * fi;
+    }
+}
+
+// Green's function of OP and, for DUAL (OP a single layer), also the double layer of the same
+// family at the same point pair (shared r and e^{ikr}).
+template <int OP, bool DUAL, typename T>
+__device__ __forceinline__ void green2(T dx, T dy, T dz, T r2, const T *nx, const T *ny, T kr, T ki, T &g1re, T &g1im,
+                                       T &g2re, T &g2im) {
+    if constexpr (DUAL) {
+        green_sl_dl<OpTraits<OP>::is_complex, T>(dx, dy, dz, r2, ny, kr, ki, g1re, g1im, g2re, g2im);
+    } else {
+        green<OP, T>(dx, dy, dz, r2, nx, ny, kr, ki, g1re, g1im);
+        g2re = g2im = T(0);
+    }
+}
+
 // Compensated (Kahan) accumulator; plain sum when K = false.
 template <typename T, bool K>
 struct Acc {

@@ -7,7 +7,7 @@ namespace bem {
 static int regular_entry(int op, const bem_space_t *test, const bem_space_t *trial, const bem_dof_blocks_t *tblocks,
                          const bem_dof_blocks_t *sblocks, int grids_identical, const bem_quad_rule_t *rule,
                          double k_re, double k_im, int precision, double mixed_near_factor, int kahan, void *out,
-                         int64_t ld, int out_dtype, int accumulate, void *stream) {
+                         void *out2, int64_t ld, int out_dtype, int accumulate, void *stream) {
     clear_error();
     if (!test || !trial || !rule || !out) {
         set_error("null argument");
@@ -49,6 +49,7 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
     a.ki = k_im;
     a.near_factor = mixed_near_factor;
     a.out = out;
+    a.out2 = out2;
     a.ld = ld;
     a.identical = grids_identical;
     a.accumulate = accumulate;
@@ -61,6 +62,12 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
                          sblocks->n_blocks, sblocks->max_dofs, sblocks->max_incidence};
     }
     hipStream_t s = (hipStream_t)stream;
+    if (out2) {
+        if (op == BEM_LAPLACE_SINGLE_LAYER) return regular_launch_dual_0(a, precision, kahan, tile, out_dtype, s);
+        if (op == BEM_HELMHOLTZ_SINGLE_LAYER) return regular_launch_dual_3(a, precision, kahan, tile, out_dtype, s);
+        set_error("the single/double layer pair needs a single-layer operator, got %d", op);
+        return BEM_ERR_UNSUPPORTED;
+    }
     switch (op) {
     case 0: return regular_launch_0(a, precision, kahan, tile, out_dtype, s);
     case 1: return regular_launch_1(a, precision, kahan, tile, out_dtype, s);
@@ -82,7 +89,7 @@ extern "C" int bem_regular_assemble(int op, const bem_space_t *test, const bem_s
                                     double mixed_near_factor, int kahan, void *out, int64_t ld, int out_dtype,
                                     int accumulate, void *stream) {
     return regular_entry(op, test, trial, nullptr, nullptr, grids_identical, rule, k_re, k_im, precision,
-                         mixed_near_factor, kahan, out, ld, out_dtype, accumulate, stream);
+                         mixed_near_factor, kahan, out, nullptr, ld, out_dtype, accumulate, stream);
 }
 
 extern "C" int bem_regular_assemble_blocked(int op, const bem_space_t *test, const bem_space_t *trial,
@@ -96,5 +103,20 @@ extern "C" int bem_regular_assemble_blocked(int op, const bem_space_t *test, con
         return BEM_ERR_INVALID_ARGUMENT;
     }
     return regular_entry(op, test, trial, test_blocks, trial_blocks, grids_identical, rule, k_re, k_im, precision,
-                         mixed_near_factor, kahan, out, ld, out_dtype, 0, stream);
+                         mixed_near_factor, kahan, out, nullptr, ld, out_dtype, 0, stream);
+}
+
+extern "C" int bem_regular_assemble_sl_dl(int op, const bem_space_t *test, const bem_space_t *trial,
+                                          const bem_dof_blocks_t *test_blocks, const bem_dof_blocks_t *trial_blocks,
+                                          int grids_identical, const bem_quad_rule_t *rule, double k_re, double k_im,
+                                          int precision, double mixed_near_factor, int kahan, void *out_single,
+                                          void *out_double, int64_t ld, int out_dtype, int accumulate, void *stream) {
+    if (!out_double) {
+        clear_error();
+        set_error("null argument");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    return regular_entry(op, test, trial, test_blocks, trial_blocks, grids_identical, rule, k_re, k_im, precision,
+                         mixed_near_factor, kahan, out_single, out_double, ld, out_dtype,
+                         test_blocks ? 0 : accumulate, stream);
 }

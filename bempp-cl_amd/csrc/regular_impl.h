@@ -66,10 +66,12 @@ struct RegularArgs {
     double kr, ki;
     double near_factor;
     void *out;
+    void *out2;  // DUAL kernels: the double layer (out holds the single layer)
     int64_t ld;
     int identical;
     int accumulate;
 };
+
 
 // NXP: 1 in MIXED mode, where the absolute FP64 test points are not needed
 template <int TM, int NXP = TM>
@@ -88,27 +90,61 @@ struct TestTile {
 };
 
 // ------------------------------------------------------------------ FP64 pair
-template <int OP, int NTS, int NSS, int QT>
-__device__ __forceinline__ void pair_fp64_geom(const double *__restrict__ gt, const double *__restrict__ gf, double tnm,
-                                               double snm, const RuleArgs &rule, double kr, double ki,
-                                               C2d (&S)[NTS][NSS]) {
-    const int nq = QT > 0 ? QT : rule.n;
-    double nx[3] = {gt[9] * tnm, gt[10] * tnm, gt[11] * tnm};
-    double ny[3] = {gf[9] * snm, gf[10] * snm, gf[11] * snm};
-    double jt = gt[12], jf = gf[12];
+// T[b] += g u_b(eta_q) with u = (1, eta0, eta1)
+template <int NSS>
+__device__ __forceinline__ void add_trial_moments(C2d (&T)[NSS], double gre, double gim, double s0, double s1) {
+    T[0].re += gre;
+    T[0].im += gim;
+    if constexpr (NSS == 3) {
+        T[1].re += gre * s0;
+        T[1].im += gim * s0;
+        T[2].re += gre * s1;
+        T[2].im += gim * s1;
+    }
+}
+
+// S[a][b] += u_a(xi_p) T[b] with u = (wp, wp xi0, wp xi1)
+template <int NTS, int NSS>
+__device__ __forceinline__ void add_test_moments(C2d (&S)[NTS][NSS], const C2d (&T)[NSS], double wp, double xi,
+                                                 double eta) {
+    const double u[3] = {wp, wp * xi, wp * eta};
+#pragma unroll
+    for (int a = 0; a < NTS; ++a)
+#pragma unroll
+        for (int b = 0; b < NSS; ++b) {
+            S[a][b].re += u[a] * T[b].re;
+            S[a][b].im += u[a] * T[b].im;
+        }
+}
+
+template <int NTS, int NSS>
+__device__ __forceinline__ void zero_moments(C2d (&S)[NTS][NSS]) {
 #pragma unroll
     for (int a = 0; a < NTS; ++a)
 #pragma unroll
         for (int b = 0; b < NSS; ++b) S[a][b].re = S[a][b].im = 0.0;
+}
+
+// FP64 pair from the geometry table (any rule). DUAL: S2 receives the double layer.
+template <int OP, int NTS, int NSS, int QT, bool DUAL = false>
+__device__ __forceinline__ void pair_fp64_geom(const double *__restrict__ gt, const double *__restrict__ gf, double tnm,
+                                               double snm, const RuleArgs &rule, double kr, double ki,
+                                               C2d (&S)[NTS][NSS], C2d (&S2)[NTS][NSS]) {
+    const int nq = QT > 0 ? QT : rule.n;
+    double nx[3] = {gt[9] * tnm, gt[10] * tnm, gt[11] * tnm};
+    double ny[3] = {gf[9] * snm, gf[10] * snm, gf[11] * snm};
+    double jt = gt[12], jf = gf[12];
+    zero_moments(S);
+    if constexpr (DUAL) zero_moments(S2);
 #pragma unroll 1
     for (int p = 0; p < nq; ++p) {
         double xi = rule.x[p], eta = rule.y[p];
         double x0 = gt[0] + gt[3] * xi + gt[6] * eta;
         double x1 = gt[1] + gt[4] * xi + gt[7] * eta;
         double x2 = gt[2] + gt[5] * xi + gt[8] * eta;
-        C2d T[NSS];
+        C2d T[NSS], T2[NSS];
 #pragma unroll
-        for (int b = 0; b < NSS; ++b) T[b].re = T[b].im = 0.0;
+        for (int b = 0; b < NSS; ++b) T[b].re = T[b].im = T2[b].re = T2[b].im = 0.0;
 #pragma unroll(QT > 0 ? QT : 1)
         for (int q = 0; q < nq; ++q) {
             double s0 = rule.x[q], s1 = rule.y[q];
@@ -116,77 +152,44 @@ __device__ __forceinline__ void pair_fp64_geom(const double *__restrict__ gt, co
             double dy = gf[1] + gf[4] * s0 + gf[7] * s1 - x1;
             double dz = gf[2] + gf[5] * s0 + gf[8] * s1 - x2;
             double r2 = dx * dx + dy * dy + dz * dz;
-            double gre, gim;
-            green<OP, double>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim);
-            double wq = rule.w[q] * jf;
-            gre *= wq;
-            gim *= wq;
-            T[0].re += gre;
-            T[0].im += gim;
-            if constexpr (NSS == 3) {
-                T[1].re += gre * s0;
-                T[1].im += gim * s0;
-                T[2].re += gre * s1;
-                T[2].im += gim * s1;
-            }
+            double gre, gim, g2re, g2im;
+            green2<OP, DUAL, double>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im);
+            const double wq = rule.w[q] * jf;
+            add_trial_moments<NSS>(T, gre * wq, gim * wq, s0, s1);
+            if constexpr (DUAL) add_trial_moments<NSS>(T2, g2re * wq, g2im * wq, s0, s1);
         }
-        double wp = rule.w[p] * jt;
-        double u[3] = {wp, wp * xi, wp * eta};
-#pragma unroll
-        for (int a = 0; a < NTS; ++a)
-#pragma unroll
-            for (int b = 0; b < NSS; ++b) {
-                S[a][b].re += u[a] * T[b].re;
-                S[a][b].im += u[a] * T[b].im;
-            }
+        add_test_moments<NTS, NSS>(S, T, rule.w[p] * jt, xi, eta);
+        if constexpr (DUAL) add_test_moments<NTS, NSS>(S2, T2, rule.w[p] * jt, xi, eta);
     }
 }
 
-// FP64 pair with test points from LDS and trial points from registers
-template <int OP, int NTS, int NSS, int QT, int TM, int NXP>
+// FP64 pair with test points from LDS and trial points from registers. DUAL: S2 = double layer.
+template <int OP, int NTS, int NSS, int QT, int TM, int NXP, bool DUAL = false>
 __device__ __forceinline__ void pair_fp64_regs(const TestTile<TM, NXP> &T_, int t, const double *gt, double tnm,
                                                const double (&yq)[QT > 0 ? QT : 1][3], const double (&wq)[QT > 0 ? QT : 1],
                                                const double *ny, const RuleArgs &rule, double kr, double ki,
-                                               C2d (&S)[NTS][NSS]) {
+                                               C2d (&S)[NTS][NSS], C2d (&S2)[NTS][NSS]) {
     double nx[3] = {gt[9] * tnm, gt[10] * tnm, gt[11] * tnm};
     double jt = gt[12];
-#pragma unroll
-    for (int a = 0; a < NTS; ++a)
-#pragma unroll
-        for (int b = 0; b < NSS; ++b) S[a][b].re = S[a][b].im = 0.0;
+    zero_moments(S);
+    if constexpr (DUAL) zero_moments(S2);
 #pragma unroll 1
     for (int p = 0; p < QT; ++p) {
         double x0 = T_.xp[t][p][0], x1 = T_.xp[t][p][1], x2 = T_.xp[t][p][2];
-        C2d T[NSS];
+        C2d T[NSS], T2[NSS];
 #pragma unroll
-        for (int b = 0; b < NSS; ++b) T[b].re = T[b].im = 0.0;
+        for (int b = 0; b < NSS; ++b) T[b].re = T[b].im = T2[b].re = T2[b].im = 0.0;
 #pragma unroll
         for (int q = 0; q < QT; ++q) {
             double dx = yq[q][0] - x0, dy = yq[q][1] - x1, dz = yq[q][2] - x2;
             double r2 = dx * dx + dy * dy + dz * dz;
-            double gre, gim;
-            green<OP, double>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim);
-            gre *= wq[q];
-            gim *= wq[q];
-            T[0].re += gre;
-            T[0].im += gim;
-            if constexpr (NSS == 3) {
-                double s0 = rule.x[q], s1 = rule.y[q];
-                T[1].re += gre * s0;
-                T[1].im += gim * s0;
-                T[2].re += gre * s1;
-                T[2].im += gim * s1;
-            }
+            double gre, gim, g2re, g2im;
+            green2<OP, DUAL, double>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im);
+            add_trial_moments<NSS>(T, gre * wq[q], gim * wq[q], rule.x[q], rule.y[q]);
+            if constexpr (DUAL) add_trial_moments<NSS>(T2, g2re * wq[q], g2im * wq[q], rule.x[q], rule.y[q]);
         }
-        double wp = rule.w[p] * jt;
-        double u[3] = {wp, wp * rule.x[p], wp * rule.y[p]};
-#pragma unroll
-        for (int a = 0; a < NTS; ++a)
-#pragma unroll
-            for (int b = 0; b < NSS; ++b) {
-                S[a][b].re += u[a] * T[b].re;
-                S[a][b].im += u[a] * T[b].im;
-            }
+        add_test_moments<NTS, NSS>(S, T, rule.w[p] * jt, rule.x[p], rule.y[p]);
+        if constexpr (DUAL) add_test_moments<NTS, NSS>(S2, T2, rule.w[p] * jt, rule.x[p], rule.y[p]);
     }
 }
 
@@ -207,11 +210,14 @@ __device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf
 
 // DAMP: complex wavenumber (e^{-Im(k) r} factor); a template parameter so that real
 // wavenumbers do not pay for an if-converted exponential per evaluation.
-template <int OP, int NTS, int NSS, int QT, int TM, bool KAHAN, bool DAMP, int NXP>
+// DUAL (OP a single layer): S2 receives the double layer of the same family, sharing the
+// distance, 1/r and e^{ik delta} of every point pair.
+template <int OP, int NTS, int NSS, int QT, int TM, bool KAHAN, bool DAMP, bool DUAL, int NXP>
 __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, const TrialF32<QT> &F, double d0, double d1,
                                           double d2, double dist2, const RuleArgs &rule, double kr, double ki,
-                                          C2f (&S)[NTS][NSS]) {
+                                          C2f (&S)[NTS][NSS], C2f (&S2)[NTS][NSS]) {
     constexpr bool HELM = OP >= kHelmholtzSL;
+    static_assert(!DUAL || OP == kLaplaceSL || OP == kHelmholtzSL, "DUAL pairs a single layer with its double layer");
     const int nq = QT > 0 ? QT : rule.n;
     const float fd0 = (float)d0, fd1 = (float)d1, fd2 = (float)d2;
     // reference distance r0 (FP32 value) and the FP64 residual |d|^2 - r0^2
@@ -231,12 +237,17 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
     }
 
     Acc<float, KAHAN> Sr[NTS][NSS], Si[NTS][NSS];
+    Acc<float, KAHAN> Sr2[DUAL ? NTS : 1][DUAL ? NSS : 1], Si2[DUAL ? NTS : 1][DUAL ? NSS : 1];
 #pragma unroll
     for (int i = 0; i < NTS; ++i)
 #pragma unroll
         for (int j = 0; j < NSS; ++j) {
             Sr[i][j].zero();
             Si[i][j].zero();
+            if constexpr (DUAL) {
+                Sr2[i][j].zero();
+                Si2[i][j].zero();
+            }
         }
     // the test-point loop stays rolled: unrolling it lets the scheduler keep all 36
     // evaluations in flight, which exhausts the VGPR file and drops occupancy to 1 wave
@@ -248,23 +259,27 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
         const float Ap = resid - 2.f * (fd0 * T_.a[t][p][0] + fd1 * T_.a[t][p][1] + fd2 * T_.a[t][p][2]) + T_.aa[t][p];
         const float tw0 = 2.f * w0, tw1 = 2.f * w1, tw2 = 2.f * w2;
         float wn = 0.f, wnx = 0.f;
-        if constexpr (OP == kLaplaceDL || OP == kHelmholtzDL) wn = w0 * F.ny[0] + w1 * F.ny[1] + w2 * F.ny[2];
+        if constexpr (OP == kLaplaceDL || OP == kHelmholtzDL || DUAL) wn = w0 * F.ny[0] + w1 * F.ny[1] + w2 * F.ny[2];
         if constexpr (OP == kLaplaceADL || OP == kHelmholtzADL)
             wnx = -(w0 * T_.nx[t][0] + w1 * T_.nx[t][1] + w2 * T_.nx[t][2]);
-        float Tr[NSS], Ti[NSS];
+        float Tr[NSS], Ti[NSS], Tr2[DUAL ? NSS : 1], Ti2[DUAL ? NSS : 1];
 #pragma unroll
-        for (int b = 0; b < NSS; ++b) Tr[b] = Ti[b] = 0.f;
+        for (int b = 0; b < NSS; ++b) {
+            Tr[b] = Ti[b] = 0.f;
+            if constexpr (DUAL) Tr2[b] = Ti2[b] = 0.f;
+        }
 #pragma unroll(QT > 0 ? QT : 1)
         for (int q = 0; q < nq; ++q) {
             // t = |y - x|^2 - r0^2
             const float tt = Ap + F.bb[q] + tw0 * F.b[q][0] + tw1 * F.b[q][1] + tw2 * F.b[q][2];
             const float r2 = r0sq + tt;
             const float rinv = __builtin_amdgcn_rsqf(r2);
-            float gre, gim;
+            float gre, gim, g2re = 0.f, g2im = 0.f;
             if constexpr (!HELM) {
                 float g = rinv * F.wq[q];
                 if constexpr (OP == kLaplaceSL) {
                     gre = g;
+                    if constexpr (DUAL) g2re = -(wn + F.bn[q]) * g * rinv * rinv;
                 } else if constexpr (OP == kLaplaceDL) {
                     gre = -(wn + F.bn[q]) * g * rinv * rinv;
                 } else {  // (y - x).n_x = bnx - wnx
@@ -281,6 +296,12 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
                 if constexpr (OP == kHelmholtzSL || OP == kMaxwellEFIE) {
                     gre = c * g;
                     gim = s * g;
+                    if constexpr (DUAL) {
+                        const float gd = g * (wn + F.bn[q]) * rinv * rinv;
+                        const float ar = DAMP ? -1.f - ki_f * r : -1.f, ai = kr_f * r;
+                        g2re = (ar * c - ai * s) * gd;
+                        g2im = (ai * c + ar * s) * gd;
+                    }
                 } else {
                     float dot;
                     if constexpr (OP == kHelmholtzDL) dot = wn + F.bn[q];
@@ -293,6 +314,10 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
             }
             Tr[0] += gre;
             if constexpr (HELM) Ti[0] += gim;
+            if constexpr (DUAL) {
+                Tr2[0] += g2re;
+                if constexpr (HELM) Ti2[0] += g2im;
+            }
             if constexpr (NSS == 3) {
                 const float s0 = (float)rule.x[q], s1 = (float)rule.y[q];
                 Tr[1] += gre * s0;
@@ -300,6 +325,14 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
                 if constexpr (HELM) {
                     Ti[1] += gim * s0;
                     Ti[2] += gim * s1;
+                }
+                if constexpr (DUAL) {
+                    Tr2[1] += g2re * s0;
+                    Tr2[2] += g2re * s1;
+                    if constexpr (HELM) {
+                        Ti2[1] += g2im * s0;
+                        Ti2[2] += g2im * s1;
+                    }
                 }
             }
         }
@@ -311,6 +344,10 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
             for (int j = 0; j < NSS; ++j) {
                 Sr[i][j].add(u[i] * Tr[j]);
                 if constexpr (HELM) Si[i][j].add(u[i] * Ti[j]);
+                if constexpr (DUAL) {
+                    Sr2[i][j].add(u[i] * Tr2[j]);
+                    if constexpr (HELM) Si2[i][j].add(u[i] * Ti2[j]);
+                }
             }
     }
     // multiply by e^{i k r0} e^{-ki r0} (FP64 phase reduced to one revolution)
@@ -330,6 +367,12 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
             float si = HELM ? Si[i][j].s - Si[i][j].c : 0.f;
             S[i][j].re = pr * sr - pi * si;
             S[i][j].im = pr * si + pi * sr;
+            if constexpr (DUAL) {
+                sr = Sr2[i][j].s - Sr2[i][j].c;
+                si = HELM ? Si2[i][j].s - Si2[i][j].c : 0.f;
+                S2[i][j].re = pr * sr - pi * si;
+                S2[i][j].im = pr * si + pi * sr;
+            }
         }
 }
 
@@ -387,12 +430,15 @@ __device__ __forceinline__ void store_direct(void *out, int64_t ld, int row, int
 }
 
 // ------------------------------------------------------------------ kernel
+// DUAL: OP is a single layer and the kernel also assembles the double layer of the same family
+// into a.out2 from the same point-pair geometry and phase factors.
 template <int OP, int NTS, int NSS, int QT, int TM, bool MIXED, bool KAHAN, bool TILE, int MAXU_T, int MAXU_S,
-          typename OutT>
+          typename OutT, bool DUAL = false>
 __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs a) {
     constexpr bool CPLX = OpTraits<OP>::is_complex;
     constexpr int NT = NTS, NS = NSS;
     constexpr int NC = CPLX ? 2 : 1;
+    constexpr int OP2 = OP + 1;  // the double layer of the family (DUAL only)
     // OWNER: the trial-side reduction of the tile runs without atomics. For every test
     // element each lane stores its local block to LDS; after a barrier the lane owning trial
     // slot s sums the block entries of the (element, j) incidences of s and updates its own
@@ -403,6 +449,7 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
 
     __shared__ TestTile<TM, MIXED ? 1 : TM> T_;
     __shared__ AccT acc[TILE ? MAXU_T * MAXU_S * NC : 1];
+    __shared__ AccT acc2[TILE && DUAL ? MAXU_T * MAXU_S * NC : 1];
     __shared__ float stage[OWNER ? NT * NS * NC : 1][OWNER ? kRegThreads : 1];
 
     const int tid = threadIdx.x;
@@ -425,7 +472,10 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
     const int ntile = max(0, min(t1 - t0, TM));
 
     if constexpr (TILE) {
-        for (int i = tid; i < MAXU_T * MAXU_S * (CPLX ? 2 : 1); i += kRegThreads) acc[i] = (AccT)0;
+        for (int i = tid; i < MAXU_T * MAXU_S * (CPLX ? 2 : 1); i += kRegThreads) {
+            acc[i] = (AccT)0;
+            if constexpr (DUAL) acc2[i] = (AccT)0;
+        }
     }
     if (tid < ntile) {
         const int pos = t0 + tid;
@@ -541,14 +591,16 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
             adj = v0 == fv0 || v0 == fv1 || v0 == fv2 || v1 == fv0 || v1 == fv1 || v1 == fv2 || v2 == fv0 ||
                   v2 == fv1 || v2 == fv2;
         }
-        C2d S[NTS][NSS];   // FP64 moments (FP64 mode; MIXED paths that need the FP64 epilogue)
-        C2f Sf[NTS][NSS];  // FP32 moments (MIXED)
+        C2d S[NTS][NSS], S2[NTS][NSS];   // FP64 moments (FP64 mode; MIXED paths that need the FP64 epilogue)
+        C2f Sf[NTS][NSS], Sf2[NTS][NSS];  // FP32 moments (MIXED)
 #pragma unroll
         for (int i = 0; i < NTS; ++i)
 #pragma unroll
             for (int j = 0; j < NSS; ++j) {
                 S[i][j].re = S[i][j].im = 0.0;
                 Sf[i][j].re = Sf[i][j].im = 0.f;
+                S2[i][j].re = S2[i][j].im = 0.0;
+                Sf2[i][j].re = Sf2[i][j].im = 0.f;
             }
 
         // MIXED: near pairs are left to the FP64 near-field kernel (bem_near_assemble)
@@ -557,9 +609,10 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
             if constexpr (!MIXED) {
                 double tnm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
                 if constexpr (QT > 0)
-                    pair_fp64_regs<OP, NTS, NSS, QT, TM>(T_, t, gt, tnm, yq, wq64, ny64, a.rule, a.kr, a.ki, S);
+                    pair_fp64_regs<OP, NTS, NSS, QT, TM, MIXED ? 1 : TM, DUAL>(T_, t, gt, tnm, yq, wq64, ny64, a.rule,
+                                                                             a.kr, a.ki, S, S2);
                 else
-                    pair_fp64_geom<OP, NTS, NSS, QT>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S);
+                    pair_fp64_geom<OP, NTS, NSS, QT, DUAL>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2);
             } else {
                 const double cfv[3] = {cf0, cf1, cf2};
                 double dist2;
@@ -567,9 +620,11 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
                 if (!skip) {
                     double d0 = cf0 - T_.c[t][0], d1 = cf1 - T_.c[t][1], d2 = cf2 - T_.c[t][2];
                     if (OpTraits<OP>::is_complex && a.ki != 0.0)
-                        pair_fp32<OP, NTS, NSS, QT, TM, KAHAN, true>(T_, t, F32, d0, d1, d2, dist2, a.rule, a.kr, a.ki, Sf);
+                        pair_fp32<OP, NTS, NSS, QT, TM, KAHAN, true, DUAL>(T_, t, F32, d0, d1, d2, dist2, a.rule, a.kr,
+                                                                           a.ki, Sf, Sf2);
                     else
-                        pair_fp32<OP, NTS, NSS, QT, TM, KAHAN, false>(T_, t, F32, d0, d1, d2, dist2, a.rule, a.kr, a.ki, Sf);
+                        pair_fp32<OP, NTS, NSS, QT, TM, KAHAN, false, DUAL>(T_, t, F32, d0, d1, d2, dist2, a.rule, a.kr,
+                                                                            a.ki, Sf, Sf2);
                 }
             }
         }
@@ -580,53 +635,62 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
                 for (int j = 0; j < NSS; ++j) {
                     S[i][j].re = (double)Sf[i][j].re;
                     S[i][j].im = (double)Sf[i][j].im;
+                    if constexpr (DUAL) {
+                        S2[i][j].re = (double)Sf2[i][j].re;
+                        S2[i][j].im = (double)Sf2[i][j].im;
+                    }
                 }
         }
 
         if constexpr (OWNER) {
-            // stage this lane's block (zero for idle lanes, skipped pairs, zero multipliers)
-            C2f V[NTS][NSS];
-            if constexpr (OP != kMaxwellEFIE) scalar_block_f32<NTS, NSS>(Sf, V);
+            // stage this lane's block (zero for idle lanes, skipped pairs, zero multipliers), then
+            // the owner lane of every trial slot adds the entries of its incidences to its tile column
+            auto owner_pass = [&](const C2f(&Sm)[NTS][NSS], const C2d(&Sd)[NTS][NSS], AccT *tile) {
+                C2f V[NTS][NSS];
+                if constexpr (OP != kMaxwellEFIE) scalar_block_f32<NTS, NSS>(Sm, V);
 #pragma unroll
-            for (int i = 0; i < NT; ++i)
+                for (int i = 0; i < NT; ++i)
 #pragma unroll
-                for (int j = 0; j < NS; ++j) {
-                    float vr = 0.f, vi = 0.f;
-                    const float m = (float)T_.mult[t][i] * (float)mf[j];
-                    if (valid && !skip && m != 0.f) {
-                        if constexpr (OP == kMaxwellEFIE) {
-                            C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
-                            vr = (float)v.re * m;
-                            vi = (float)v.im * m;
-                        } else {
-                            vr = V[i][j].re * m;
-                            vi = V[i][j].im * m;
+                    for (int j = 0; j < NS; ++j) {
+                        float vr = 0.f, vi = 0.f;
+                        const float m = (float)T_.mult[t][i] * (float)mf[j];
+                        if (valid && !skip && m != 0.f) {
+                            if constexpr (OP == kMaxwellEFIE) {
+                                C2d v = local_value<OP, NTS, NSS>(Sd, i, j, gt, gf, a.kr, a.ki);
+                                vr = (float)v.re * m;
+                                vi = (float)v.im * m;
+                            } else {
+                                vr = V[i][j].re * m;
+                                vi = V[i][j].im * m;
+                            }
                         }
+                        stage[(j * NT + i) * NC][tid] = vr;
+                        if constexpr (CPLX) stage[(j * NT + i) * NC + 1][tid] = vi;
                     }
-                    stage[(j * NT + i) * NC][tid] = vr;
-                    if constexpr (CPLX) stage[(j * NT + i) * NC + 1][tid] = vi;
-                }
-            __syncthreads();
-            if (tid < own_nds) {
+                __syncthreads();
+                if (tid < own_nds) {
 #pragma unroll
-                for (int i = 0; i < NT; ++i) {
-                    if (T_.mult[t][i] == 0.0) continue;
-                    float sr = 0.f, si = 0.f;
+                    for (int i = 0; i < NT; ++i) {
+                        if (T_.mult[t][i] == 0.0) continue;
+                        float sr = 0.f, si = 0.f;
 #pragma unroll
-                    for (int k = 0; k < kMaxInc; ++k) {
-                        const int code = own_inc[k];
-                        if (code >= 0) {
-                            const int p = code / 3, j = code - 3 * p;
-                            sr += stage[(j * NT + i) * NC][p];
-                            if constexpr (CPLX) si += stage[(j * NT + i) * NC + 1][p];
+                        for (int k = 0; k < kMaxInc; ++k) {
+                            const int code = own_inc[k];
+                            if (code >= 0) {
+                                const int p = code / 3, j = code - 3 * p;
+                                sr += stage[(j * NT + i) * NC][p];
+                                if constexpr (CPLX) si += stage[(j * NT + i) * NC + 1][p];
+                            }
                         }
+                        AccT *dst = tile + NC * (T_.slot[t][i] * MAXU_S + tid);
+                        dst[0] += (AccT)sr;
+                        if constexpr (CPLX) dst[1] += (AccT)si;
                     }
-                    AccT *dst = acc + NC * (T_.slot[t][i] * MAXU_S + tid);
-                    dst[0] += (AccT)sr;
-                    if constexpr (CPLX) dst[1] += (AccT)si;
                 }
-            }
-            __syncthreads();
+                __syncthreads();
+            };
+            owner_pass(Sf, S, acc);
+            if constexpr (DUAL) owner_pass(Sf2, S2, acc2);
         } else if (valid) {
 #pragma unroll
             for (int i = 0; i < NT; ++i) {
@@ -637,9 +701,14 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
                     if constexpr (TILE) {
                         if (m != 0.0 && !skip) {
                             C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
-                            AccT *slot = acc + (CPLX ? 2 : 1) * (T_.slot[t][i] * MAXU_S + slot_f[j]);
-                            unsafeAtomicAdd(slot, (AccT)(v.re * m));
-                            if constexpr (CPLX) unsafeAtomicAdd(slot + 1, (AccT)(v.im * m));
+                            const int off = (CPLX ? 2 : 1) * (T_.slot[t][i] * MAXU_S + slot_f[j]);
+                            unsafeAtomicAdd(acc + off, (AccT)(v.re * m));
+                            if constexpr (CPLX) unsafeAtomicAdd(acc + off + 1, (AccT)(v.im * m));
+                            if constexpr (DUAL) {
+                                C2d v2 = local_value<OP2, NTS, NSS>(S2, i, j, gt, gf, a.kr, a.ki);
+                                unsafeAtomicAdd(acc2 + off, (AccT)(v2.re * m));
+                                if constexpr (CPLX) unsafeAtomicAdd(acc2 + off + 1, (AccT)(v2.im * m));
+                            }
                         }
                     } else {
                         C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
@@ -648,6 +717,12 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
                         int row = a.test.l2g[(int64_t)e * NT + i];
                         int col = a.trial.l2g[(int64_t)f * NS + j];
                         store_direct<OutT, CPLX>(a.out, a.ld, row, col, v, a.accumulate);
+                        if constexpr (DUAL) {
+                            C2d v2 = local_value<OP2, NTS, NSS>(S2, i, j, gt, gf, a.kr, a.ki);
+                            v2.re *= m;
+                            v2.im *= m;
+                            store_direct<OutT, CPLX>(a.out2, a.ld, row, col, v2, a.accumulate);
+                        }
                     }
                 }
             }
@@ -659,23 +734,27 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
         const int ndt = min(a.tb.ndofs[blockIdx.y], MAXU_T), nds = min(a.sb.ndofs[blockIdx.x], MAXU_S);
         const int32_t *dt = a.tb.dofs + (int64_t)blockIdx.y * a.tb.maxu;
         const int32_t *ds = a.sb.dofs + (int64_t)blockIdx.x * a.sb.maxu;
-        OutT *o = reinterpret_cast<OutT *>(a.out);
-        for (int idx = tid; idx < ndt * nds; idx += kRegThreads) {
-            const int ia = idx / nds, is = idx - ia * nds;
-            const AccT *src = acc + (CPLX ? 2 : 1) * (ia * MAXU_S + is);
-            const int64_t pos = (int64_t)dt[ia] * a.ld + ds[is];
-            if constexpr (CPLX) {
-                if (src[0] != (AccT)0) unsafeAtomicAdd(o + 2 * pos, src[0]);
-                if (src[1] != (AccT)0) unsafeAtomicAdd(o + 2 * pos + 1, src[1]);
-            } else {
-                if (src[0] != (AccT)0) unsafeAtomicAdd(o + pos, src[0]);
+        auto flush = [&](const AccT *tile, void *out) {
+            OutT *o = reinterpret_cast<OutT *>(out);
+            for (int idx = tid; idx < ndt * nds; idx += kRegThreads) {
+                const int ia = idx / nds, is = idx - ia * nds;
+                const AccT *src = tile + (CPLX ? 2 : 1) * (ia * MAXU_S + is);
+                const int64_t pos = (int64_t)dt[ia] * a.ld + ds[is];
+                if constexpr (CPLX) {
+                    if (src[0] != (AccT)0) unsafeAtomicAdd(o + 2 * pos, src[0]);
+                    if (src[1] != (AccT)0) unsafeAtomicAdd(o + 2 * pos + 1, src[1]);
+                } else {
+                    if (src[0] != (AccT)0) unsafeAtomicAdd(o + pos, src[0]);
+                }
             }
-        }
+        };
+        flush(acc, a.out);
+        if constexpr (DUAL) flush(acc2, a.out2);
     }
 }
 
 // ------------------------------------------------------------------ dispatch
-template <int OP, int NTS, int NSS, int QT, bool MIXED, typename OutT>
+template <int OP, int NTS, int NSS, int QT, bool MIXED, typename OutT, bool DUAL = false>
 static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
     constexpr int TM = 16;
     constexpr bool CPLX = OpTraits<OP>::is_complex;
@@ -703,8 +782,8 @@ static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
         return BEM_ERR_INVALID_ARGUMENT;
     }
 #define BEM_LAUNCH(K, T)                                                                                         \
-    hipLaunchKernelGGL((regular_kernel<OP, NTS, NSS, QT, TM, MIXED, K, T, (T ? MT : 1), (T ? MS : 1), OutT>), grid, \
-                       dim3(kRegThreads), 0, stream, a)
+    hipLaunchKernelGGL((regular_kernel<OP, NTS, NSS, QT, TM, MIXED, K, T, (T ? MT : 1), (T ? MS : 1), OutT, DUAL>), \
+                       grid, dim3(kRegThreads), 0, stream, a)
     if (tile) {
         if (MIXED && kahan) BEM_LAUNCH(true, true);
         else BEM_LAUNCH(false, true);
@@ -716,39 +795,43 @@ static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
     return check_launch("regular_kernel");
 }
 
-template <int OP, int NTS, int NSS, bool MIXED, typename OutT>
+template <int OP, int NTS, int NSS, bool MIXED, typename OutT, bool DUAL = false>
 static int launch2(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
-    if (a.rule.n == 6) return launch3<OP, NTS, NSS, 6, MIXED, OutT>(a, kahan, tile, stream);
-    return launch3<OP, NTS, NSS, 0, MIXED, OutT>(a, kahan, tile, stream);
+    if (a.rule.n == 6) return launch3<OP, NTS, NSS, 6, MIXED, OutT, DUAL>(a, kahan, tile, stream);
+    return launch3<OP, NTS, NSS, 0, MIXED, OutT, DUAL>(a, kahan, tile, stream);
 }
 
-template <int OP, bool MIXED, typename OutT>
+template <int OP, bool MIXED, typename OutT, bool DUAL = false>
 static int launch1(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
     if constexpr (OP == kMaxwellEFIE) {
         return launch2<OP, 3, 3, MIXED, OutT>(a, kahan, tile, stream);
     } else {
         int nt = nshape_of(a.test.shape), ns = nshape_of(a.trial.shape);
-        if (nt == 1 && ns == 1) return launch2<OP, 1, 1, MIXED, OutT>(a, kahan, tile, stream);
-        if (nt == 1 && ns == 3) return launch2<OP, 1, 3, MIXED, OutT>(a, kahan, tile, stream);
-        if (nt == 3 && ns == 1) return launch2<OP, 3, 1, MIXED, OutT>(a, kahan, tile, stream);
-        return launch2<OP, 3, 3, MIXED, OutT>(a, kahan, tile, stream);
+        if (nt == 1 && ns == 1) return launch2<OP, 1, 1, MIXED, OutT, DUAL>(a, kahan, tile, stream);
+        if (nt == 1 && ns == 3) return launch2<OP, 1, 3, MIXED, OutT, DUAL>(a, kahan, tile, stream);
+        if (nt == 3 && ns == 1) return launch2<OP, 3, 1, MIXED, OutT, DUAL>(a, kahan, tile, stream);
+        return launch2<OP, 3, 3, MIXED, OutT, DUAL>(a, kahan, tile, stream);
     }
 }
 
-template <int OP>
+template <int OP, bool DUAL = false>
 static int launch_op(const RegularArgs &a, int precision, int kahan, bool tile, int out_dtype, hipStream_t stream) {
     constexpr bool CPLX = OpTraits<OP>::is_complex;
     const bool mixed = precision == BEM_PRECISION_MIXED;
     if (CPLX) {
         if (out_dtype == BEM_C128)
-            return mixed ? launch1<OP, true, double>(a, kahan, tile, stream) : launch1<OP, false, double>(a, kahan, tile, stream);
+            return mixed ? launch1<OP, true, double, DUAL>(a, kahan, tile, stream)
+                         : launch1<OP, false, double, DUAL>(a, kahan, tile, stream);
         if (out_dtype == BEM_C64)
-            return mixed ? launch1<OP, true, float>(a, kahan, tile, stream) : launch1<OP, false, float>(a, kahan, tile, stream);
+            return mixed ? launch1<OP, true, float, DUAL>(a, kahan, tile, stream)
+                         : launch1<OP, false, float, DUAL>(a, kahan, tile, stream);
     } else {
         if (out_dtype == BEM_F64)
-            return mixed ? launch1<OP, true, double>(a, kahan, tile, stream) : launch1<OP, false, double>(a, kahan, tile, stream);
+            return mixed ? launch1<OP, true, double, DUAL>(a, kahan, tile, stream)
+                         : launch1<OP, false, double, DUAL>(a, kahan, tile, stream);
         if (out_dtype == BEM_F32)
-            return mixed ? launch1<OP, true, float>(a, kahan, tile, stream) : launch1<OP, false, float>(a, kahan, tile, stream);
+            return mixed ? launch1<OP, true, float, DUAL>(a, kahan, tile, stream)
+                         : launch1<OP, false, float, DUAL>(a, kahan, tile, stream);
     }
     set_error("output dtype %d does not match operator %d", out_dtype, OP);
     return BEM_ERR_INVALID_ARGUMENT;
@@ -766,5 +849,8 @@ BEM_DECLARE_REGULAR_OP(4)
 BEM_DECLARE_REGULAR_OP(5)
 BEM_DECLARE_REGULAR_OP(6)
 #undef BEM_DECLARE_REGULAR_OP
+// single layer + double layer in one pass (regular_dual<k>.hip, k = the single layer)
+int regular_launch_dual_0(const RegularArgs &a, int precision, int kahan, bool tile, int out_dtype, hipStream_t s);
+int regular_launch_dual_3(const RegularArgs &a, int precision, int kahan, bool tile, int out_dtype, hipStream_t s);
 
 }  // namespace bem

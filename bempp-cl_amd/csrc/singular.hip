@@ -21,6 +21,7 @@ struct SingularArgs {
     const double *tpts, *spts, *wts;
     double kr, ki;
     void *out;
+    void *out2;  // DUAL: double layer (out: single layer)
     int64_t ld;
 };
 
@@ -31,7 +32,9 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
-template <int OP, int NTS, int NSS, typename OutT>
+// DUAL: OP is a single layer; the double layer of the same family (sharing r and e^{ikr} at
+// every point) goes to a.out2.
+template <int OP, int NTS, int NSS, typename OutT, bool DUAL = false>
 __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
     constexpr bool CPLX = OpTraits<OP>::is_complex;
     const int lane = threadIdx.x & 63;
@@ -46,11 +49,14 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
     double nx[3] = {gt[9] * tnm, gt[10] * tnm, gt[11] * tnm};
     double ny[3] = {gf[9] * snm, gf[10] * snm, gf[11] * snm};
 
-    double Sr[NTS][NSS], Si[NTS][NSS];
+    double Sr[NTS][NSS], Si[NTS][NSS], Sr2[DUAL ? NTS : 1][DUAL ? NSS : 1], Si2[DUAL ? NTS : 1][DUAL ? NSS : 1];
 #pragma unroll
     for (int i = 0; i < NTS; ++i)
 #pragma unroll
-        for (int j = 0; j < NSS; ++j) Sr[i][j] = Si[i][j] = 0.0;
+        for (int j = 0; j < NSS; ++j) {
+            Sr[i][j] = Si[i][j] = 0.0;
+            if constexpr (DUAL) Sr2[i][j] = Si2[i][j] = 0.0;
+        }
 
     for (int k = lane; k < np; k += 64) {
         double xi0 = a.tpts[2 * (int64_t)(toff + k)], xi1 = a.tpts[2 * (int64_t)(toff + k) + 1];
@@ -60,10 +66,12 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
         double dy = (gf[1] + gf[4] * et0 + gf[7] * et1) - (gt[1] + gt[4] * xi0 + gt[7] * xi1);
         double dz = (gf[2] + gf[5] * et0 + gf[8] * et1) - (gt[2] + gt[5] * xi0 + gt[8] * xi1);
         double r2 = dx * dx + dy * dy + dz * dz;
-        double gre, gim;
-        green<OP, double>(dx, dy, dz, r2, nx, ny, a.kr, a.ki, gre, gim);
+        double gre, gim, g2re, g2im;
+        green2<OP, DUAL, double>(dx, dy, dz, r2, nx, ny, a.kr, a.ki, gre, gim, g2re, g2im);
         gre *= w;
         gim *= w;
+        g2re *= w;
+        g2im *= w;
         double ut[3] = {1.0, xi0, xi1}, us[3] = {1.0, et0, et1};
 #pragma unroll
         for (int i = 0; i < NTS; ++i)
@@ -72,26 +80,43 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
                 double u = ut[i] * us[j];
                 Sr[i][j] += gre * u;
                 Si[i][j] += gim * u;
+                if constexpr (DUAL) {
+                    Sr2[i][j] += g2re * u;
+                    Si2[i][j] += g2im * u;
+                }
             }
     }
     const double jj = gt[12] * gf[12];
-    C2d S[NTS][NSS];
+    C2d S[NTS][NSS], S2[NTS][NSS];
 #pragma unroll
     for (int i = 0; i < NTS; ++i)
 #pragma unroll
         for (int j = 0; j < NSS; ++j) {
             S[i][j].re = wave_sum(Sr[i][j]) * jj;
             S[i][j].im = CPLX ? wave_sum(Si[i][j]) * jj : 0.0;
+            if constexpr (DUAL) {
+                S2[i][j].re = wave_sum(Sr2[i][j]) * jj;
+                S2[i][j].im = CPLX ? wave_sum(Si2[i][j]) * jj : 0.0;
+            }
         }
-    // lanes 0 .. NTS*NSS-1 each add one entry
-    if (lane < NTS * NSS) {
-        int i = lane / NSS, j = lane % NSS;
+    // lanes 0 .. NTS*NSS-1 each add one entry (DUAL: lanes NTS*NSS .. 2*NTS*NSS-1 the double layer)
+    const int nb = NTS * NSS;
+    if (lane < (DUAL ? 2 * nb : nb)) {
+        const bool second = DUAL && lane >= nb;
+        const int l = second ? lane - nb : lane;
+        int i = l / NSS, j = l % NSS;
         double m = a.test.mult[(int64_t)e * NTS + i] * a.trial.mult[(int64_t)f * NSS + j];
         if (m != 0.0) {
-            C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
+            C2d v;
+            if constexpr (DUAL) {
+                v = second ? local_value<OP + 1, NTS, NSS>(S2, i, j, gt, gf, a.kr, a.ki)
+                           : local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
+            } else {
+                v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
+            }
             int row = a.test.l2g[(int64_t)e * NTS + i];
             int col = a.trial.l2g[(int64_t)f * NSS + j];
-            OutT *o = reinterpret_cast<OutT *>(a.out);
+            OutT *o = reinterpret_cast<OutT *>(second ? a.out2 : a.out);
             int64_t idx = (int64_t)row * a.ld + col;
             if constexpr (CPLX) {
                 atomic_add(o + 2 * idx, (OutT)(v.re * m));
@@ -103,7 +128,7 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
     }
 }
 
-template <int OP, typename OutT>
+template <int OP, typename OutT, bool DUAL>
 static int launch_sing(const SingularArgs &a, hipStream_t s) {
     dim3 grid((a.n_pairs + 3) / 4);
     if (a.n_pairs == 0) return BEM_OK;
@@ -111,21 +136,21 @@ static int launch_sing(const SingularArgs &a, hipStream_t s) {
         hipLaunchKernelGGL((singular_kernel<OP, 3, 3, OutT>), grid, dim3(256), 0, s, a);
     } else {
         int nt = nshape_of(a.test.shape), ns = nshape_of(a.trial.shape);
-        if (nt == 1 && ns == 1) hipLaunchKernelGGL((singular_kernel<OP, 1, 1, OutT>), grid, dim3(256), 0, s, a);
-        else if (nt == 1 && ns == 3) hipLaunchKernelGGL((singular_kernel<OP, 1, 3, OutT>), grid, dim3(256), 0, s, a);
-        else if (nt == 3 && ns == 1) hipLaunchKernelGGL((singular_kernel<OP, 3, 1, OutT>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((singular_kernel<OP, 3, 3, OutT>), grid, dim3(256), 0, s, a);
+        if (nt == 1 && ns == 1) hipLaunchKernelGGL((singular_kernel<OP, 1, 1, OutT, DUAL>), grid, dim3(256), 0, s, a);
+        else if (nt == 1 && ns == 3) hipLaunchKernelGGL((singular_kernel<OP, 1, 3, OutT, DUAL>), grid, dim3(256), 0, s, a);
+        else if (nt == 3 && ns == 1) hipLaunchKernelGGL((singular_kernel<OP, 3, 1, OutT, DUAL>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((singular_kernel<OP, 3, 3, OutT, DUAL>), grid, dim3(256), 0, s, a);
     }
     return check_launch("singular_kernel");
 }
 
-template <int OP>
+template <int OP, bool DUAL = false>
 static int launch_sing_op(const SingularArgs &a, int out_dtype, hipStream_t s) {
     constexpr bool CPLX = OpTraits<OP>::is_complex;
-    if (CPLX && out_dtype == BEM_C128) return launch_sing<OP, double>(a, s);
-    if (CPLX && out_dtype == BEM_C64) return launch_sing<OP, float>(a, s);
-    if (!CPLX && out_dtype == BEM_F64) return launch_sing<OP, double>(a, s);
-    if (!CPLX && out_dtype == BEM_F32) return launch_sing<OP, float>(a, s);
+    if (CPLX && out_dtype == BEM_C128) return launch_sing<OP, double, DUAL>(a, s);
+    if (CPLX && out_dtype == BEM_C64) return launch_sing<OP, float, DUAL>(a, s);
+    if (!CPLX && out_dtype == BEM_F64) return launch_sing<OP, double, DUAL>(a, s);
+    if (!CPLX && out_dtype == BEM_F32) return launch_sing<OP, float, DUAL>(a, s);
     set_error("output dtype %d does not match operator %d", out_dtype, OP);
     return BEM_ERR_INVALID_ARGUMENT;
 }
@@ -134,10 +159,10 @@ static int launch_sing_op(const SingularArgs &a, int out_dtype, hipStream_t s) {
 
 using namespace bem;
 
-extern "C" int bem_singular_assemble(int op, const bem_space_t *test, const bem_space_t *trial, int n_pairs,
-                                     const int32_t *pair_data, const double *test_points, const double *trial_points,
-                                     const double *weights, double k_re, double k_im, void *out, int64_t ld,
-                                     int out_dtype, void *stream) {
+static int singular_entry(int op, const bem_space_t *test, const bem_space_t *trial, int n_pairs,
+                          const int32_t *pair_data, const double *test_points, const double *trial_points,
+                          const double *weights, double k_re, double k_im, void *out, void *out2, int64_t ld,
+                          int out_dtype, void *stream) {
     clear_error();
     if (!test || !trial || !out || (n_pairs > 0 && (!pair_data || !test_points || !trial_points || !weights))) {
         set_error("null argument");
@@ -160,8 +185,15 @@ extern "C" int bem_singular_assemble(int op, const bem_space_t *test, const bem_
     a.kr = k_re;
     a.ki = k_im;
     a.out = out;
+    a.out2 = out2;
     a.ld = ld;
     hipStream_t s = (hipStream_t)stream;
+    if (out2) {
+        if (op == BEM_LAPLACE_SINGLE_LAYER) return launch_sing_op<kLaplaceSL, true>(a, out_dtype, s);
+        if (op == BEM_HELMHOLTZ_SINGLE_LAYER) return launch_sing_op<kHelmholtzSL, true>(a, out_dtype, s);
+        set_error("the single/double layer pair needs a single-layer operator, got %d", op);
+        return BEM_ERR_UNSUPPORTED;
+    }
     switch (op) {
     case BEM_LAPLACE_SINGLE_LAYER: return launch_sing_op<kLaplaceSL>(a, out_dtype, s);
     case BEM_LAPLACE_DOUBLE_LAYER: return launch_sing_op<kLaplaceDL>(a, out_dtype, s);
@@ -172,4 +204,26 @@ extern "C" int bem_singular_assemble(int op, const bem_space_t *test, const bem_
     case BEM_MAXWELL_ELECTRIC_FIELD: return launch_sing_op<kMaxwellEFIE>(a, out_dtype, s);
     default: set_error("unsupported operator %d", op); return BEM_ERR_UNSUPPORTED;
     }
+}
+
+extern "C" int bem_singular_assemble(int op, const bem_space_t *test, const bem_space_t *trial, int n_pairs,
+                                     const int32_t *pair_data, const double *test_points, const double *trial_points,
+                                     const double *weights, double k_re, double k_im, void *out, int64_t ld,
+                                     int out_dtype, void *stream) {
+    return singular_entry(op, test, trial, n_pairs, pair_data, test_points, trial_points, weights, k_re, k_im, out,
+                          nullptr, ld, out_dtype, stream);
+}
+
+extern "C" int bem_singular_assemble_sl_dl(int op, const bem_space_t *test, const bem_space_t *trial, int n_pairs,
+                                           const int32_t *pair_data, const double *test_points,
+                                           const double *trial_points, const double *weights, double k_re, double k_im,
+                                           void *out_single, void *out_double, int64_t ld, int out_dtype,
+                                           void *stream) {
+    if (!out_double) {
+        clear_error();
+        set_error("null argument");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    return singular_entry(op, test, trial, n_pairs, pair_data, test_points, trial_points, weights, k_re, k_im,
+                          out_single, out_double, ld, out_dtype, stream);
 }

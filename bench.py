@@ -4,14 +4,17 @@
 Workload (one "step"): mixed-precision (FP32 far-field / FP64 near-field and singular,
 complex64 output) dense Galerkin assembly of the Helmholtz single-layer and double-layer
 operators, P1 x P1 continuous, wavenumber k = 4, on the level-5 icosahedral unit sphere
-(20,480 flat triangles, 10,242 vertices): two 10,242 x 10,242 complex64 matrices in HBM.
-Reported value: matrix entries per second (2 * 10,242^2 per step). Outside the timed region
-the same operators are assembled in FP64 to report the relative Frobenius error of the
-mixed result, the FP64 throughput, the potential evaluation of a seeded complex Gaussian P1
-density at 16,384 points on the sphere of radius 2, kernel-level roofline figures (HIP
-events on the launch stream) and the CPU baseline (FP64 oracle on the host cores, row sample).
+(20,480 flat triangles, 10,242 vertices): two 10,242 x 10,242 complex64 matrices in HBM,
+both assembled in one pass (shared point-pair geometry and e^{ikr}).
+Reported value: matrix entries per second (2 * 10,242^2 per step). Outside the timed region:
+the two operators assembled separately, the FP64 assemblies (separate = the plain path, and
+fused) with the relative Frobenius error of the mixed result, a Kahan-summation variant, the
+potential of a seeded complex Gaussian P1 density at 16,384 points on the sphere of radius 2,
+the frequency sweep (32 wavenumbers, DP0, level 4), the loss/gradient workload (level 6),
+kernel-level roofline figures (HIP events on the launch stream) and the CPU baseline (FP64
+oracle on the host cores, row sample).
 
-Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-sweep] [--no-adjoint]
 """
 
 import argparse
@@ -376,7 +379,7 @@ def main():
     from bempp_amd.api.operators.boundary import helmholtz
     from bempp_amd.api.operators.potential import helmholtz as hpot
     from bempp_amd.core import hip_backend as hb
-    from bempp_amd.core.dense_assembler import hip_dense_assemble
+    from bempp_amd.core.dense_assembler import hip_dense_assemble, hip_dense_assemble_sl_dl
     from bempp_amd.core.singular_assembler import singular_rule
 
     torch.cuda.set_device(0)
@@ -404,17 +407,18 @@ def main():
     n_regular_pairs = F * F - n_sing
     n_evals = n_regular_pairs * 36
     outs = {name: torch.empty((N, N), dtype=torch.complex64, device="cuda") for name in ops}
+    SL, DL = "helmholtz_single_layer", "helmholtz_double_layer"
+    evkeys = ("regular_start", "regular_end", "singular_end")
 
     def step(events=None):
-        for name in ops:
-            hip_dense_assemble(space, space, params, desc[name], out=outs[name], accumulate=False,
-                               events=None if events is None else events[name])
+        # single and double layer in one pass (shared point-pair geometry and e^{ikr})
+        hip_dense_assemble_sl_dl(space, space, params, desc[SL], desc[DL], out_single=outs[SL],
+                                 out_double=outs[DL], events=events)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    ev = [{name: {key: torch.cuda.Event(enable_timing=True) for key in ("regular_start", "regular_end", "singular_end")}
-           for name in ops} for _ in range(args.steps)]
+    ev = [{key: torch.cuda.Event(enable_timing=True) for key in evkeys} for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
@@ -424,14 +428,30 @@ def main():
     ms_per_step = 1e3 * (t1 - t0) / args.steps
     entries_per_step = 2 * N * N
     value = entries_per_step / ((t1 - t0) / args.steps)
-    kern = {}
+    reg = float(np.mean([e["regular_start"].elapsed_time(e["regular_end"]) for e in ev]))
+    sing = float(np.mean([e["regular_end"].elapsed_time(e["singular_end"]) for e in ev]))
+    flop_pair = FLOP_PER_EVAL[SL] + FLOP_PER_EVAL[DL]
+    kern = {"fused_regular_ms": reg, "fused_near_singular_ms": sing,
+            "fused_regular_tflops": n_evals * flop_pair / (reg * 1e-3) / 1e12}
+    log(f"mixed fused step {ms_per_step:.2f} ms, {value / 1e9:.2f} G entries/s; kernels {kern}")
+
+    # the same two operators assembled separately (one weak_form() each)
+    sep_ev = {name: {key: torch.cuda.Event(enable_timing=True) for key in evkeys} for name in ops}
     for name in ops:
-        reg = np.mean([e[name]["regular_start"].elapsed_time(e[name]["regular_end"]) for e in ev])
-        sing = np.mean([e[name]["regular_end"].elapsed_time(e[name]["singular_end"]) for e in ev])
-        flops = n_evals * FLOP_PER_EVAL[name]
-        kern[name] = {"regular_ms": float(reg), "singular_ms": float(sing),
-                      "regular_tflops": flops / (reg * 1e-3) / 1e12}
-    log(f"mixed step {ms_per_step:.2f} ms, {value / 1e9:.2f} G entries/s; kernels {kern}")
+        hip_dense_assemble(space, space, params, desc[name], out=outs[name], accumulate=False)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for name in ops:
+        hip_dense_assemble(space, space, params, desc[name], out=outs[name], accumulate=False, events=sep_ev[name])
+    torch.cuda.synchronize()
+    sep_ms = 1e3 * (time.perf_counter() - t)
+    for name in ops:
+        r = sep_ev[name]["regular_start"].elapsed_time(sep_ev[name]["regular_end"])
+        kern[name] = {"regular_ms": r, "singular_ms": sep_ev[name]["regular_end"].elapsed_time(sep_ev[name]["singular_end"]),
+                      "regular_tflops": n_evals * FLOP_PER_EVAL[name] / (r * 1e-3) / 1e12}
+    kern["separate_ms_per_step"] = sep_ms
+    step()  # leave the fused result in outs for the error check
+    torch.cuda.synchronize()
 
     result = {
         "metric": METRIC,
@@ -448,7 +468,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "Helmholtz single+double layer, P1xP1, icosphere level 5 (20480 triangles, 10242 vertices), "
-                        "k=4.0, mixed FP32/FP64 -> complex64, dense in HBM",
+                        "k=4.0, mixed FP32/FP64 -> complex64, dense in HBM, both operators in one pass",
             "triangles": F,
             "dofs": N,
             "wavenumber": k,
@@ -457,55 +477,66 @@ def main():
             "singular_pairs": n_sing,
         },
     }
-    name0 = "helmholtz_single_layer"
-    sl = kern[name0]
     result["roofline"] = {
-        "kernel": f"regular_kernel<{name0}, P1xP1, mixed>",
+        "kernel": "regular_kernel<helmholtz_single_layer + double layer (fused), P1xP1, mixed>",
         "bound": "valu",
-        "achieved": sl["regular_tflops"],
+        "achieved": kern["fused_regular_tflops"],
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
-        "frac": sl["regular_tflops"] / PEAK_FP32_TFLOPS,
-        "traffic": traffic_from_profiles("regular_kernel_helmholtz_single_layer_p1_level5_mixed"),
-        "algorithmic_flop_per_launch": n_evals * FLOP_PER_EVAL[name0],
-        "flop_per_eval": FLOP_PER_EVAL[name0],
+        "frac": kern["fused_regular_tflops"] / PEAK_FP32_TFLOPS,
+        "traffic": traffic_from_profiles("regular_kernel_helmholtz_sl_dl_p1_level5_mixed"),
+        "algorithmic_flop_per_launch": n_evals * flop_pair,
+        "flop_per_eval": f"{FLOP_PER_EVAL[SL]} (single layer) + {FLOP_PER_EVAL[DL]} (double layer)",
         "evals_per_launch": n_evals,
-        "launch_ms": sl["regular_ms"],
+        "launch_ms": reg,
     }
     result["kernels"] = kern
 
     if not args.no_reference:
+        # FP64: the plain path (one assembly per operator) and the fused FP64 pass
         errs, t64 = {}, 0.0
-        for name, (f, _) in ops.items():
+        a64 = {}
+        for name in ops:
             torch.cuda.synchronize()
             t = time.perf_counter()
-            a64 = hip_dense_assemble(space, space, params, desc64[name])
+            a64[name] = hip_dense_assemble(space, space, params, desc64[name])
             torch.cuda.synchronize()
             t64 += time.perf_counter() - t
-            errs[name] = rel_fro(outs[name], a64)
-            del a64
+            errs[name] = rel_fro(outs[name], a64[name])
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        f1, f2 = hip_dense_assemble_sl_dl(space, space, params, desc64[SL], desc64[DL])
+        torch.cuda.synchronize()
+        t64f = time.perf_counter() - t
+        fused64_err = max(rel_fro(f1, a64[SL]), rel_fro(f2, a64[DL]))
+        del f1, f2, a64
         result["rel_err_vs_fp64"] = errs
         result["fp64_entries_per_s"] = entries_per_step / t64
         result["speedup_vs_fp64"] = value / result["fp64_entries_per_s"]
-        log(f"fp64 step {1e3 * t64:.1f} ms, rel err {errs}")
+        result["fp64_fused_entries_per_s"] = entries_per_step / t64f
+        result["speedup_vs_fp64_fused"] = value / result["fp64_fused_entries_per_s"]
+        result["fp64_fused_vs_separate_rel_err"] = fused64_err
+        result["speedup_separate_mixed_vs_fp64"] = t64 / (sep_ms * 1e-3)
+        log(f"fp64 separate {1e3 * t64:.1f} ms, fused {1e3 * t64f:.1f} ms, rel err {errs}")
 
         # the same step with Kahan-compensated FP32 sums (parameters.hip.kahan)
         from bempp_amd.api.utils.parameters import DefaultParameters
 
         pk = DefaultParameters()
         pk.hip.kahan = True
-        kres = {}
-        for name, (f, _) in ops.items():
-            dk = f(space, space, space, k, precision="single", parameters=pk).descriptor
-            hip_dense_assemble(space, space, pk, dk, out=outs[name], accumulate=False)
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            hip_dense_assemble(space, space, pk, dk, out=outs[name], accumulate=False)
-            torch.cuda.synchronize()
-            ms = 1e3 * (time.perf_counter() - t)
-            a64 = hip_dense_assemble(space, space, params, desc64[name])
-            kres[name] = {"ms": ms, "rel_err_vs_fp64": rel_fro(outs[name], a64)}
-            del a64
+        dk = {name: f(space, space, space, k, precision="single", parameters=pk).descriptor
+              for name, (f, _) in ops.items()}
+        hip_dense_assemble_sl_dl(space, space, pk, dk[SL], dk[DL], out_single=outs[SL], out_double=outs[DL])
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        hip_dense_assemble_sl_dl(space, space, pk, dk[SL], dk[DL], out_single=outs[SL], out_double=outs[DL])
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t)
+        kres = {"ms_per_step": ms}
+        for name in ops:
+            ref = hip_dense_assemble(space, space, params, desc64[name])
+            kres[name] = {"rel_err_vs_fp64": rel_fro(outs[name], ref)}
+            del ref
         result["kahan_variant"] = kres
         log(f"kahan variant {kres}")
 

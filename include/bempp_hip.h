@@ -142,6 +142,22 @@ int bem_regular_assemble_blocked(int op, const bem_space_t *test, const bem_spac
                                  int out_dtype, void *stream);
 
 /*
+ * Single layer and double layer of one family in one pass (no single reference counterpart:
+ * the reference assembles each operator separately, bempp_cl/core/numba_kernels.py:980-1074
+ * with the kernels of numba_kernels.py:176-320). op: BEM_LAPLACE_SINGLE_LAYER or
+ * BEM_HELMHOLTZ_SINGLE_LAYER; the matching double layer goes to out_double. Both operators
+ * share the spaces, the wavenumber and ld/out_dtype; every point pair's distance, 1/r and
+ * e^{ikr} are computed once for both. With test_blocks/trial_blocks NULL this behaves as
+ * bem_regular_assemble (localised spaces, `accumulate` as there), otherwise as
+ * bem_regular_assemble_blocked (sums added to both outputs).
+ */
+int bem_regular_assemble_sl_dl(int op, const bem_space_t *test, const bem_space_t *trial,
+                               const bem_dof_blocks_t *test_blocks, const bem_dof_blocks_t *trial_blocks,
+                               int grids_identical, const bem_quad_rule_t *rule, double k_re, double k_im,
+                               int precision, double mixed_near_factor, int kahan, void *out_single, void *out_double,
+                               int64_t ld, int out_dtype, int accumulate, void *stream);
+
+/*
  * Near-field regular pairs of the MIXED mode, in FP64 (no reference counterpart: the
  * reference computes every regular pair in one precision). The MIXED regular kernels skip
  * every non-adjacent pair with |c_test - c_trial| < factor * (diam_test + diam_trial) / 2;
@@ -152,6 +168,11 @@ int bem_regular_assemble_blocked(int op, const bem_space_t *test, const bem_spac
 int bem_near_assemble(int op, const bem_space_t *test, const bem_space_t *trial, int n_pairs, const int32_t *pairs,
                       int grids_identical, const bem_quad_rule_t *rule, double k_re, double k_im,
                       double near_factor, void *out, int64_t ld, int out_dtype, void *stream);
+/* bem_near_assemble for a single layer (op) and its double layer in one pass (see bem_regular_assemble_sl_dl). */
+int bem_near_assemble_sl_dl(int op, const bem_space_t *test, const bem_space_t *trial, int n_pairs,
+                            const int32_t *pairs, int grids_identical, const bem_quad_rule_t *rule, double k_re,
+                            double k_im, double near_factor, void *out_single, void *out_double, int64_t ld,
+                            int out_dtype, void *stream);
 
 /*
  * Singular (Sauter-Schwab) part, always FP64: for each of n_pairs pairs p,
@@ -166,6 +187,11 @@ int bem_singular_assemble(int op, const bem_space_t *test, const bem_space_t *tr
                           const int32_t *pair_data, const double *test_points, const double *trial_points,
                           const double *weights, double k_re, double k_im, void *out, int64_t ld, int out_dtype,
                           void *stream);
+/* bem_singular_assemble for a single layer (op) and its double layer in one pass (shared r and e^{ikr}). */
+int bem_singular_assemble_sl_dl(int op, const bem_space_t *test, const bem_space_t *trial, int n_pairs,
+                                const int32_t *pair_data, const double *test_points, const double *trial_points,
+                                const double *weights, double k_re, double k_im, void *out_single, void *out_double,
+                                int64_t ld, int out_dtype, void *stream);
 
 /*
  * Dense potential u(x_m) (bempp_cl/core/numba_kernels.py:1788-1849, 2281-2359):
