@@ -13,6 +13,8 @@ import numpy as _np
 # limits of the device tile (BEM_TILE_* in include/bempp_hip.h)
 TEST_TILE_ELEMENTS = 16
 TEST_TILE_DOFS = 20
+PAIR_TEST_TILE_ELEMENTS = 8  # single + double layer in one pass (BEM_TILE_PAIR_TEST_*)
+PAIR_TEST_TILE_DOFS = 12
 TRIAL_BLOCK_ELEMENTS = 256
 TRIAL_BLOCK_DOFS = 160
 MAX_INCIDENCE = 12
@@ -171,9 +173,11 @@ _DEVICE_CACHE = {}
 
 
 def device_blocks(space, role):
-    """Device blocking of a space as test ('test') or trial ('trial') side."""
+    """Device blocking of a space as test ('test', 'test_pair' for the fused pair) or trial ('trial') side."""
     if role == "test":
         me, md = TEST_TILE_ELEMENTS, TEST_TILE_DOFS
+    elif role == "test_pair":
+        me, md = PAIR_TEST_TILE_ELEMENTS, PAIR_TEST_TILE_DOFS
     else:
         me, md = TRIAL_BLOCK_ELEMENTS, TRIAL_BLOCK_DOFS
     key = (space.id, me, md)

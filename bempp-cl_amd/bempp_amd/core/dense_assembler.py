@@ -238,7 +238,8 @@ def hip_dense_assemble_sl_dl(domain, dual_to_range, parameters, sl_descriptor, d
     else:
         from bempp_amd.core.blocking import device_blocks
 
-        hb.regular_assemble_sl_dl(op, test, trial, device_blocks(dual_to_range, "test"), device_blocks(domain, "trial"),
+        hb.regular_assemble_sl_dl(op, test, trial, device_blocks(dual_to_range, "test_pair"),
+                                  device_blocks(domain, "trial"),
                                   grids_identical, rule, k, mode, outs[0], outs[1], near_factor=near, kahan=kahan)
     if events is not None:
         events["regular_end"].record()

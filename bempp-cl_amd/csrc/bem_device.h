@@ -38,14 +38,46 @@ struct OpTraits {
 
 __device__ __forceinline__ double rsqrt_t(double x) { return 1.0 / sqrt(x); }
 __device__ __forceinline__ float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
-__device__ __forceinline__ void sincos_t(double x, double *s, double *c) { sincos(x, s, c); }
+
+// sin and cos of |x| <= 1 from their Taylor series through x^17 and x^18 (remainders
+// |x|^19/19! and |x|^20/20!, below 1e-17), with no range reduction.
+__device__ __forceinline__ void sincos_small(double x, double *s, double *c) {
+    const double x2 = x * x;
+    double ps = 1.0 / 355687428096000.0;  // 1/17!
+    ps = ps * x2 - 1.0 / 1307674368000.0;  // 1/15!
+    ps = ps * x2 + 1.0 / 6227020800.0;     // 1/13!
+    ps = ps * x2 - 1.0 / 39916800.0;       // 1/11!
+    ps = ps * x2 + 1.0 / 362880.0;         // 1/9!
+    ps = ps * x2 - 1.0 / 5040.0;
+    ps = ps * x2 + 1.0 / 120.0;
+    ps = ps * x2 - 1.0 / 6.0;
+    *s = x + x * x2 * ps;
+    double pc = 1.0 / 6402373705728000.0;  // 1/18!
+    pc = pc * x2 - 1.0 / 20922789888000.0;  // 1/16!
+    pc = pc * x2 + 1.0 / 87178291200.0;     // 1/14!
+    pc = pc * x2 - 1.0 / 479001600.0;       // 1/12!
+    pc = pc * x2 + 1.0 / 3628800.0;         // 1/10!
+    pc = pc * x2 - 1.0 / 40320.0;
+    pc = pc * x2 + 1.0 / 720.0;
+    pc = pc * x2 - 1.0 / 24.0;
+    pc = pc * x2 + 0.5;
+    *c = 1.0 - x2 * pc;
+}
+
+// SMALL: the caller guarantees |x| <= 1 (FP64 only)
+template <bool SMALL = false>
+__device__ __forceinline__ void sincos_t(double x, double *s, double *c) {
+    if constexpr (SMALL) sincos_small(x, s, c);
+    else sincos(x, s, c);
+}
+template <bool SMALL = false>
 __device__ __forceinline__ void sincos_t(float x, float *s, float *c) { sincosf(x, s, c); }
 __device__ __forceinline__ double exp_t(double x) { return exp(x); }
 __device__ __forceinline__ float exp_t(float x) { return __expf(x); }
 
 // Green's function of operator OP for d = y - x, r2 = |d|^2.
 // nx: test normal (ADL), ny: trial normal (DL). Returns (re, im).
-template <int OP, typename T>
+template <int OP, typename T, bool SMALL = false>
 __device__ __forceinline__ void green(T dx, T dy, T dz, T r2, const T *nx, const T *ny, T kr, T ki, T &gre, T &gim) {
     T rinv = rsqrt_t(r2);
     if constexpr (OP == kLaplaceSL) {
@@ -62,7 +94,7 @@ __device__ __forceinline__ void green(T dx, T dy, T dz, T r2, const T *nx, const
     } else if constexpr (OP == kHelmholtzSL || OP == kMaxwellEFIE) {
         T r = r2 * rinv;
         T s, c;
-        sincos_t(kr * r, &s, &c);
+        sincos_t<SMALL>(kr * r, &s, &c);
         T g = T(kInv4Pi) * rinv;
         if (ki != T(0)) g *= exp_t(-ki * r);
         gre = c * g;
@@ -73,7 +105,7 @@ __device__ __forceinline__ void green(T dx, T dy, T dz, T r2, const T *nx, const
         else dot = -(dx * nx[0] + dy * nx[1] + dz * nx[2]);
         T r = r2 * rinv;
         T s, c;
-        sincos_t(kr * r, &s, &c);
+        sincos_t<SMALL>(kr * r, &s, &c);
         T g = T(kInv4Pi) * dot * rinv * rinv * rinv;
         if (ki != T(0)) g *= exp_t(-ki * r);
         T fr = c * g, fi = s * g;
@@ -85,7 +117,7 @@ __device__ __forceinline__ void green(T dx, T dy, T dz, T r2, const T *nx, const
 
 // Single and double layer of one family at the same point pair, sharing r, 1/r and e^{ikr}:
 // (g1re, g1im) = SL, (g2re, g2im) = DL with the trial normal ny.
-template <bool HELM, typename T>
+template <bool HELM, typename T, bool SMALL = false>
 __device__ __forceinline__ void green_sl_dl(T dx, T dy, T dz, T r2, const T *ny, T kr, T ki, T &g1re, T &g1im,
                                             T &g2re, T &g2im) {
     const T rinv = rsqrt_t(r2);
@@ -98,7 +130,7 @@ __device__ __forceinline__ void green_sl_dl(T dx, T dy, T dz, T r2, const T *ny,
     } else {
         const T r = r2 * rinv;
         T s, c;
-        sincos_t(kr * r, &s, &c);
+        sincos_t<SMALL>(kr * r, &s, &c);
         T g = T(kInv4Pi) * rinv;
         if (ki != T(0)) g *= exp_t(-ki * r);
         g1re = c * g;
@@ -113,13 +145,13 @@ __device__ __forceinline__ void green_sl_dl(T dx, T dy, T dz, T r2, const T *ny,
 
 // Green's function of OP and, for DUAL (OP a single layer), also the double layer of the same
 // family at the same point pair (shared r and e^{ikr}).
-template <int OP, bool DUAL, typename T>
+template <int OP, bool DUAL, typename T, bool SMALL = false>
 __device__ __forceinline__ void green2(T dx, T dy, T dz, T r2, const T *nx, const T *ny, T kr, T ki, T &g1re, T &g1im,
                                        T &g2re, T &g2im) {
     if constexpr (DUAL) {
-        green_sl_dl<OpTraits<OP>::is_complex, T>(dx, dy, dz, r2, ny, kr, ki, g1re, g1im, g2re, g2im);
+        green_sl_dl<OpTraits<OP>::is_complex, T, SMALL>(dx, dy, dz, r2, ny, kr, ki, g1re, g1im, g2re, g2im);
     } else {
-        green<OP, T>(dx, dy, dz, r2, nx, ny, kr, ki, g1re, g1im);
+        green<OP, T, SMALL>(dx, dy, dz, r2, nx, ny, kr, ki, g1re, g1im);
         g2re = g2im = T(0);
     }
 }

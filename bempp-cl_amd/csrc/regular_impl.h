@@ -434,7 +434,7 @@ __device__ __forceinline__ void store_direct(void *out, int64_t ld, int row, int
 // into a.out2 from the same point-pair geometry and phase factors.
 template <int OP, int NTS, int NSS, int QT, int TM, bool MIXED, bool KAHAN, bool TILE, int MAXU_T, int MAXU_S,
           typename OutT, bool DUAL = false>
-__global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs a) {
+__global__ __launch_bounds__(kRegThreads, DUAL ? 3 : 1) void regular_kernel(const RegularArgs a) {
     constexpr bool CPLX = OpTraits<OP>::is_complex;
     constexpr int NT = NTS, NS = NSS;
     constexpr int NC = CPLX ? 2 : 1;
@@ -756,10 +756,11 @@ __global__ __launch_bounds__(kRegThreads) void regular_kernel(const RegularArgs 
 // ------------------------------------------------------------------ dispatch
 template <int OP, int NTS, int NSS, int QT, bool MIXED, typename OutT, bool DUAL = false>
 static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
-    constexpr int TM = 16;
+    // DUAL keeps two LDS tiles; its smaller test tiles keep the LDS footprint at 3 workgroups per CU
+    constexpr int TM = DUAL ? BEM_TILE_PAIR_TEST_ELEMENTS : BEM_TILE_TEST_ELEMENTS;
     constexpr bool CPLX = OpTraits<OP>::is_complex;
     // LDS tile of distinct dofs: test slots x trial slots (BEM_TILE_TEST_DOFS x BEM_TILE_TRIAL_DOFS)
-    constexpr int MT = BEM_TILE_TEST_DOFS;
+    constexpr int MT = DUAL ? BEM_TILE_PAIR_TEST_DOFS : BEM_TILE_TEST_DOFS;
     constexpr int MS = BEM_TILE_TRIAL_DOFS;
     (void)CPLX;
     dim3 grid;

@@ -58,34 +58,41 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
             if constexpr (DUAL) Sr2[i][j] = Si2[i][j] = 0.0;
         }
 
-    for (int k = lane; k < np; k += 64) {
-        double xi0 = a.tpts[2 * (int64_t)(toff + k)], xi1 = a.tpts[2 * (int64_t)(toff + k) + 1];
-        double et0 = a.spts[2 * (int64_t)(soff + k)], et1 = a.spts[2 * (int64_t)(soff + k) + 1];
-        double w = a.wts[woff + k];
-        double dx = (gf[0] + gf[3] * et0 + gf[6] * et1) - (gt[0] + gt[3] * xi0 + gt[6] * xi1);
-        double dy = (gf[1] + gf[4] * et0 + gf[7] * et1) - (gt[1] + gt[4] * xi0 + gt[7] * xi1);
-        double dz = (gf[2] + gf[5] * et0 + gf[8] * et1) - (gt[2] + gt[5] * xi0 + gt[8] * xi1);
-        double r2 = dx * dx + dy * dy + dz * dz;
-        double gre, gim, g2re, g2im;
-        green2<OP, DUAL, double>(dx, dy, dz, r2, nx, ny, a.kr, a.ki, gre, gim, g2re, g2im);
-        gre *= w;
-        gim *= w;
-        g2re *= w;
-        g2im *= w;
-        double ut[3] = {1.0, xi0, xi1}, us[3] = {1.0, et0, et1};
+    // every distance within the pair is at most diam_t + diam_f (triangles sharing a vertex), so a
+    // wave-uniform bound selects the series sin/cos without range reduction when k r <= 1
+    auto points = [&](auto small_tag) {
+        constexpr bool SMALL = decltype(small_tag)::value;
+        for (int k = lane; k < np; k += 64) {
+            double xi0 = a.tpts[2 * (int64_t)(toff + k)], xi1 = a.tpts[2 * (int64_t)(toff + k) + 1];
+            double et0 = a.spts[2 * (int64_t)(soff + k)], et1 = a.spts[2 * (int64_t)(soff + k) + 1];
+            double w = a.wts[woff + k];
+            double dx = (gf[0] + gf[3] * et0 + gf[6] * et1) - (gt[0] + gt[3] * xi0 + gt[6] * xi1);
+            double dy = (gf[1] + gf[4] * et0 + gf[7] * et1) - (gt[1] + gt[4] * xi0 + gt[7] * xi1);
+            double dz = (gf[2] + gf[5] * et0 + gf[8] * et1) - (gt[2] + gt[5] * xi0 + gt[8] * xi1);
+            double r2 = dx * dx + dy * dy + dz * dz;
+            double gre, gim, g2re, g2im;
+            green2<OP, DUAL, double, SMALL>(dx, dy, dz, r2, nx, ny, a.kr, a.ki, gre, gim, g2re, g2im);
+            gre *= w;
+            gim *= w;
+            g2re *= w;
+            g2im *= w;
+            double ut[3] = {1.0, xi0, xi1}, us[3] = {1.0, et0, et1};
 #pragma unroll
-        for (int i = 0; i < NTS; ++i)
+            for (int i = 0; i < NTS; ++i)
 #pragma unroll
-            for (int j = 0; j < NSS; ++j) {
-                double u = ut[i] * us[j];
-                Sr[i][j] += gre * u;
-                Si[i][j] += gim * u;
-                if constexpr (DUAL) {
-                    Sr2[i][j] += g2re * u;
-                    Si2[i][j] += g2im * u;
+                for (int j = 0; j < NSS; ++j) {
+                    double u = ut[i] * us[j];
+                    Sr[i][j] += gre * u;
+                    Si[i][j] += gim * u;
+                    if constexpr (DUAL) {
+                        Sr2[i][j] += g2re * u;
+                        Si2[i][j] += g2im * u;
+                    }
                 }
-            }
-    }
+        }
+    };
+    if (fabs(a.kr) * (gt[13] + gf[13]) <= 1.0) points(std::true_type{});
+    else points(std::false_type{});
     const double jj = gt[12] * gf[12];
     C2d S[NTS][NSS], S2[NTS][NSS];
 #pragma unroll

@@ -54,6 +54,7 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB device to device (reference dispatch for HBM byte counters)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the frequency-sweep measurement")
+    ap.add_argument("--no-efie", action="store_true", help="skip the Maxwell EFIE measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the loss/gradient measurement")
     ap.add_argument("--adjoint-level", type=int, default=6)
     ap.add_argument("--adjoint-k", type=float, default=8.0)
@@ -240,6 +241,82 @@ def _full_dp0(space, coeffs):
     l2g = torch.from_numpy(space.local2global[:, 0].astype(np.int64)).to(coeffs.device)
     mult = torch.from_numpy(np.ascontiguousarray(space.local_multipliers[:, 0])).to(coeffs.device)
     return (coeffs[l2g] * mult).contiguous()
+
+
+def efie_bench(level, k, reps):
+    """Maxwell electric field operator, RWG x SNC, and the E-field potential at 65,536 points.
+
+    Mixed assembly (complex64) timed with HIP events; compared with the FP64 assembly. The
+    potential of a seeded complex Gaussian RWG density is evaluated on the radius-10 sphere.
+    """
+    import numpy as np
+    import torch
+
+    import bempp_amd.api as bempp
+    from bempp_amd.api.operators.boundary import maxwell
+    from bempp_amd.api.operators.potential import maxwell as mpot
+    from bempp_amd.core.dense_assembler import hip_dense_assemble
+    from bempp_amd.core.singular_assembler import singular_rule
+
+    grid = bempp.shapes.icosphere(level)
+    rwg = bempp.function_space(grid, "RWG", 0)
+    snc = bempp.function_space(grid, "SNC", 0)
+    params = bempp.GLOBAL_PARAMETERS
+    F, N = grid.number_of_elements, rwg.global_dof_count
+    d32 = maxwell.electric_field(rwg, rwg, snc, k, precision="single").descriptor
+    d64 = maxwell.electric_field(rwg, rwg, snc, k, precision="double").descriptor
+    out = hip_dense_assemble(rwg, snc, params, d32)
+    torch.cuda.synchronize()
+    evs = [{key: _ev() for key in ("regular_start", "regular_end", "singular_end")} for _ in range(reps)]
+    t0 = time.perf_counter()
+    for e in evs:
+        hip_dense_assemble(rwg, snc, params, d32, out=out, accumulate=False, events=e)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    reg = float(np.mean([e["regular_start"].elapsed_time(e["regular_end"]) for e in evs]))
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    ref = hip_dense_assemble(rwg, snc, params, d64)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t
+    err = rel_fro(out, ref)
+    del ref, out
+    n_sing = singular_rule(grid, params.quadrature.singular, rwg.support, snc.support).host.number_of_pairs
+    n_evals = (F * F - n_sing) * 36
+    # E-field potential: 65,536 points on the radius-10 sphere, complex Gaussian RWG density
+    rng = np.random.default_rng(2)
+    c = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    g = np.random.default_rng(3).standard_normal((3, 65536))
+    pts = 10.0 * g / np.linalg.norm(g, axis=0)
+    fun = bempp.GridFunction(rwg, coefficients=c)
+    p32 = mpot.electric_field(rwg, pts, k, precision="single")
+    p64 = mpot.electric_field(rwg, pts, k, precision="double")
+    u64 = p64.evaluate(fun)
+    p32.evaluate(fun)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        u32 = p32.evaluate(fun)
+    t_pot = (time.perf_counter() - t) / reps
+    return {
+        "workload": f"Maxwell EFIE RWG x SNC, icosphere level {level} ({F} triangles, {N} edges), k={k}, "
+                    "mixed -> complex64; E-field potential at 65536 points, radius 10",
+        "entries_per_s": N * N / wall,
+        "ms": 1e3 * wall,
+        "regular_kernel_ms": reg,
+        "regular_tflops": n_evals * 73 / (reg * 1e-3) / 1e12,
+        "flop_per_eval": 73,
+        "rel_err_vs_fp64": err,
+        "fp64_ms": 1e3 * t64,
+        "speedup_vs_fp64": t64 / wall,
+        "potential": {
+            "points": pts.shape[1],
+            "evals_per_s": pts.shape[1] / t_pot,
+            "ms": 1e3 * t_pot,
+            "tflops": pts.shape[1] * F * 6 * 91 / t_pot / 1e12,
+            "rel_err_vs_fp64": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
+        },
+    }
 
 
 def frequency_sweep_bench(level, reps):
@@ -565,6 +642,10 @@ def main():
             "rel_err_vs_fp64": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
         }
         log(f"potential {result['potential']}")
+
+    if not args.no_efie:
+        result["efie"] = efie_bench(4, 2.0, 3)
+        log(f"efie {result['efie']}")
 
     if not args.no_sweep:
         result["frequency_sweep"] = frequency_sweep_bench(args.sweep_level, args.sweep_reps)

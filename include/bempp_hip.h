@@ -113,6 +113,9 @@ typedef struct bem_dof_blocks {
 #define BEM_TILE_TRIAL_ELEMENTS 256
 #define BEM_TILE_TRIAL_DOFS 160
 #define BEM_TILE_MAX_INCIDENCE 12
+/* test tiles of bem_regular_assemble_sl_dl with blocks (two LDS tiles) */
+#define BEM_TILE_PAIR_TEST_ELEMENTS 8
+#define BEM_TILE_PAIR_TEST_DOFS 12
 
 /*
  * Regular part of a dense Galerkin operator: all (test, trial) element pairs, except,
@@ -149,7 +152,8 @@ int bem_regular_assemble_blocked(int op, const bem_space_t *test, const bem_spac
  * share the spaces, the wavenumber and ld/out_dtype; every point pair's distance, 1/r and
  * e^{ikr} are computed once for both. With test_blocks/trial_blocks NULL this behaves as
  * bem_regular_assemble (localised spaces, `accumulate` as there), otherwise as
- * bem_regular_assemble_blocked (sums added to both outputs).
+ * bem_regular_assemble_blocked (sums added to both outputs) with test blocks of at most
+ * BEM_TILE_PAIR_TEST_ELEMENTS elements and BEM_TILE_PAIR_TEST_DOFS dofs.
  */
 int bem_regular_assemble_sl_dl(int op, const bem_space_t *test, const bem_space_t *trial,
                                const bem_dof_blocks_t *test_blocks, const bem_dof_blocks_t *trial_blocks,

@@ -102,3 +102,21 @@ def test_potentials_vs_oracle(op, sk, k, ico2, default_parameters, hip_available
     assert rel_fro(u64, ref) < 1e-12
     u32 = factory(*args, parameters=default_parameters, precision="single").evaluate(fun)
     assert rel_fro(u32, u64) < 1e-6
+
+
+@pytest.mark.parametrize("k", [2.5, 0.5 + 0.2j])
+def test_small_kr_singular_series_vs_oracle(k, default_parameters, hip_available):
+    """Level 3 (diameters ~0.15): k (diam_t + diam_f) <= 1 on every singular pair, so the singular
+    kernel evaluates e^{ikr} from its Taylor series; FP64 must still agree with the oracle."""
+    from bempp_amd.api import function_space
+    from bempp_amd.api.grid.shapes import icosphere
+    from bempp_amd.api.operators.boundary import helmholtz
+    from oracle import oracle as O
+
+    g = icosphere(3)
+    assert abs(k.real if isinstance(k, complex) else k) * 2 * g.diameters.max() <= 1.0
+    p1 = function_space(g, "P", 1)
+    for fn, name in ((helmholtz.single_layer, "helmholtz_single_layer"), (helmholtz.double_layer, "helmholtz_double_layer")):
+        A = fn(p1, p1, p1, k, parameters=default_parameters, precision="double").weak_form().to_dense()
+        ref = O.assemble(name, g.vertices, g.elements, p1, p1, k)
+        assert rel_fro(A, ref) < 1e-12
