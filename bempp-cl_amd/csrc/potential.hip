@@ -168,13 +168,17 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_fp64(const PotArgs
     }
 }
 
-// MIXED evaluation: FP32 per source, FP64 per (target, tile)
+// MIXED evaluation: FP32 per source, FP64 per (target, group of sources). EFIE uses groups of
+// 32 sources (a few neighbouring elements) as references, the scalar kernels one group per tile.
 template <int OP>
 __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed(const PotArgs a) {
     constexpr int KD = OP == kMaxwellEFIE ? 3 : 1;
     constexpr int NF = OP == kMaxwellEFIE ? kSrcFields : 8;
+    constexpr int GS = OP == kMaxwellEFIE ? 32 : kPotTile;  // sources per reference group
+    constexpr int NG = kPotTile / GS;
     __shared__ float sh[NF][kPotTile];
-    __shared__ double sc[3];
+    __shared__ double sc[NG][3];
+    __shared__ double wsum[NG][OP == kMaxwellEFIE ? 8 : 1];
     const int m = blockIdx.x * kPotThreads + threadIdx.x;
     const int chunk = blockIdx.y;
     const int s0 = chunk * a.chunk_len;
@@ -191,20 +195,38 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed(const PotArg
     for (int t0 = s0; t0 < s1; t0 += kPotTile) {
         const int nt = min(kPotTile, s1 - t0);
         __syncthreads();
-        if (threadIdx.x == 0) {
-            int sm = t0 + nt / 2;
-            for (int d = 0; d < 3; ++d) sc[d] = a.src[(int64_t)d * a.nsrc_pad + sm];
-        }
-        __syncthreads();
         {
+            // reference point of this thread's group: its middle source (a real source)
+            const int g = threadIdx.x / GS;
+            const int cidx = t0 + min(g * GS + GS / 2, nt - 1);
+            double c[3];
+            for (int d = 0; d < 3; ++d) c[d] = a.src[(int64_t)d * a.nsrc_pad + cidx];
+            if (threadIdx.x % GS == 0)
+                for (int d = 0; d < 3; ++d) sc[g][d] = c[d];
             int s = t0 + threadIdx.x;
             bool ok = threadIdx.x < nt;
-            for (int d = 0; d < 3; ++d) sh[d][threadIdx.x] = ok ? (float)(sc[d] - a.src[(int64_t)d * a.nsrc_pad + s]) : 0.f;
+            for (int d = 0; d < 3; ++d) sh[d][threadIdx.x] = ok ? (float)(c[d] - a.src[(int64_t)d * a.nsrc_pad + s]) : 0.f;
             for (int k = 3; k < NF; ++k) sh[k][threadIdx.x] = ok ? (float)a.src[(int64_t)k * a.nsrc_pad + s] : 0.f;
+            if constexpr (OP == kMaxwellEFIE) {
+                // FP64 group sums of the source weights t2 (fields 6, 7) and t1 (fields 8..13)
+                double v[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) v[k] = ok ? a.src[(int64_t)(6 + k) * a.nsrc_pad + s] : 0.0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+#pragma unroll
+                    for (int off = GS / 2; off >= 1; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+                }
+                if (threadIdx.x % GS == 0)
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) wsum[g][k] = v[k];
+            }
         }
         __syncthreads();
-        // per (target, tile) FP64 reference distance and phase
-        double v0d = x0 - sc[0], v1d = x1 - sc[1], v2d = x2 - sc[2];
+        for (int g = 0; g < NG && g * GS < nt; ++g) {
+        const int j0 = g * GS, j1 = min(j0 + GS, nt);
+        // per (target, group) FP64 reference distance and phase
+        double v0d = x0 - sc[g][0], v1d = x1 - sc[g][1], v2d = x2 - sc[g][2];
         double r0sq_d = v0d * v0d + v1d * v1d + v2d * v2d;
         double r0d = sqrt(r0sq_d);
         double ph_s, ph_c;
@@ -212,9 +234,17 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed(const PotArg
         double damp0 = a.ki != 0.0 ? exp(-a.ki * r0d) : 1.0;
         const float v0 = (float)v0d, v1 = (float)v1d, v2 = (float)v2d;
         const float r0 = (float)r0d, r0sq = (float)r0sq_d;
+        // EFIE: ik, alpha = 1/(ik), 1/r0 powers and h0 = h(r0) = 1/r0^2 - alpha/r0^3
+        const float ik_re = -ki, ik_im = kr;
+        const float kk = kr * kr + ki * ki;
+        const float al_re = kk > 0.f ? -ki / kk : 0.f, al_im = kk > 0.f ? -kr / kk : 0.f;
+        const float r0inv = (float)(1.0 / r0d), r0inv2 = r0inv * r0inv, r0inv3 = r0inv2 * r0inv;
+        const float h0r = r0inv2 - al_re * r0inv3, h0i = -al_im * r0inv3;
+        (void)h0r;
+        (void)h0i;
         float Ar[KD], Ai[KD];
         for (int d = 0; d < KD; ++d) Ar[d] = Ai[d] = 0.f;
-        for (int j = 0; j < nt; ++j) {
+        for (int j = j0; j < j1; ++j) {
             // x - y = v + u, u = c - y
             float u0 = sh[0][j], u1 = sh[1][j], u2 = sh[2][j];
             float w0 = v0 + u0, w1 = v1 + u1, w2 = v2 + u2;
@@ -242,31 +272,67 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed(const PotArg
                 float cr = sh[6][j], ci = sh[7][j];
                 Ar[0] += fr * cr - fi * ci;
                 Ai[0] += fr * ci + fi * cr;
-            } else {  // EFIE
-                float ikr_re = -ki, ikr_im = kr;
-                float nr = ikr_re * r - 1.f, ni = ikr_im * r;
-                float dr = ikr_re * r2, di = ikr_im * r2;
-                float den = dr * dr + di * di;
-                float qr = (nr * dr + ni * di) / den, qi = (ni * dr - nr * di) / den;
-                float t2r = sh[6][j], t2i = sh[7][j];
-                float qtr = qr * t2r - qi * t2i, qti = qr * t2i + qi * t2r;
-                float wv[3] = {w0, w1, w2};
+            } else {  // EFIE: the tile-centre kernel value is subtracted here and added back in FP64
+                (void)er;
+                (void)ei;
+                // E - 1 = e^{ik delta} - 1 from half-angle sin/cos (no cancellation for small delta)
+                float sh2, ch2;
+                sincos_t(0.5f * kr * delta, &sh2, &ch2);
+                float em1r = -2.f * sh2 * sh2, em1i = 2.f * sh2 * ch2;
+                if (ki != 0.f) {
+                    const float dmp = __expf(-ki * delta), dm1 = expm1f(-ki * delta);
+                    em1r = dm1 + dmp * em1r;
+                    em1i = dmp * em1i;
+                }
+                const float Er = 1.f + em1r, Ei = em1i;
+                // t1: ik (E / r - 1 / r0) t1,  E / r - 1 / r0 = (E - 1) / r - delta / (r r0)
+                const float fr = em1r * rinv - delta * rinv * r0inv, fi = em1i * rinv;
+                const float cr1 = ik_re * fr - ik_im * fi, ci1 = ik_re * fi + ik_im * fr;
+                // t2: -((E dh + (E - 1) h0) v + E h u) t2, h = 1/r^2 - alpha/r^3
+                const float rinv2 = rinv * rinv, rinv3 = rinv2 * rinv;
+                const float dh1 = -tt * rinv2 * r0inv2, dh3 = delta * (r2 + r * r0 + r0sq) * rinv3 * r0inv3;
+                const float dhr = dh1 + al_re * dh3, dhi = al_im * dh3;  // h(r) - h(r0)
+                const float hr = rinv2 - al_re * rinv3, hi = -al_im * rinv3;
+                const float pvr = (Er * dhr - Ei * dhi) + (em1r * h0r - em1i * h0i);
+                const float pvi = (Er * dhi + Ei * dhr) + (em1r * h0i + em1i * h0r);
+                const float pur = Er * hr - Ei * hi, pui = Er * hi + Ei * hr;
+                const float t2r = sh[6][j], t2i = sh[7][j];
+                const float vv[3] = {v0, v1, v2}, uu[3] = {u0, u1, u2};
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
-                    float t1r = sh[8 + 2 * d][j], t1i = sh[9 + 2 * d][j];
-                    float tr = (ikr_re * t1r - ikr_im * t1i) - wv[d] * qtr;
-                    float ti = (ikr_re * t1i + ikr_im * t1r) - wv[d] * qti;
-                    Ar[d] += er * tr - ei * ti;
-                    Ai[d] += er * ti + ei * tr;
+                    const float t1r = sh[8 + 2 * d][j], t1i = sh[9 + 2 * d][j];
+                    const float gr = pvr * vv[d] + pur * uu[d], gi = pvi * vv[d] + pui * uu[d];
+                    Ar[d] += float(kInv4Pi) * ((cr1 * t1r - ci1 * t1i) - (gr * t2r - gi * t2i));
+                    Ai[d] += float(kInv4Pi) * ((cr1 * t1i + ci1 * t1r) - (gr * t2i + gi * t2r));
                 }
             }
         }
         // multiply the tile sum by e^{ik r0} (Laplace: kr = ki = 0 gives 1)
         double pr = ph_c * damp0, pi = ph_s * damp0;
+        if constexpr (OP == kMaxwellEFIE) {
+            // tile-centre term in FP64: ik T1 / r0 - h0 v T2 (times 1/(4 pi), e^{ik r0} below)
+            double T[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) T[k] = wsum[g][k];
+            const double ikr_d = -a.ki, iki_d = a.kr, kkd = a.kr * a.kr + a.ki * a.ki;
+            const double alr = kkd > 0.0 ? -a.ki / kkd : 0.0, ali = kkd > 0.0 ? -a.kr / kkd : 0.0;
+            const double i0 = 1.0 / r0d, i2 = i0 * i0, i3 = i2 * i0;
+            const double hr0 = i2 - alr * i3, hi0 = -ali * i3;
+            const double vd[3] = {v0d, v1d, v2d};
+            for (int d = 0; d < KD; ++d) {
+                const double t1r = T[2 + 2 * d], t1i = T[3 + 2 * d];
+                const double gr = hr0 * vd[d], gi = hi0 * vd[d];
+                const double cr = kInv4Pi * ((ikr_d * t1r - iki_d * t1i) * i0 - (gr * T[0] - gi * T[1]));
+                const double ci = kInv4Pi * ((ikr_d * t1i + iki_d * t1r) * i0 - (gr * T[1] + gi * T[0]));
+                accr[d] += pr * cr - pi * ci;
+                acci[d] += pr * ci + pi * cr;
+            }
+        }
         for (int d = 0; d < KD; ++d) {
             accr[d] += pr * Ar[d] - pi * Ai[d];
             acci[d] += pr * Ai[d] + pi * Ar[d];
         }
+        }  // groups
     }
     if (m < a.npts) {
         for (int d = 0; d < KD; ++d) {

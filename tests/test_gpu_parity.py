@@ -120,3 +120,28 @@ def test_small_kr_singular_series_vs_oracle(k, default_parameters, hip_available
         A = fn(p1, p1, p1, k, parameters=default_parameters, precision="double").weak_form().to_dense()
         ref = O.assemble(name, g.vertices, g.elements, p1, p1, k)
         assert rel_fro(A, ref) < 1e-12
+
+
+@pytest.mark.parametrize("k", [2.0, 0.5])
+@pytest.mark.parametrize("density", ["random", "smooth"])
+def test_efie_potential_far_mixed(k, density, default_parameters, hip_available):
+    """E-field potential at radius 10: the divergence term of RWG densities cancels in pairs
+    (each basis function has zero total divergence), which the mixed kernel must not amplify."""
+    import bempp_amd.api as bempp
+    from bempp_amd.api.operators.potential import maxwell as mpot
+
+    g = bempp.shapes.icosphere(3)
+    rwg = bempp.function_space(g, "RWG", 0)
+    n = rwg.global_dof_count
+    if density == "random":
+        rng = np.random.default_rng(2)
+        c = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    else:
+        mid = 0.5 * (g.vertices[:, g.edges[0]] + g.vertices[:, g.edges[1]])
+        c = np.cos(2 * mid[0]) + 1j * np.sin(mid[1])
+    pts = np.random.default_rng(3).standard_normal((3, 2048))
+    pts = 10.0 * pts / np.linalg.norm(pts, axis=0)
+    fun = bempp.GridFunction(rwg, coefficients=c)
+    u64 = mpot.electric_field(rwg, pts, k, parameters=default_parameters, precision="double").evaluate(fun)
+    u32 = mpot.electric_field(rwg, pts, k, parameters=default_parameters, precision="single").evaluate(fun)
+    assert np.linalg.norm(u32 - u64) / np.linalg.norm(u64) < 1e-6
