@@ -29,11 +29,16 @@ class _Hip(object):
     Off by default: on the level-5 sphere (Helmholtz P1, k = 4) it changes the relative
     error against FP64 from 1.05e-7 to 1.03e-7 and costs 23% of the single-layer kernel
     time; the error is dominated by the FP32 kernel evaluation and complex64 storage.
+    symmetric: when a single layer and its double layer are assembled together on one P1
+    space (assemble_weak_forms), evaluate every unordered element pair once and obtain the
+    reversed pair from the same distances and phase factors (the single layer's block is the
+    transpose, the double layer's only needs the other normal). False visits each ordered pair.
     """
 
     def __init__(self):
         self.mixed_near_field_factor = 2.0
         self.kahan = False
+        self.symmetric = True
         self.tile_config = None
 
 

@@ -15,6 +15,9 @@ TEST_TILE_ELEMENTS = 16
 TEST_TILE_DOFS = 20
 PAIR_TEST_TILE_ELEMENTS = 8  # single + double layer in one pass (BEM_TILE_PAIR_TEST_*)
 PAIR_TEST_TILE_DOFS = 12
+SYM_TEST_TILE_ELEMENTS = 8  # symmetric single + double layer pass (BEM_TILE_SYM_TEST_*)
+SYM_TEST_TILE_DOFS = 12
+SYM_RUN_ELEMENTS = 48  # test elements per work item (BEM_TILE_SYM_RUN_ELEMENTS)
 TRIAL_BLOCK_ELEMENTS = 256
 TRIAL_BLOCK_DOFS = 160
 MAX_INCIDENCE = 12
@@ -64,8 +67,10 @@ class Blocks(object):
                         raise ValueError("a dof has more element incidences in one block than the device supports")
                     inc[b, s, k] = 3 * (pos - self.start[b]) + i
                     fill[b, s] = k + 1
-        self.incidence = inc
-        return inc
+        # keep only the incidence columns some dof uses (the kernel loops over them)
+        used = max(1, int(fill.max()) if fill.size else 1)
+        self.incidence = _np.ascontiguousarray(inc[:, :, :used])
+        return self.incidence
 
     @property
     def n_blocks(self):
@@ -165,7 +170,7 @@ class DeviceBlocks(object):
         )
         self.struct = hb.BemDofBlocks(
             blocks.n_blocks, blocks.max_dofs, self.start.data_ptr(), self.dofs.data_ptr(), self.ndofs.data_ptr(),
-            self.slots.data_ptr(), self.incidence.data_ptr(), MAX_INCIDENCE,
+            self.slots.data_ptr(), self.incidence.data_ptr(), int(blocks.incidence.shape[2]),
         )
 
 
@@ -183,6 +188,88 @@ def device_blocks(space, role):
     key = (space.id, me, md)
     if key not in _DEVICE_CACHE:
         _DEVICE_CACHE[key] = DeviceBlocks(space, make_blocks(space, me, md))
+    return _DEVICE_CACHE[key]
+
+
+def _tiles_per_item(n_pairs):
+    """Test tiles per work item: runs as long as possible (the trial state is set up once per
+    item) while leaving >= ~16k items, i.e. ~30 workgroups per resident slot on 256 CUs."""
+    return int(max(1, min(32, n_pairs // 16384)))
+
+
+def make_symmetric_blocks(space, tiles_per_item=None):
+    """Blocking of the symmetric pass: (trial blocks, nested test tiles, tile -> block, work list).
+
+    The trial blocks are those of make_blocks. Every trial block is cut into test tiles of at
+    most SYM_TEST_TILE_ELEMENTS elements and SYM_TEST_TILE_DOFS dofs over the same element
+    order, so each tile lies inside one block. Block b meets the tiles of blocks 0..b, a
+    contiguous range; the work list (int32 [n, 3]: block, first tile, end tile) cuts that range
+    into runs. With the kernel's rule for the tile's own block (only trial elements later in
+    the order), each unordered element pair is visited once.
+    """
+    key = ("sym", space.id, tiles_per_item)
+    if key in _CACHE:
+        return _CACHE[key]
+    trial = make_blocks(space, TRIAL_BLOCK_ELEMENTS, TRIAL_BLOCK_DOFS)
+    order = trial.order
+    l2g = space.local2global
+    mult = space.local_multipliers
+    n = len(order)
+    starts, dofs_rows, ndofs, owner = [], [], [], []
+    slots = _np.zeros((n, 3), dtype=_np.int32)
+    for b in range(trial.n_blocks):
+        lo, hi = int(trial.start[b]), int(trial.start[b + 1])
+        sub = _greedy_blocks(order[lo:hi], l2g, mult, SYM_TEST_TILE_ELEMENTS, SYM_TEST_TILE_DOFS)
+        starts.extend(int(lo + st) for st in sub.start[:-1])
+        dofs_rows.append(sub.dofs)
+        ndofs.append(sub.ndofs)
+        slots[lo:hi] = sub.slots
+        owner.extend([b] * sub.n_blocks)
+    starts.append(n)
+    tiles = Blocks(order.copy(), _np.asarray(starts, dtype=_np.int32),
+                   _np.concatenate(dofs_rows) if dofs_rows else _np.zeros((0, SYM_TEST_TILE_DOFS), _np.int32),
+                   _np.concatenate(ndofs) if ndofs else _np.zeros(0, _np.int32), slots, SYM_TEST_TILE_DOFS,
+                   trial.nshape)
+    owner = _np.asarray(owner, dtype=_np.int64)
+    # tiles of blocks 0..b: [0, tiles_end[b])
+    tiles_end = _np.searchsorted(owner, _np.arange(trial.n_blocks), side="right")
+    g = min(tiles_per_item or _tiles_per_item(int(tiles_end.sum())), SYM_RUN_ELEMENTS // SYM_TEST_TILE_ELEMENTS)
+    items = [(b, t, min(t + g, int(te))) for b, te in enumerate(tiles_end) for t in range(0, int(te), g)]
+    items.sort(key=lambda it: (it[1], it[0]))  # concurrent workgroups share test tiles (output rows)
+    work = _np.asarray(items, dtype=_np.int32).reshape(-1, 3)
+    result = (trial, tiles, owner, work)
+    _CACHE[key] = result
+    return result
+
+
+class DeviceSymmetricBlocks(object):
+    """Device arrays of make_symmetric_blocks: trial blocks (DeviceBlocks), test tiles, work list."""
+
+    def __init__(self, space):
+        import torch
+        from bempp_amd.core import hip_backend as hb
+
+        dev = hb.device()
+        _, tiles, owner, work = make_symmetric_blocks(space)
+        self.trial = device_blocks(space, "trial")
+        self.tiles = tiles
+        self.tile_block = owner
+        self.start = torch.from_numpy(tiles.start).to(dev)
+        self.dofs = torch.from_numpy(_np.ascontiguousarray(tiles.dofs)).to(dev)
+        self.ndofs = torch.from_numpy(tiles.ndofs).to(dev)
+        self.slots = torch.from_numpy(tiles.slots).to(dev)
+        self.work = torch.from_numpy(work).to(dev)
+        self.n_work = int(work.shape[0])
+        self.tiles_struct = hb.BemDofBlocks(
+            tiles.n_blocks, tiles.max_dofs, self.start.data_ptr(), self.dofs.data_ptr(), self.ndofs.data_ptr(),
+            self.slots.data_ptr(), None, 0,
+        )
+
+
+def device_symmetric_blocks(space):
+    key = ("sym", space.id)
+    if key not in _DEVICE_CACHE:
+        _DEVICE_CACHE[key] = DeviceSymmetricBlocks(space)
     return _DEVICE_CACHE[key]
 
 

@@ -181,6 +181,16 @@ def can_fuse_sl_dl(first, second):
     )
 
 
+def use_symmetric_pass(domain, dual_to_range):
+    """True if the fused pair can run the symmetric pass: one P1 space on both sides."""
+    return (
+        domain == dual_to_range
+        and not _is_localised_injective(domain)
+        and domain.codomain_dimension == 1
+        and domain.number_of_shape_functions == 3
+    )
+
+
 def hip_dense_assemble_sl_dl(domain, dual_to_range, parameters, sl_descriptor, dl_descriptor, out_single=None,
                              out_double=None, events=None):
     """Assemble a single layer and the double layer of the same family in one pass.
@@ -216,23 +226,31 @@ def hip_dense_assemble_sl_dl(domain, dual_to_range, parameters, sl_descriptor, d
         and dual_to_range.number_of_support_elements * dual_to_range.number_of_shape_functions == rows
         and domain.number_of_support_elements * domain.number_of_shape_functions == cols
     )
+    hip_opts = getattr(parameters, "hip", None)
+    near = getattr(hip_opts, "mixed_near_field_factor", 2.0) if hip_opts is not None else 2.0
+    kahan = getattr(hip_opts, "kahan", False) if hip_opts is not None else False
+    symmetric = (
+        getattr(hip_opts, "symmetric", True) if hip_opts is not None else True
+    ) and use_symmetric_pass(domain, dual_to_range)
+    k = _wavenumber(sl_descriptor)
     outs = []
     for o in (out_single, out_double):
         if o is None:
             o = torch.empty((rows, cols), dtype=tdtype, device=dev)
         elif tuple(o.shape) != (rows, cols) or o.dtype != tdtype or not o.is_contiguous():
             raise ValueError(f"outputs must be contiguous ({rows}, {cols}) {dtype} tensors")
-        if not full_cover:
-            hb.fill_zero(o)
+        if not full_cover and not symmetric:
+            hb.fill_zero(o)  # the symmetric pass overwrites its outputs itself
         outs.append(o)
-    hip_opts = getattr(parameters, "hip", None)
-    near = getattr(hip_opts, "mixed_near_field_factor", 2.0) if hip_opts is not None else 2.0
-    kahan = getattr(hip_opts, "kahan", False) if hip_opts is not None else False
-    k = _wavenumber(sl_descriptor)
 
     if events is not None:
         events["regular_start"].record()
-    if direct:
+    if symmetric:
+        from bempp_amd.core.blocking import device_symmetric_blocks
+
+        hb.regular_assemble_sl_dl_symmetric(op, device_symmetric_blocks(domain), rule, k, mode, outs[0], outs[1],
+                                            near_factor=near, kahan=kahan)
+    elif direct:
         hb.regular_assemble_sl_dl(op, test, trial, None, None, grids_identical, rule, k, mode, outs[0], outs[1],
                                   accumulate=0, near_factor=near, kahan=kahan)
     else:

@@ -129,6 +129,12 @@ def lib():
         L.bem_dense_matvec.restype = i
         L.bem_regular_assemble_sl_dl.argtypes = [i, SP, SP, BP, BP, i, QP, d, d, i, d, i, P, P, i64, i, i, P]
         L.bem_regular_assemble_sl_dl.restype = i
+        L.bem_regular_assemble_sl_dl_symmetric.argtypes = [i, SP, BP, BP, i, P, i64, QP, d, d, i, d, i, P, P, i64, i, P]
+        L.bem_regular_assemble_sl_dl_symmetric.restype = i
+        L.bem_dense_add_transpose.argtypes = [P, i, i64, i64, P]
+        L.bem_dense_add_transpose.restype = i
+        L.bem_dense_zero.argtypes = [P, i, i64, i64, P]
+        L.bem_dense_zero.restype = i
         L.bem_near_assemble_sl_dl.argtypes = [i, SP, SP, i, P, i, QP, d, d, d, P, P, i64, i, P]
         L.bem_near_assemble_sl_dl.restype = i
         L.bem_singular_assemble_sl_dl.argtypes = [i, SP, SP, i, P, P, P, P, d, d, P, P, i64, i, P]
@@ -423,6 +429,42 @@ def regular_assemble_sl_dl(op, test, trial, test_blocks, trial_blocks, grids_ide
             ptr(out_double), out_single.shape[-1], _dtype_code(out_single), int(accumulate), stream_handle(),
         )
     )
+
+
+def regular_assemble_sl_dl_symmetric(op, sym, rule, wavenumber, precision, out_single, out_double, near_factor=2.0,
+                                     kahan=False):
+    """Single layer (op) and its double layer on one P1 space, every unordered pair once.
+
+    sym: blocking.DeviceSymmetricBlocks of the space. Both outputs are overwritten with the
+    regular part (square, contiguous, same dtype).
+    """
+    k = complex(wavenumber)
+    n = out_single.shape[0]
+    for o in (out_single, out_double):
+        if o.dim() != 2 or tuple(o.shape) != (n, n) or not o.is_contiguous() or o.dtype != out_single.dtype:
+            raise ValueError("outputs must be contiguous square matrices of one dtype")
+    check(
+        lib().bem_regular_assemble_sl_dl_symmetric(
+            OPERATORS[op], ctypes.byref(sym.trial.space_struct), ctypes.byref(sym.tiles_struct),
+            ctypes.byref(sym.trial.struct), int(sym.n_work), ptr(sym.work), n, ctypes.byref(rule), k.real, k.imag,
+            precision, float(near_factor), int(bool(kahan)), ptr(out_single), ptr(out_double), n,
+            _dtype_code(out_single), stream_handle(),
+        )
+    )
+
+
+def dense_add_transpose(a):
+    """a <- a + a^T in place (square contiguous device matrix)."""
+    if a.dim() != 2 or a.shape[0] != a.shape[1] or not a.is_contiguous():
+        raise ValueError("expected a contiguous square matrix")
+    check(lib().bem_dense_add_transpose(ptr(a), _dtype_code(a), a.shape[0], a.shape[1], stream_handle()))
+
+
+def dense_zero(a):
+    """a <- 0 (square contiguous device matrix) with the library's fill kernel."""
+    if a.dim() != 2 or a.shape[0] != a.shape[1] or not a.is_contiguous():
+        raise ValueError("expected a contiguous square matrix")
+    check(lib().bem_dense_zero(ptr(a), _dtype_code(a), a.shape[0], a.shape[1], stream_handle()))
 
 
 def near_assemble_sl_dl(op, test, trial, pairs, grids_identical, rule, wavenumber, near_factor, out_single,

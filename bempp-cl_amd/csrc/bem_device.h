@@ -143,6 +143,42 @@ __device__ __forceinline__ void green_sl_dl(T dx, T dy, T dz, T r2, const T *ny,
     }
 }
 
+// Single layer, double layer (normal n_y) and the double layer of the reversed pair (x on the
+// trial side, y on the test side: kernel G'(r) (x - y).n_x / r), all from one distance and one
+// e^{ikr}; (dx, dy, dz) = y - x. The third value is the (f, e) double-layer entry that the
+// symmetric pass computes together with the (e, f) entries.
+template <bool HELM, typename T, bool SMALL = false>
+__device__ __forceinline__ void green_sl_dl_rev(T dx, T dy, T dz, T r2, const T *nx, const T *ny, T kr, T ki, T &g1re,
+                                                T &g1im, T &g2re, T &g2im, T &g3re, T &g3im) {
+    const T rinv = rsqrt_t(r2);
+    const T dot2 = dx * ny[0] + dy * ny[1] + dz * ny[2];
+    const T dot3 = -(dx * nx[0] + dy * nx[1] + dz * nx[2]);
+    if constexpr (!HELM) {
+        g1re = T(kInv4Pi) * rinv;
+        g1im = T(0);
+        const T h = -g1re * rinv * rinv;
+        g2re = h * dot2;
+        g3re = h * dot3;
+        g2im = g3im = T(0);
+    } else {
+        const T r = r2 * rinv;
+        T s, c;
+        sincos_t<SMALL>(kr * r, &s, &c);
+        T g = T(kInv4Pi) * rinv;
+        if (ki != T(0)) g *= exp_t(-ki * r);
+        g1re = c * g;
+        g1im = s * g;
+        const T gd = g * rinv * rinv;
+        const T fr = c * gd, fi = s * gd;
+        const T a = T(-1) - ki * r, b = kr * r;
+        const T hr = a * fr - b * fi, hi = b * fr + a * fi;
+        g2re = hr * dot2;
+        g2im = hi * dot2;
+        g3re = hr * dot3;
+        g3im = hi * dot3;
+    }
+}
+
 // Green's function of OP and, for DUAL (OP a single layer), also the double layer of the same
 // family at the same point pair (shared r and e^{ikr}).
 template <int OP, bool DUAL, typename T, bool SMALL = false>

@@ -130,10 +130,92 @@ int launch(int trans, const void *A, int64_t rows, int64_t cols, int64_t ld, con
     return check_launch("matvec_t_kernel");
 }
 
+// ---------------------------------------------------------------- square-matrix helpers
+// A <- A + A^T (n x n, leading dimension ld), in place: the workgroup of tile pair (bi, bj),
+// bi <= bj, reads both 32 x 32 tiles into LDS before writing either.
+template <typename E>
+__device__ __forceinline__ E eadd(E a, E b) {
+    return a + b;
+}
+template <>
+__device__ __forceinline__ float2 eadd(float2 a, float2 b) {
+    return make_float2(a.x + b.x, a.y + b.y);
+}
+template <>
+__device__ __forceinline__ double2 eadd(double2 a, double2 b) {
+    return make_double2(a.x + b.x, a.y + b.y);
+}
+
+template <typename E>
+__global__ __launch_bounds__(256) void add_transpose_kernel(E *A, int64_t n, int64_t ld) {
+    const int64_t bi = blockIdx.y, bj = blockIdx.x;
+    if (bi > bj) return;  // uniform per workgroup, before any barrier
+    __shared__ E X[32][33], Y[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int64_t r0 = bi * 32, c0 = bj * 32;
+    const bool off = bi != bj;
+    for (int r = ty; r < 32; r += 8) {
+        if (r0 + r < n && c0 + tx < n) X[r][tx] = A[(r0 + r) * ld + c0 + tx];
+        if (off && c0 + r < n && r0 + tx < n) Y[r][tx] = A[(c0 + r) * ld + r0 + tx];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        // A[r0 + r][c0 + tx] += A[c0 + tx][r0 + r]
+        if (r0 + r < n && c0 + tx < n) A[(r0 + r) * ld + c0 + tx] = eadd(X[r][tx], off ? Y[tx][r] : X[tx][r]);
+        if (off && c0 + r < n && r0 + tx < n) A[(c0 + r) * ld + r0 + tx] = eadd(Y[r][tx], X[tx][r]);
+    }
+}
+
+template <typename E>
+__global__ __launch_bounds__(256) void zero_square_kernel(E *A, int64_t n, int64_t ld) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    for (int64_t r = blockIdx.y; r < n; r += gridDim.y) A[r * ld + c] = E{};
+}
+
+template <typename E>
+static int square_launch(int what, void *A, int64_t n, int64_t ld, hipStream_t s) {
+    if (what == 0) {
+        const unsigned rows = (unsigned)(n < 4096 ? n : 4096);
+        hipLaunchKernelGGL((zero_square_kernel<E>), dim3((unsigned)((n + 255) / 256), rows), dim3(256), 0, s, (E *)A, n,
+                           ld);
+        return check_launch("zero_square_kernel");
+    }
+    const unsigned nb = (unsigned)((n + 31) / 32);
+    hipLaunchKernelGGL((add_transpose_kernel<E>), dim3(nb, nb), dim3(256), 0, s, (E *)A, n, ld);
+    return check_launch("add_transpose_kernel");
+}
+
 }  // namespace mv
+
+int square_matrix_op(int what, void *A, int dtype, int64_t n, int64_t ld, hipStream_t s) {
+    if (!A || n < 0 || ld < n || n > 65535 * 32) {
+        set_error("invalid square matrix (n = %lld, ld = %lld)", (long long)n, (long long)ld);
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (n == 0) return BEM_OK;
+    switch (dtype) {
+    case BEM_F32: return mv::square_launch<float>(what, A, n, ld, s);
+    case BEM_F64: return mv::square_launch<double>(what, A, n, ld, s);
+    case BEM_C64: return mv::square_launch<float2>(what, A, n, ld, s);
+    case BEM_C128: return mv::square_launch<double2>(what, A, n, ld, s);
+    default: set_error("unknown matrix dtype %d", dtype); return BEM_ERR_INVALID_ARGUMENT;
+    }
+}
+
 }  // namespace bem
 
 using namespace bem;
+
+extern "C" int bem_dense_add_transpose(void *A, int dtype, int64_t n, int64_t ld, void *stream) {
+    clear_error();
+    return square_matrix_op(1, A, dtype, n, ld, (hipStream_t)stream);
+}
+
+extern "C" int bem_dense_zero(void *A, int dtype, int64_t n, int64_t ld, void *stream) {
+    clear_error();
+    return square_matrix_op(0, A, dtype, n, ld, (hipStream_t)stream);
+}
 
 extern "C" int bem_dense_matvec(int trans, const void *A, int a_dtype, int64_t rows, int64_t cols, int64_t ld,
                                 const void *x, void *y, void *stream) {

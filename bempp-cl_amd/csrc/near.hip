@@ -41,8 +41,8 @@ __global__ __launch_bounds__(256) void near_kernel(const NearArgs a) {
     if (!pair_is_near(ct, cf, gt[13], gf[13], a.near_factor, &dist2)) return;
     double tnm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
     double snm = a.trial.nmult ? (double)a.trial.nmult[f] : 1.0;
-    C2d S[NTS][NSS], S2[NTS][NSS];
-    pair_fp64_geom<OP, NTS, NSS, QT, DUAL>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2);
+    C2d S[NTS][NSS], S2[NTS][NSS], S3[NTS][NSS];
+    pair_fp64_geom<OP, NTS, NSS, QT, DUAL>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2, S3);
     auto scatter = [&](const C2d(&Sm)[NTS][NSS], auto op_tag, void *out) {
         constexpr int OPX = decltype(op_tag)::value;
         OutT *o = reinterpret_cast<OutT *>(out);

@@ -7,7 +7,8 @@ namespace bem {
 static int regular_entry(int op, const bem_space_t *test, const bem_space_t *trial, const bem_dof_blocks_t *tblocks,
                          const bem_dof_blocks_t *sblocks, int grids_identical, const bem_quad_rule_t *rule,
                          double k_re, double k_im, int precision, double mixed_near_factor, int kahan, void *out,
-                         void *out2, int64_t ld, int out_dtype, int accumulate, void *stream) {
+                         void *out2, int64_t ld, int out_dtype, int accumulate, void *stream,
+                         const int32_t *work = nullptr, int nwork = 0) {
     clear_error();
     if (!test || !trial || !rule || !out) {
         set_error("null argument");
@@ -53,6 +54,8 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
     a.ld = ld;
     a.identical = grids_identical;
     a.accumulate = accumulate;
+    a.work = work;
+    a.nwork = nwork;
     a.tb = BlockArgs{nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, 0};
     a.sb = a.tb;
     if (tile) {
@@ -62,6 +65,12 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
                          sblocks->n_blocks, sblocks->max_dofs, sblocks->max_incidence};
     }
     hipStream_t s = (hipStream_t)stream;
+    if (work) {
+        if (op == BEM_LAPLACE_SINGLE_LAYER) return regular_launch_sym_0(a, precision, kahan, out_dtype, s);
+        if (op == BEM_HELMHOLTZ_SINGLE_LAYER) return regular_launch_sym_3(a, precision, kahan, out_dtype, s);
+        set_error("the symmetric single/double layer pass needs a single-layer operator, got %d", op);
+        return BEM_ERR_UNSUPPORTED;
+    }
     if (out2) {
         if (op == BEM_LAPLACE_SINGLE_LAYER) return regular_launch_dual_0(a, precision, kahan, tile, out_dtype, s);
         if (op == BEM_HELMHOLTZ_SINGLE_LAYER) return regular_launch_dual_3(a, precision, kahan, tile, out_dtype, s);
@@ -119,4 +128,38 @@ extern "C" int bem_regular_assemble_sl_dl(int op, const bem_space_t *test, const
     return regular_entry(op, test, trial, test_blocks, trial_blocks, grids_identical, rule, k_re, k_im, precision,
                          mixed_near_factor, kahan, out_single, out_double, ld, out_dtype,
                          test_blocks ? 0 : accumulate, stream);
+}
+
+extern "C" int bem_regular_assemble_sl_dl_symmetric(int op, const bem_space_t *space,
+                                                    const bem_dof_blocks_t *test_tiles,
+                                                    const bem_dof_blocks_t *trial_blocks, int n_work,
+                                                    const int32_t *work, int64_t n_dofs, const bem_quad_rule_t *rule,
+                                                    double k_re, double k_im, int precision, double mixed_near_factor,
+                                                    int kahan, void *out_single, void *out_double, int64_t ld,
+                                                    int out_dtype, void *stream) {
+    clear_error();
+    if (!space || !test_tiles || !trial_blocks || !out_single || !out_double || (n_work > 0 && !work)) {
+        set_error("null argument");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (n_work < 0 || n_dofs < 0 || ld < n_dofs) {
+        set_error("invalid work list or matrix shape");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (test_tiles->max_dofs > BEM_TILE_SYM_TEST_DOFS) {
+        set_error("test tiles of the symmetric pass hold at most %d dofs", BEM_TILE_SYM_TEST_DOFS);
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    int rc;
+    // both outputs are overwritten: zero, then the pairs (e, f) with e before f in the element
+    // order, then the single layer's transpose
+    if ((rc = square_matrix_op(0, out_single, out_dtype, n_dofs, ld, s))) return rc;
+    if ((rc = square_matrix_op(0, out_double, out_dtype, n_dofs, ld, s))) return rc;
+    if (n_work > 0) {
+        rc = regular_entry(op, space, space, test_tiles, trial_blocks, 1, rule, k_re, k_im, precision,
+                           mixed_near_factor, kahan, out_single, out_double, ld, out_dtype, 0, stream, work, n_work);
+        if (rc) return rc;
+    }
+    return square_matrix_op(1, out_single, out_dtype, n_dofs, ld, s);
 }

@@ -70,11 +70,14 @@ struct RegularArgs {
     int64_t ld;
     int identical;
     int accumulate;
+    // SYM kernels: work item w is trial block work[3w] with the test tiles [work[3w + 1], work[3w + 2])
+    const int32_t *work;
+    int nwork;
 };
 
 
-// NXP: 1 in MIXED mode, where the absolute FP64 test points are not needed
-template <int TM, int NXP = TM>
+// NXP: 1 in MIXED mode, where the absolute FP64 test points are not needed; Q: points per element
+template <int TM, int NXP = TM, int Q = kMaxQ>
 struct TestTile {
     int elem[TM];
     int vid[TM][3];
@@ -82,12 +85,15 @@ struct TestTile {
     double mult[TM][3];
     double c[TM][3];
     double diam[TM];
-    double xp[NXP][kMaxQ][3];  // absolute test points (FP64 paths)
-    float a[TM][kMaxQ][3];    // offsets from the centroid (FP32 path)
-    float aa[TM][kMaxQ];      // |a|^2
-    float wp[TM][kMaxQ];      // w_p J_e
-    float nx[TM][3];          // test normal (multiplied)
+    double xp[NXP][Q][3];  // absolute test points (FP64 paths)
+    float a[TM][Q][3];     // offsets from the centroid (FP32 path)
+    float aa[TM][Q];       // |a|^2
+    float wp[TM][Q];       // w_p J_e
+    float nx[TM][3];       // test normal (multiplied)
 };
+
+// test elements staged per SYM work item (a run of test tiles)
+constexpr int kSymRunElements = BEM_TILE_SYM_RUN_ELEMENTS;
 
 // ------------------------------------------------------------------ FP64 pair
 // T[b] += g u_b(eta_q) with u = (1, eta0, eta1)
@@ -125,26 +131,43 @@ __device__ __forceinline__ void zero_moments(C2d (&S)[NTS][NSS]) {
         for (int b = 0; b < NSS; ++b) S[a][b].re = S[a][b].im = 0.0;
 }
 
-// FP64 pair from the geometry table (any rule). DUAL: S2 receives the double layer.
-template <int OP, int NTS, int NSS, int QT, bool DUAL = false>
+// Green's function values of one point pair: OP (g), for DUAL the double layer (g2) and for
+// SYM also the double layer of the reversed pair (g3, test normal nx).
+template <int OP, bool DUAL, bool SYM>
+__device__ __forceinline__ void green3(double dx, double dy, double dz, double r2, const double *nx, const double *ny,
+                                       double kr, double ki, double &gre, double &gim, double &g2re, double &g2im,
+                                       double &g3re, double &g3im) {
+    if constexpr (SYM) {
+        green_sl_dl_rev<OpTraits<OP>::is_complex, double>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im, g3re,
+                                                          g3im);
+    } else {
+        green2<OP, DUAL, double>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im);
+        g3re = g3im = 0.0;
+    }
+}
+
+// FP64 pair from the geometry table (any rule). DUAL: S2 receives the double layer; SYM: S3
+// receives the double layer of the reversed pair (indexed like S: test side first).
+template <int OP, int NTS, int NSS, int QT, bool DUAL = false, bool SYM = false>
 __device__ __forceinline__ void pair_fp64_geom(const double *__restrict__ gt, const double *__restrict__ gf, double tnm,
                                                double snm, const RuleArgs &rule, double kr, double ki,
-                                               C2d (&S)[NTS][NSS], C2d (&S2)[NTS][NSS]) {
+                                               C2d (&S)[NTS][NSS], C2d (&S2)[NTS][NSS], C2d (&S3)[NTS][NSS]) {
     const int nq = QT > 0 ? QT : rule.n;
     double nx[3] = {gt[9] * tnm, gt[10] * tnm, gt[11] * tnm};
     double ny[3] = {gf[9] * snm, gf[10] * snm, gf[11] * snm};
     double jt = gt[12], jf = gf[12];
     zero_moments(S);
     if constexpr (DUAL) zero_moments(S2);
+    if constexpr (SYM) zero_moments(S3);
 #pragma unroll 1
     for (int p = 0; p < nq; ++p) {
         double xi = rule.x[p], eta = rule.y[p];
         double x0 = gt[0] + gt[3] * xi + gt[6] * eta;
         double x1 = gt[1] + gt[4] * xi + gt[7] * eta;
         double x2 = gt[2] + gt[5] * xi + gt[8] * eta;
-        C2d T[NSS], T2[NSS];
+        C2d T[NSS], T2[NSS], T3[NSS];
 #pragma unroll
-        for (int b = 0; b < NSS; ++b) T[b].re = T[b].im = T2[b].re = T2[b].im = 0.0;
+        for (int b = 0; b < NSS; ++b) T[b].re = T[b].im = T2[b].re = T2[b].im = T3[b].re = T3[b].im = 0.0;
 #pragma unroll(QT > 0 ? QT : 1)
         for (int q = 0; q < nq; ++q) {
             double s0 = rule.x[q], s1 = rule.y[q];
@@ -152,44 +175,50 @@ __device__ __forceinline__ void pair_fp64_geom(const double *__restrict__ gt, co
             double dy = gf[1] + gf[4] * s0 + gf[7] * s1 - x1;
             double dz = gf[2] + gf[5] * s0 + gf[8] * s1 - x2;
             double r2 = dx * dx + dy * dy + dz * dz;
-            double gre, gim, g2re, g2im;
-            green2<OP, DUAL, double>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im);
+            double gre, gim, g2re, g2im, g3re, g3im;
+            green3<OP, DUAL, SYM>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im, g3re, g3im);
             const double wq = rule.w[q] * jf;
             add_trial_moments<NSS>(T, gre * wq, gim * wq, s0, s1);
             if constexpr (DUAL) add_trial_moments<NSS>(T2, g2re * wq, g2im * wq, s0, s1);
+            if constexpr (SYM) add_trial_moments<NSS>(T3, g3re * wq, g3im * wq, s0, s1);
         }
         add_test_moments<NTS, NSS>(S, T, rule.w[p] * jt, xi, eta);
         if constexpr (DUAL) add_test_moments<NTS, NSS>(S2, T2, rule.w[p] * jt, xi, eta);
+        if constexpr (SYM) add_test_moments<NTS, NSS>(S3, T3, rule.w[p] * jt, xi, eta);
     }
 }
 
-// FP64 pair with test points from LDS and trial points from registers. DUAL: S2 = double layer.
-template <int OP, int NTS, int NSS, int QT, int TM, int NXP, bool DUAL = false>
-__device__ __forceinline__ void pair_fp64_regs(const TestTile<TM, NXP> &T_, int t, const double *gt, double tnm,
+// FP64 pair with test points from LDS and trial points from registers. DUAL: S2 = double
+// layer; SYM: S3 = double layer of the reversed pair.
+template <int OP, int NTS, int NSS, int QT, int TM, int NXP, bool DUAL = false, bool SYM = false>
+__device__ __forceinline__ void pair_fp64_regs(const TestTile<TM, NXP, QT> &T_, int t, const double *gt, double tnm,
                                                const double (&yq)[QT > 0 ? QT : 1][3], const double (&wq)[QT > 0 ? QT : 1],
                                                const double *ny, const RuleArgs &rule, double kr, double ki,
-                                               C2d (&S)[NTS][NSS], C2d (&S2)[NTS][NSS]) {
+                                               C2d (&S)[NTS][NSS], C2d (&S2)[NTS][NSS], C2d (&S3)[NTS][NSS]) {
     double nx[3] = {gt[9] * tnm, gt[10] * tnm, gt[11] * tnm};
     double jt = gt[12];
     zero_moments(S);
     if constexpr (DUAL) zero_moments(S2);
+    if constexpr (SYM) zero_moments(S3);
 #pragma unroll 1
     for (int p = 0; p < QT; ++p) {
         double x0 = T_.xp[t][p][0], x1 = T_.xp[t][p][1], x2 = T_.xp[t][p][2];
-        C2d T[NSS], T2[NSS];
+        C2d T[NSS], T2[NSS], T3[NSS];
 #pragma unroll
-        for (int b = 0; b < NSS; ++b) T[b].re = T[b].im = T2[b].re = T2[b].im = 0.0;
+        for (int b = 0; b < NSS; ++b) T[b].re = T[b].im = T2[b].re = T2[b].im = T3[b].re = T3[b].im = 0.0;
 #pragma unroll
         for (int q = 0; q < QT; ++q) {
             double dx = yq[q][0] - x0, dy = yq[q][1] - x1, dz = yq[q][2] - x2;
             double r2 = dx * dx + dy * dy + dz * dz;
-            double gre, gim, g2re, g2im;
-            green2<OP, DUAL, double>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im);
+            double gre, gim, g2re, g2im, g3re, g3im;
+            green3<OP, DUAL, SYM>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im, g3re, g3im);
             add_trial_moments<NSS>(T, gre * wq[q], gim * wq[q], rule.x[q], rule.y[q]);
             if constexpr (DUAL) add_trial_moments<NSS>(T2, g2re * wq[q], g2im * wq[q], rule.x[q], rule.y[q]);
+            if constexpr (SYM) add_trial_moments<NSS>(T3, g3re * wq[q], g3im * wq[q], rule.x[q], rule.y[q]);
         }
         add_test_moments<NTS, NSS>(S, T, rule.w[p] * jt, rule.x[p], rule.y[p]);
         if constexpr (DUAL) add_test_moments<NTS, NSS>(S2, T2, rule.w[p] * jt, rule.x[p], rule.y[p]);
+        if constexpr (SYM) add_test_moments<NTS, NSS>(S3, T3, rule.w[p] * jt, rule.x[p], rule.y[p]);
     }
 }
 
@@ -211,13 +240,17 @@ __device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf
 // DAMP: complex wavenumber (e^{-Im(k) r} factor); a template parameter so that real
 // wavenumbers do not pay for an if-converted exponential per evaluation.
 // DUAL (OP a single layer): S2 receives the double layer of the same family, sharing the
-// distance, 1/r and e^{ik delta} of every point pair.
-template <int OP, int NTS, int NSS, int QT, int TM, bool KAHAN, bool DAMP, bool DUAL, int NXP>
-__device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, const TrialF32<QT> &F, double d0, double d1,
+// distance, 1/r and e^{ik delta} of every point pair. SYM (with DUAL): S3 receives the double
+// layer of the reversed pair (normal of the test element), from the same values again.
+template <int OP, int NTS, int NSS, int QT, int TM, bool KAHAN, bool DAMP, bool DUAL, int NXP, bool SYM = false>
+__device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP, (QT > 0 ? QT : kMaxQ)> &T_, int t, const TrialF32<QT> &F,
+                                          double d0, double d1,
                                           double d2, double dist2, const RuleArgs &rule, double kr, double ki,
-                                          C2f (&S)[NTS][NSS], C2f (&S2)[NTS][NSS]) {
+                                          C2f (&S)[NTS][NSS], C2f (&S2)[NTS][NSS], C2f (&S3)[NTS][NSS]) {
     constexpr bool HELM = OP >= kHelmholtzSL;
     static_assert(!DUAL || OP == kLaplaceSL || OP == kHelmholtzSL, "DUAL pairs a single layer with its double layer");
+    static_assert(!SYM || DUAL, "SYM extends the DUAL pass");
+    constexpr bool NXDOT = OP == kLaplaceADL || OP == kHelmholtzADL || SYM;  // needs (x - y).n_x
     const int nq = QT > 0 ? QT : rule.n;
     const float fd0 = (float)d0, fd1 = (float)d1, fd2 = (float)d2;
     // reference distance r0 (FP32 value) and the FP64 residual |d|^2 - r0^2
@@ -230,7 +263,7 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
 
     // test normal projections of the trial offsets (ADL)
     float bnx[TrialF32<QT>::Q];
-    if constexpr (OP == kLaplaceADL || OP == kHelmholtzADL) {
+    if constexpr (NXDOT) {
 #pragma unroll(QT > 0 ? QT : 1)
         for (int q = 0; q < nq; ++q)
             bnx[q] = F.b[q][0] * T_.nx[t][0] + F.b[q][1] * T_.nx[t][1] + F.b[q][2] * T_.nx[t][2];
@@ -238,6 +271,7 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
 
     Acc<float, KAHAN> Sr[NTS][NSS], Si[NTS][NSS];
     Acc<float, KAHAN> Sr2[DUAL ? NTS : 1][DUAL ? NSS : 1], Si2[DUAL ? NTS : 1][DUAL ? NSS : 1];
+    Acc<float, KAHAN> Sr3[SYM ? NTS : 1][SYM ? NSS : 1], Si3[SYM ? NTS : 1][SYM ? NSS : 1];
 #pragma unroll
     for (int i = 0; i < NTS; ++i)
 #pragma unroll
@@ -247,6 +281,10 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
             if constexpr (DUAL) {
                 Sr2[i][j].zero();
                 Si2[i][j].zero();
+            }
+            if constexpr (SYM) {
+                Sr3[i][j].zero();
+                Si3[i][j].zero();
             }
         }
     // the test-point loop stays rolled: unrolling it lets the scheduler keep all 36
@@ -260,13 +298,13 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
         const float tw0 = 2.f * w0, tw1 = 2.f * w1, tw2 = 2.f * w2;
         float wn = 0.f, wnx = 0.f;
         if constexpr (OP == kLaplaceDL || OP == kHelmholtzDL || DUAL) wn = w0 * F.ny[0] + w1 * F.ny[1] + w2 * F.ny[2];
-        if constexpr (OP == kLaplaceADL || OP == kHelmholtzADL)
-            wnx = -(w0 * T_.nx[t][0] + w1 * T_.nx[t][1] + w2 * T_.nx[t][2]);
-        float Tr[NSS], Ti[NSS], Tr2[DUAL ? NSS : 1], Ti2[DUAL ? NSS : 1];
+        if constexpr (NXDOT) wnx = -(w0 * T_.nx[t][0] + w1 * T_.nx[t][1] + w2 * T_.nx[t][2]);
+        float Tr[NSS], Ti[NSS], Tr2[DUAL ? NSS : 1], Ti2[DUAL ? NSS : 1], Tr3[SYM ? NSS : 1], Ti3[SYM ? NSS : 1];
 #pragma unroll
         for (int b = 0; b < NSS; ++b) {
             Tr[b] = Ti[b] = 0.f;
             if constexpr (DUAL) Tr2[b] = Ti2[b] = 0.f;
+            if constexpr (SYM) Tr3[b] = Ti3[b] = 0.f;
         }
 #pragma unroll(QT > 0 ? QT : 1)
         for (int q = 0; q < nq; ++q) {
@@ -274,12 +312,18 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
             const float tt = Ap + F.bb[q] + tw0 * F.b[q][0] + tw1 * F.b[q][1] + tw2 * F.b[q][2];
             const float r2 = r0sq + tt;
             const float rinv = __builtin_amdgcn_rsqf(r2);
-            float gre, gim, g2re = 0.f, g2im = 0.f;
+            float gre, gim, g2re = 0.f, g2im = 0.f, g3re = 0.f, g3im = 0.f;
             if constexpr (!HELM) {
                 float g = rinv * F.wq[q];
                 if constexpr (OP == kLaplaceSL) {
                     gre = g;
-                    if constexpr (DUAL) g2re = -(wn + F.bn[q]) * g * rinv * rinv;
+                    if constexpr (SYM) {
+                        const float h = -g * rinv * rinv;
+                        g2re = (wn + F.bn[q]) * h;
+                        g3re = (wnx - bnx[q]) * h;
+                    } else if constexpr (DUAL) {
+                        g2re = -(wn + F.bn[q]) * g * rinv * rinv;
+                    }
                 } else if constexpr (OP == kLaplaceDL) {
                     gre = -(wn + F.bn[q]) * g * rinv * rinv;
                 } else {  // (y - x).n_x = bnx - wnx
@@ -296,7 +340,16 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
                 if constexpr (OP == kHelmholtzSL || OP == kMaxwellEFIE) {
                     gre = c * g;
                     gim = s * g;
-                    if constexpr (DUAL) {
+                    if constexpr (SYM) {
+                        const float gd = g * rinv * rinv;
+                        const float ar = DAMP ? -1.f - ki_f * r : -1.f, ai = kr_f * r;
+                        const float hr = (ar * c - ai * s) * gd, hi = (ai * c + ar * s) * gd;
+                        const float dot2 = wn + F.bn[q], dot3 = wnx - bnx[q];
+                        g2re = hr * dot2;
+                        g2im = hi * dot2;
+                        g3re = hr * dot3;
+                        g3im = hi * dot3;
+                    } else if constexpr (DUAL) {
                         const float gd = g * (wn + F.bn[q]) * rinv * rinv;
                         const float ar = DAMP ? -1.f - ki_f * r : -1.f, ai = kr_f * r;
                         g2re = (ar * c - ai * s) * gd;
@@ -318,6 +371,10 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
                 Tr2[0] += g2re;
                 if constexpr (HELM) Ti2[0] += g2im;
             }
+            if constexpr (SYM) {
+                Tr3[0] += g3re;
+                if constexpr (HELM) Ti3[0] += g3im;
+            }
             if constexpr (NSS == 3) {
                 const float s0 = (float)rule.x[q], s1 = (float)rule.y[q];
                 Tr[1] += gre * s0;
@@ -334,6 +391,14 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
                         Ti2[2] += g2im * s1;
                     }
                 }
+                if constexpr (SYM) {
+                    Tr3[1] += g3re * s0;
+                    Tr3[2] += g3re * s1;
+                    if constexpr (HELM) {
+                        Ti3[1] += g3im * s0;
+                        Ti3[2] += g3im * s1;
+                    }
+                }
             }
         }
         const float wp = T_.wp[t][p];
@@ -347,6 +412,10 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
                 if constexpr (DUAL) {
                     Sr2[i][j].add(u[i] * Tr2[j]);
                     if constexpr (HELM) Si2[i][j].add(u[i] * Ti2[j]);
+                }
+                if constexpr (SYM) {
+                    Sr3[i][j].add(u[i] * Tr3[j]);
+                    if constexpr (HELM) Si3[i][j].add(u[i] * Ti3[j]);
                 }
             }
     }
@@ -372,6 +441,12 @@ __device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP> &T_, int t, co
                 si = HELM ? Si2[i][j].s - Si2[i][j].c : 0.f;
                 S2[i][j].re = pr * sr - pi * si;
                 S2[i][j].im = pr * si + pi * sr;
+            }
+            if constexpr (SYM) {
+                sr = Sr3[i][j].s - Sr3[i][j].c;
+                si = HELM ? Si3[i][j].s - Si3[i][j].c : 0.f;
+                S3[i][j].re = pr * sr - pi * si;
+                S3[i][j].im = pr * si + pi * sr;
             }
         }
 }
@@ -432,9 +507,18 @@ __device__ __forceinline__ void store_direct(void *out, int64_t ld, int row, int
 // ------------------------------------------------------------------ kernel
 // DUAL: OP is a single layer and the kernel also assembles the double layer of the same family
 // into a.out2 from the same point-pair geometry and phase factors.
+// SYM (DUAL, TILE, test space == trial space on one grid): every unordered element pair is
+// evaluated once. The test tiles are cut inside the trial blocks of one element order; a work
+// item is a trial block with a run of test tiles, all of them in this block or in earlier ones
+// (the workgroup keeps its trial elements in registers across the run). In the tile's own
+// block only the pairs with list position of the trial element > that of the test element
+// count. Each pair (e, f) adds its single-layer block S_ef to a.out at (e, f) only (the caller
+// adds the transpose afterwards: S_fe = S_ef^T), its double-layer block to a.out2 at (e, f)
+// and the double layer of the reversed pair (test normal) to a.out2 at (f, e).
 template <int OP, int NTS, int NSS, int QT, int TM, bool MIXED, bool KAHAN, bool TILE, int MAXU_T, int MAXU_S,
-          typename OutT, bool DUAL = false>
-__global__ __launch_bounds__(kRegThreads, DUAL ? 3 : 1) void regular_kernel(const RegularArgs a) {
+          typename OutT, bool DUAL = false, bool SYM = false>
+__global__ __launch_bounds__(kRegThreads, DUAL && MIXED ? (SYM ? 2 : 3) : 1) void regular_kernel(const RegularArgs a) {
+    static_assert(!SYM || (DUAL && TILE), "the symmetric pass is a tiled single + double layer pass");
     constexpr bool CPLX = OpTraits<OP>::is_complex;
     constexpr int NT = NTS, NS = NSS;
     constexpr int NC = CPLX ? 2 : 1;
@@ -447,21 +531,34 @@ __global__ __launch_bounds__(kRegThreads, DUAL ? 3 : 1) void regular_kernel(cons
     using AccT = OutT;  // LDS accumulation type: float in MIXED (complex64), double in FP64 mode
     const int nq = QT > 0 ? QT : a.rule.n;
 
-    __shared__ TestTile<TM, MIXED ? 1 : TM> T_;
+    // test elements in LDS: one tile, or for SYM the whole run of tiles of the work item
+    constexpr int TMC = SYM ? kSymRunElements : TM;
+    constexpr int QC = QT > 0 ? QT : kMaxQ;
+    __shared__ TestTile<TMC, MIXED ? 1 : TMC, QC> T_;
     __shared__ AccT acc[TILE ? MAXU_T * MAXU_S * NC : 1];
     __shared__ AccT acc2[TILE && DUAL ? MAXU_T * MAXU_S * NC : 1];
-    __shared__ float stage[OWNER ? NT * NS * NC : 1][OWNER ? kRegThreads : 1];
+    __shared__ AccT acc3[TILE && SYM ? MAXU_T * MAXU_S * NC : 1];
+    // owner-pass stage, [(j * NT + i)][lane][re, im]; column kRegThreads of the rows j = 0 stays
+    // zero and stands in for missing incidences, so the gather below needs no branches
+    constexpr int SROW = kRegThreads + 1;
+    __shared__ __align__(16) float stage[OWNER ? NT * NS * SROW * NC : 1];
 
     const int tid = threadIdx.x;
-    int t0, t1, s0, s1;
+    int block_id = blockIdx.x, tile_begin = blockIdx.y, tile_end = blockIdx.y + 1;
+    if constexpr (SYM) {
+        block_id = a.work[3 * blockIdx.x];
+        tile_begin = a.work[3 * blockIdx.x + 1];
+        tile_end = a.work[3 * blockIdx.x + 2];
+        // malformed work items (the host never produces them); uniform per workgroup, before any barrier
+        if (block_id < 0 || block_id >= a.sb.nblocks || tile_begin < 0 || tile_end > a.tb.nblocks ||
+            tile_begin > tile_end)
+            return;
+    }
+    int s0, s1;
     if constexpr (TILE) {
-        t0 = a.tb.start[blockIdx.y];
-        t1 = a.tb.start[blockIdx.y + 1];
-        s0 = a.sb.start[blockIdx.x];
-        s1 = a.sb.start[blockIdx.x + 1];
+        s0 = a.sb.start[block_id];
+        s1 = a.sb.start[block_id + 1];
     } else {
-        t0 = blockIdx.y * TM;
-        t1 = min(t0 + TM, a.test.n);
         s0 = blockIdx.x * kRegThreads;
         s1 = min(s0 + kRegThreads, a.trial.n);
     }
@@ -469,43 +566,49 @@ __global__ __launch_bounds__(kRegThreads, DUAL ? 3 : 1) void regular_kernel(cons
     s1 = min(s1, s0 + kRegThreads);
     const int jf = s0 + tid;
     const bool valid = jf < s1;
-    const int ntile = max(0, min(t1 - t0, TM));
 
-    if constexpr (TILE) {
-        for (int i = tid; i < MAXU_T * MAXU_S * (CPLX ? 2 : 1); i += kRegThreads) {
-            acc[i] = (AccT)0;
-            if constexpr (DUAL) acc2[i] = (AccT)0;
-        }
-    }
-    if (tid < ntile) {
-        const int pos = t0 + tid;
-        const int e = a.test.elements[pos];
-        const double *g = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
-        T_.elem[tid] = e;
-        for (int i = 0; i < 3; ++i) T_.vid[tid][i] = a.test.vids[4 * e + i];
-        for (int i = 0; i < NT; ++i) {
-            T_.mult[tid][i] = a.test.mult[(int64_t)e * NT + i];
-            if constexpr (TILE) T_.slot[tid][i] = min(max(a.tb.slots[3 * pos + i], 0), MAXU_T - 1);
-        }
-        centroid_rn(g, T_.c[tid]);
-        T_.diam[tid] = g[13];
-        double nm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
-        T_.nx[tid][0] = (float)(g[9] * nm);
-        T_.nx[tid][1] = (float)(g[10] * nm);
-        T_.nx[tid][2] = (float)(g[11] * nm);
-        for (int p = 0; p < nq; ++p) {
-            double xi = a.rule.x[p], eta = a.rule.y[p];
-            for (int d = 0; d < 3; ++d) {
-                double off = g[3 + d] * (xi - 1.0 / 3.0) + g[6 + d] * (eta - 1.0 / 3.0);
-                if constexpr (!MIXED) T_.xp[tid][p][d] = g[d] + g[3 + d] * xi + g[6 + d] * eta;
-                T_.a[tid][p][d] = (float)off;
+    // stage the test elements at list positions [pos0, pos0 + n) into T_[0, n): all lanes share
+    // the element tasks and the (element, quadrature point) tasks
+    auto stage_tests = [&](int pos0, int n) {
+        for (int task = tid; task < n * (1 + nq); task += kRegThreads) {
+            const int el = task < n ? task : (task - n) / nq;
+            const int pos = pos0 + el;
+            const int e = a.test.elements[pos];
+            const double *g = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
+            if (task < n) {
+                T_.elem[el] = e;
+                for (int i = 0; i < 3; ++i) T_.vid[el][i] = a.test.vids[4 * e + i];
+                for (int i = 0; i < NT; ++i) {
+                    T_.mult[el][i] = a.test.mult[(int64_t)e * NT + i];
+                    if constexpr (TILE) T_.slot[el][i] = min(max(a.tb.slots[3 * pos + i], 0), MAXU_T - 1);
+                }
+                centroid_rn(g, T_.c[el]);
+                T_.diam[el] = g[13];
+                const double nm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
+                T_.nx[el][0] = (float)(g[9] * nm);
+                T_.nx[el][1] = (float)(g[10] * nm);
+                T_.nx[el][2] = (float)(g[11] * nm);
+            } else {
+                const int p = (task - n) - el * nq;
+                const double xi = a.rule.x[p], eta = a.rule.y[p];
+                float off[3];
+                for (int d = 0; d < 3; ++d) {
+                    off[d] = (float)(g[3 + d] * (xi - 1.0 / 3.0) + g[6 + d] * (eta - 1.0 / 3.0));
+                    if constexpr (!MIXED) T_.xp[el][p][d] = g[d] + g[3 + d] * xi + g[6 + d] * eta;
+                    T_.a[el][p][d] = off[d];
+                }
+                T_.aa[el][p] = off[0] * off[0] + off[1] * off[1] + off[2] * off[2];
+                T_.wp[el][p] = (float)(a.rule.w[p] * g[12]);
             }
-            T_.aa[tid][p] = T_.a[tid][p][0] * T_.a[tid][p][0] + T_.a[tid][p][1] * T_.a[tid][p][1] +
-                            T_.a[tid][p][2] * T_.a[tid][p][2];
-            T_.wp[tid][p] = (float)(a.rule.w[p] * g[12]);
         }
+    };
+    int run0 = 0;
+    if constexpr (SYM) {
+        run0 = a.tb.start[tile_begin];
+        const int run_n = a.tb.start[tile_end] - run0;
+        if (run_n < 0 || run_n > TMC) return;  // malformed work item; uniform, before any barrier
+        stage_tests(run0, run_n);  // the first tile's barrier publishes it
     }
-    __syncthreads();
 
     // trial element of this lane
     int f = 0;
@@ -569,198 +672,263 @@ __global__ __launch_bounds__(kRegThreads, DUAL ? 3 : 1) void regular_kernel(cons
         }
     }
 
-    // owner lane state: the incidences (3 * lane + j) of trial slot `tid` in this block
-    int own_inc[OWNER ? kMaxInc : 1];
+    // owner lane state: the incidences (lane p, shape function j) of trial slot `tid` in this
+    // block, as stage offsets of row (j, i = 0) and column p; missing ones point at the zero column
+    int own_off[OWNER ? kMaxInc : 1];
     int own_nds = 0;
     if constexpr (OWNER) {
-        own_nds = min(a.sb.ndofs[blockIdx.x], MAXU_S);
-        const int16_t *src = a.sb.inc + ((int64_t)blockIdx.x * a.sb.maxu + min(tid, a.sb.maxu - 1)) * a.sb.maxinc;
+        own_nds = min(a.sb.ndofs[block_id], MAXU_S);
+        const int16_t *src = a.sb.inc + ((int64_t)block_id * a.sb.maxu + min(tid, a.sb.maxu - 1)) * a.sb.maxinc;
 #pragma unroll
         for (int k = 0; k < kMaxInc; ++k) {
-            int code = (tid < own_nds && k < a.sb.maxinc) ? (int)src[k] : -1;
-            own_inc[k] = code < 3 * kRegThreads ? code : -1;
+            const int code = (tid < own_nds && k < a.sb.maxinc) ? (int)src[k] : -1;
+            const int p = code / 3, j = code - 3 * p;
+            own_off[k] = (code >= 0 && code < 3 * kRegThreads) ? (j * NT * SROW + p) * NC : kRegThreads * NC;
         }
+        if (tid < NT * NC) stage[(tid / NC) * SROW * NC + kRegThreads * NC + tid % NC] = 0.f;  // zero column
     }
 
-    for (int t = 0; t < ntile; ++t) {
-        const int e = T_.elem[t];
-        const double *gt = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
-        bool adj = false;
-        if (a.identical) {
-            int v0 = T_.vid[t][0], v1 = T_.vid[t][1], v2 = T_.vid[t][2];
-            adj = v0 == fv0 || v0 == fv1 || v0 == fv2 || v1 == fv0 || v1 == fv1 || v1 == fv2 || v2 == fv0 ||
-                  v2 == fv1 || v2 == fv2;
+    for (int tile_id = tile_begin; tile_id < tile_end; ++tile_id) {
+        int t0, t1;
+        if constexpr (TILE) {
+            t0 = a.tb.start[tile_id];
+            t1 = a.tb.start[tile_id + 1];
+        } else {
+            t0 = tile_id * TM;
+            t1 = min(t0 + TM, a.test.n);
         }
-        C2d S[NTS][NSS], S2[NTS][NSS];   // FP64 moments (FP64 mode; MIXED paths that need the FP64 epilogue)
-        C2f Sf[NTS][NSS], Sf2[NTS][NSS];  // FP32 moments (MIXED)
-#pragma unroll
-        for (int i = 0; i < NTS; ++i)
-#pragma unroll
-            for (int j = 0; j < NSS; ++j) {
-                S[i][j].re = S[i][j].im = 0.0;
-                Sf[i][j].re = Sf[i][j].im = 0.f;
-                S2[i][j].re = S2[i][j].im = 0.0;
-                Sf2[i][j].re = Sf2[i][j].im = 0.f;
-            }
+        const int ntile = max(0, min(t1 - t0, TM));
+        // SYM: the test tile lies in this trial block; only pairs with jf > test position count
+        const bool diag = SYM && t0 >= s0 && t0 < s1;
+        const int toff = SYM ? t0 - run0 : 0;  // T_ index of the tile's first element
 
-        // MIXED: near pairs are left to the FP64 near-field kernel (bem_near_assemble)
-        bool skip = adj;
-        if (valid && !adj) {
-            if constexpr (!MIXED) {
-                double tnm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
-                if constexpr (QT > 0)
-                    pair_fp64_regs<OP, NTS, NSS, QT, TM, MIXED ? 1 : TM, DUAL>(T_, t, gt, tnm, yq, wq64, ny64, a.rule,
-                                                                             a.kr, a.ki, S, S2);
-                else
-                    pair_fp64_geom<OP, NTS, NSS, QT, DUAL>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2);
-            } else {
-                const double cfv[3] = {cf0, cf1, cf2};
-                double dist2;
-                skip = pair_is_near(T_.c[t], cfv, T_.diam[t], diam_f, a.near_factor, &dist2);
-                if (!skip) {
-                    double d0 = cf0 - T_.c[t][0], d1 = cf1 - T_.c[t][1], d2 = cf2 - T_.c[t][2];
-                    if (OpTraits<OP>::is_complex && a.ki != 0.0)
-                        pair_fp32<OP, NTS, NSS, QT, TM, KAHAN, true, DUAL>(T_, t, F32, d0, d1, d2, dist2, a.rule, a.kr,
-                                                                           a.ki, Sf, Sf2);
-                    else
-                        pair_fp32<OP, NTS, NSS, QT, TM, KAHAN, false, DUAL>(T_, t, F32, d0, d1, d2, dist2, a.rule, a.kr,
-                                                                            a.ki, Sf, Sf2);
-                }
+        if constexpr (TILE) {
+            for (int i = tid; i < MAXU_T * MAXU_S * (CPLX ? 2 : 1); i += kRegThreads) {
+                acc[i] = (AccT)0;
+                if constexpr (DUAL) acc2[i] = (AccT)0;
+                if constexpr (SYM) acc3[i] = (AccT)0;
             }
         }
-        if constexpr (MIXED && (!OWNER || OP == kMaxwellEFIE)) {
-#pragma unroll
+        if constexpr (!SYM) stage_tests(t0, ntile);
+        __syncthreads();
+
+        for (int t = 0; t < ntile; ++t) {
+            const int te = toff + t;
+            const int e = T_.elem[te];
+            const double *gt = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
+            bool adj = false;
+            if (a.identical) {
+                int v0 = T_.vid[te][0], v1 = T_.vid[te][1], v2 = T_.vid[te][2];
+                adj = v0 == fv0 || v0 == fv1 || v0 == fv2 || v1 == fv0 || v1 == fv1 || v1 == fv2 || v2 == fv0 ||
+                      v2 == fv1 || v2 == fv2;
+            }
+            if constexpr (SYM) adj = adj || (diag && jf <= t0 + t);  // the pair belongs to the other orientation
+            C2d S[NTS][NSS], S2[NTS][NSS], S3[NTS][NSS];     // FP64 moments (FP64 mode; MIXED paths that need the FP64 epilogue)
+            C2f Sf[NTS][NSS], Sf2[NTS][NSS], Sf3[NTS][NSS];  // FP32 moments (MIXED)
+    #pragma unroll
             for (int i = 0; i < NTS; ++i)
-#pragma unroll
+    #pragma unroll
                 for (int j = 0; j < NSS; ++j) {
-                    S[i][j].re = (double)Sf[i][j].re;
-                    S[i][j].im = (double)Sf[i][j].im;
-                    if constexpr (DUAL) {
-                        S2[i][j].re = (double)Sf2[i][j].re;
-                        S2[i][j].im = (double)Sf2[i][j].im;
-                    }
+                    S[i][j].re = S[i][j].im = 0.0;
+                    Sf[i][j].re = Sf[i][j].im = 0.f;
+                    S2[i][j].re = S2[i][j].im = 0.0;
+                    Sf2[i][j].re = Sf2[i][j].im = 0.f;
+                    S3[i][j].re = S3[i][j].im = 0.0;
+                    Sf3[i][j].re = Sf3[i][j].im = 0.f;
                 }
-        }
 
-        if constexpr (OWNER) {
-            // stage this lane's block (zero for idle lanes, skipped pairs, zero multipliers), then
-            // the owner lane of every trial slot adds the entries of its incidences to its tile column
-            auto owner_pass = [&](const C2f(&Sm)[NTS][NSS], const C2d(&Sd)[NTS][NSS], AccT *tile) {
-                C2f V[NTS][NSS];
-                if constexpr (OP != kMaxwellEFIE) scalar_block_f32<NTS, NSS>(Sm, V);
-#pragma unroll
-                for (int i = 0; i < NT; ++i)
-#pragma unroll
-                    for (int j = 0; j < NS; ++j) {
-                        float vr = 0.f, vi = 0.f;
-                        const float m = (float)T_.mult[t][i] * (float)mf[j];
-                        if (valid && !skip && m != 0.f) {
-                            if constexpr (OP == kMaxwellEFIE) {
-                                C2d v = local_value<OP, NTS, NSS>(Sd, i, j, gt, gf, a.kr, a.ki);
-                                vr = (float)v.re * m;
-                                vi = (float)v.im * m;
-                            } else {
-                                vr = V[i][j].re * m;
-                                vi = V[i][j].im * m;
-                            }
-                        }
-                        stage[(j * NT + i) * NC][tid] = vr;
-                        if constexpr (CPLX) stage[(j * NT + i) * NC + 1][tid] = vi;
-                    }
-                __syncthreads();
-                if (tid < own_nds) {
-#pragma unroll
-                    for (int i = 0; i < NT; ++i) {
-                        if (T_.mult[t][i] == 0.0) continue;
-                        float sr = 0.f, si = 0.f;
-#pragma unroll
-                        for (int k = 0; k < kMaxInc; ++k) {
-                            const int code = own_inc[k];
-                            if (code >= 0) {
-                                const int p = code / 3, j = code - 3 * p;
-                                sr += stage[(j * NT + i) * NC][p];
-                                if constexpr (CPLX) si += stage[(j * NT + i) * NC + 1][p];
-                            }
-                        }
-                        AccT *dst = tile + NC * (T_.slot[t][i] * MAXU_S + tid);
-                        dst[0] += (AccT)sr;
-                        if constexpr (CPLX) dst[1] += (AccT)si;
+            // MIXED: near pairs are left to the FP64 near-field kernel (bem_near_assemble)
+            bool skip = adj;
+            if (valid && !adj) {
+                if constexpr (!MIXED) {
+                    double tnm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
+                    if constexpr (QT > 0)
+                        pair_fp64_regs<OP, NTS, NSS, QT, TMC, MIXED ? 1 : TMC, DUAL, SYM>(T_, te, gt, tnm, yq, wq64, ny64,
+                                                                                      a.rule, a.kr, a.ki, S, S2, S3);
+                    else
+                        pair_fp64_geom<OP, NTS, NSS, QT, DUAL, SYM>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2, S3);
+                } else {
+                    const double cfv[3] = {cf0, cf1, cf2};
+                    double dist2;
+                    skip = pair_is_near(T_.c[te], cfv, T_.diam[te], diam_f, a.near_factor, &dist2);
+                    if (!skip) {
+                        double d0 = cf0 - T_.c[te][0], d1 = cf1 - T_.c[te][1], d2 = cf2 - T_.c[te][2];
+                        if (OpTraits<OP>::is_complex && a.ki != 0.0)
+                            pair_fp32<OP, NTS, NSS, QT, TMC, KAHAN, true, DUAL, 1, SYM>(T_, te, F32, d0, d1, d2, dist2, a.rule,
+                                                                                       a.kr, a.ki, Sf, Sf2, Sf3);
+                        else
+                            pair_fp32<OP, NTS, NSS, QT, TMC, KAHAN, false, DUAL, 1, SYM>(T_, te, F32, d0, d1, d2, dist2,
+                                                                                        a.rule, a.kr, a.ki, Sf, Sf2, Sf3);
                     }
                 }
-                __syncthreads();
-            };
-            owner_pass(Sf, S, acc);
-            if constexpr (DUAL) owner_pass(Sf2, S2, acc2);
-        } else if (valid) {
-#pragma unroll
-            for (int i = 0; i < NT; ++i) {
-                const double mi = T_.mult[t][i];
-#pragma unroll
-                for (int j = 0; j < NS; ++j) {
-                    const double m = mi * mf[j];
-                    if constexpr (TILE) {
-                        if (m != 0.0 && !skip) {
+            }
+            if constexpr (MIXED && (!OWNER || OP == kMaxwellEFIE)) {
+    #pragma unroll
+                for (int i = 0; i < NTS; ++i)
+    #pragma unroll
+                    for (int j = 0; j < NSS; ++j) {
+                        S[i][j].re = (double)Sf[i][j].re;
+                        S[i][j].im = (double)Sf[i][j].im;
+                        if constexpr (DUAL) {
+                            S2[i][j].re = (double)Sf2[i][j].re;
+                            S2[i][j].im = (double)Sf2[i][j].im;
+                        }
+                        if constexpr (SYM) {
+                            S3[i][j].re = (double)Sf3[i][j].re;
+                            S3[i][j].im = (double)Sf3[i][j].im;
+                        }
+                    }
+            }
+
+            if constexpr (OWNER) {
+                // stage this lane's block (zero for idle lanes, skipped pairs, zero multipliers), then
+                // the owner lane of every trial slot adds the entries of its incidences to its tile column
+                auto owner_pass = [&](const C2f(&Sm)[NTS][NSS], const C2d(&Sd)[NTS][NSS], AccT *tile) {
+                    C2f V[NTS][NSS];
+                    if constexpr (OP != kMaxwellEFIE) scalar_block_f32<NTS, NSS>(Sm, V);
+    #pragma unroll
+                    for (int i = 0; i < NT; ++i)
+    #pragma unroll
+                        for (int j = 0; j < NS; ++j) {
+                            float vr = 0.f, vi = 0.f;
+                            const float m = (float)T_.mult[te][i] * (float)mf[j];
+                            if (valid && !skip && m != 0.f) {
+                                if constexpr (OP == kMaxwellEFIE) {
+                                    C2d v = local_value<OP, NTS, NSS>(Sd, i, j, gt, gf, a.kr, a.ki);
+                                    vr = (float)v.re * m;
+                                    vi = (float)v.im * m;
+                                } else {
+                                    vr = V[i][j].re * m;
+                                    vi = V[i][j].im * m;
+                                }
+                            }
+                            float *dst = stage + ((j * NT + i) * SROW + tid) * NC;
+                            if constexpr (CPLX) *reinterpret_cast<float2 *>(dst) = make_float2(vr, vi);
+                            else dst[0] = vr;
+                        }
+                    __syncthreads();
+                    if (tid < own_nds) {
+                        // all reads of a row are issued before any sum: one LDS round trip per row
+                        auto gather = [&](int i, int k0, int k1, float &sr, float &si) {
+    #pragma unroll
+                            for (int k = k0; k < k1; ++k) {
+                                const float *src = stage + own_off[k] + i * SROW * NC;
+                                if constexpr (CPLX) {
+                                    const float2 v = *reinterpret_cast<const float2 *>(src);
+                                    sr += v.x;
+                                    si += v.y;
+                                } else {
+                                    sr += src[0];
+                                }
+                            }
+                        };
+                        constexpr int KLO = kMaxInc < 6 ? kMaxInc : 6;
+                        const bool more = a.sb.maxinc > KLO;  // uniform: vertices of valence > 6
+    #pragma unroll
+                        for (int i = 0; i < NT; ++i) {
+                            if (T_.mult[te][i] == 0.0) continue;
+                            float sr = 0.f, si = 0.f;
+                            gather(i, 0, KLO, sr, si);
+                            if (more) gather(i, KLO, kMaxInc, sr, si);
+                            AccT *dst = tile + NC * (T_.slot[te][i] * MAXU_S + tid);
+                            dst[0] += (AccT)sr;
+                            if constexpr (CPLX) dst[1] += (AccT)si;
+                        }
+                    }
+                    __syncthreads();
+                };
+                owner_pass(Sf, S, acc);
+                if constexpr (DUAL) owner_pass(Sf2, S2, acc2);
+                if constexpr (SYM) owner_pass(Sf3, S3, acc3);
+            } else if (valid) {
+    #pragma unroll
+                for (int i = 0; i < NT; ++i) {
+                    const double mi = T_.mult[te][i];
+    #pragma unroll
+                    for (int j = 0; j < NS; ++j) {
+                        const double m = mi * mf[j];
+                        if constexpr (TILE) {
+                            if (m != 0.0 && !skip) {
+                                C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
+                                const int off = (CPLX ? 2 : 1) * (T_.slot[te][i] * MAXU_S + slot_f[j]);
+                                unsafeAtomicAdd(acc + off, (AccT)(v.re * m));
+                                if constexpr (CPLX) unsafeAtomicAdd(acc + off + 1, (AccT)(v.im * m));
+                                if constexpr (DUAL) {
+                                    C2d v2 = local_value<OP2, NTS, NSS>(S2, i, j, gt, gf, a.kr, a.ki);
+                                    unsafeAtomicAdd(acc2 + off, (AccT)(v2.re * m));
+                                    if constexpr (CPLX) unsafeAtomicAdd(acc2 + off + 1, (AccT)(v2.im * m));
+                                }
+                                if constexpr (SYM) {
+                                    C2d v3 = local_value<OP2, NTS, NSS>(S3, i, j, gt, gf, a.kr, a.ki);
+                                    unsafeAtomicAdd(acc3 + off, (AccT)(v3.re * m));
+                                    if constexpr (CPLX) unsafeAtomicAdd(acc3 + off + 1, (AccT)(v3.im * m));
+                                }
+                            }
+                        } else {
                             C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
-                            const int off = (CPLX ? 2 : 1) * (T_.slot[t][i] * MAXU_S + slot_f[j]);
-                            unsafeAtomicAdd(acc + off, (AccT)(v.re * m));
-                            if constexpr (CPLX) unsafeAtomicAdd(acc + off + 1, (AccT)(v.im * m));
+                            v.re *= m;
+                            v.im *= m;
+                            int row = a.test.l2g[(int64_t)e * NT + i];
+                            int col = a.trial.l2g[(int64_t)f * NS + j];
+                            store_direct<OutT, CPLX>(a.out, a.ld, row, col, v, a.accumulate);
                             if constexpr (DUAL) {
                                 C2d v2 = local_value<OP2, NTS, NSS>(S2, i, j, gt, gf, a.kr, a.ki);
-                                unsafeAtomicAdd(acc2 + off, (AccT)(v2.re * m));
-                                if constexpr (CPLX) unsafeAtomicAdd(acc2 + off + 1, (AccT)(v2.im * m));
+                                v2.re *= m;
+                                v2.im *= m;
+                                store_direct<OutT, CPLX>(a.out2, a.ld, row, col, v2, a.accumulate);
                             }
-                        }
-                    } else {
-                        C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
-                        v.re *= m;
-                        v.im *= m;
-                        int row = a.test.l2g[(int64_t)e * NT + i];
-                        int col = a.trial.l2g[(int64_t)f * NS + j];
-                        store_direct<OutT, CPLX>(a.out, a.ld, row, col, v, a.accumulate);
-                        if constexpr (DUAL) {
-                            C2d v2 = local_value<OP2, NTS, NSS>(S2, i, j, gt, gf, a.kr, a.ki);
-                            v2.re *= m;
-                            v2.im *= m;
-                            store_direct<OutT, CPLX>(a.out2, a.ld, row, col, v2, a.accumulate);
                         }
                     }
                 }
             }
         }
-    }
 
-    if constexpr (TILE) {
-        __syncthreads();
-        const int ndt = min(a.tb.ndofs[blockIdx.y], MAXU_T), nds = min(a.sb.ndofs[blockIdx.x], MAXU_S);
-        const int32_t *dt = a.tb.dofs + (int64_t)blockIdx.y * a.tb.maxu;
-        const int32_t *ds = a.sb.dofs + (int64_t)blockIdx.x * a.sb.maxu;
-        auto flush = [&](const AccT *tile, void *out) {
-            OutT *o = reinterpret_cast<OutT *>(out);
-            for (int idx = tid; idx < ndt * nds; idx += kRegThreads) {
-                const int ia = idx / nds, is = idx - ia * nds;
-                const AccT *src = tile + (CPLX ? 2 : 1) * (ia * MAXU_S + is);
-                const int64_t pos = (int64_t)dt[ia] * a.ld + ds[is];
-                if constexpr (CPLX) {
-                    if (src[0] != (AccT)0) unsafeAtomicAdd(o + 2 * pos, src[0]);
-                    if (src[1] != (AccT)0) unsafeAtomicAdd(o + 2 * pos + 1, src[1]);
-                } else {
-                    if (src[0] != (AccT)0) unsafeAtomicAdd(o + pos, src[0]);
+        if constexpr (TILE) {
+            __syncthreads();
+            const int ndt = min(a.tb.ndofs[tile_id], MAXU_T), nds = min(a.sb.ndofs[block_id], MAXU_S);
+            const int32_t *dt = a.tb.dofs + (int64_t)tile_id * a.tb.maxu;
+            const int32_t *ds = a.sb.dofs + (int64_t)block_id * a.sb.maxu;
+            // transposed: entry (test slot ia, trial slot is) goes to (ds[is], dt[ia]); consecutive
+            // lanes then walk one output row
+            auto flush = [&](const AccT *tile, void *out, bool transposed) {
+                OutT *o = reinterpret_cast<OutT *>(out);
+                for (int idx = tid; idx < ndt * nds; idx += kRegThreads) {
+                    int ia, is;
+                    if (transposed) {
+                        is = idx / ndt;
+                        ia = idx - is * ndt;
+                    } else {
+                        ia = idx / nds;
+                        is = idx - ia * nds;
+                    }
+                    const AccT *src = tile + (CPLX ? 2 : 1) * (ia * MAXU_S + is);
+                    const int64_t pos = transposed ? (int64_t)ds[is] * a.ld + dt[ia] : (int64_t)dt[ia] * a.ld + ds[is];
+                    if constexpr (CPLX) {
+                        if (src[0] != (AccT)0) unsafeAtomicAdd(o + 2 * pos, src[0]);
+                        if (src[1] != (AccT)0) unsafeAtomicAdd(o + 2 * pos + 1, src[1]);
+                    } else {
+                        if (src[0] != (AccT)0) unsafeAtomicAdd(o + pos, src[0]);
+                    }
                 }
-            }
-        };
-        flush(acc, a.out);
-        if constexpr (DUAL) flush(acc2, a.out2);
-    }
+            };
+            flush(acc, a.out, false);
+            if constexpr (DUAL) flush(acc2, a.out2, false);
+            if constexpr (SYM) flush(acc3, a.out2, true);
+        }
+        // the next tile reuses the test tile and the LDS tiles
+        if (tile_id + 1 < tile_end) __syncthreads();
+    }  // tile loop
 }
 
 // ------------------------------------------------------------------ dispatch
-template <int OP, int NTS, int NSS, int QT, bool MIXED, typename OutT, bool DUAL = false>
+template <int OP, int NTS, int NSS, int QT, bool MIXED, typename OutT, bool DUAL = false, bool SYM = false>
 static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
-    // DUAL keeps two LDS tiles; its smaller test tiles keep the LDS footprint at 3 workgroups per CU
-    constexpr int TM = DUAL ? BEM_TILE_PAIR_TEST_ELEMENTS : BEM_TILE_TEST_ELEMENTS;
+    // DUAL keeps two LDS tiles and SYM three; their smaller test tiles keep the LDS footprint
+    // at 3 workgroups per CU
+    constexpr int TM = SYM ? BEM_TILE_SYM_TEST_ELEMENTS : DUAL ? BEM_TILE_PAIR_TEST_ELEMENTS : BEM_TILE_TEST_ELEMENTS;
     constexpr bool CPLX = OpTraits<OP>::is_complex;
     // LDS tile of distinct dofs: test slots x trial slots (BEM_TILE_TEST_DOFS x BEM_TILE_TRIAL_DOFS)
-    constexpr int MT = DUAL ? BEM_TILE_PAIR_TEST_DOFS : BEM_TILE_TEST_DOFS;
+    constexpr int MT = SYM ? BEM_TILE_SYM_TEST_DOFS : DUAL ? BEM_TILE_PAIR_TEST_DOFS : BEM_TILE_TEST_DOFS;
     constexpr int MS = BEM_TILE_TRIAL_DOFS;
     (void)CPLX;
     dim3 grid;
@@ -773,8 +941,12 @@ static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
             set_error("mixed precision needs trial incidence lists with 1..%d entries per dof", kMaxInc);
             return BEM_ERR_INVALID_ARGUMENT;
         }
-        grid = dim3(a.sb.nblocks, a.tb.nblocks);
+        grid = SYM ? dim3(a.nwork) : dim3(a.sb.nblocks, a.tb.nblocks);
     } else {
+        if (SYM) {
+            set_error("the symmetric pass needs dof blocks");
+            return BEM_ERR_INVALID_ARGUMENT;
+        }
         grid = dim3((a.trial.n + kRegThreads - 1) / kRegThreads, (a.test.n + TM - 1) / TM);
     }
     if (grid.x == 0 || grid.y == 0) return BEM_OK;
@@ -782,13 +954,14 @@ static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
         set_error("too many test tiles for one launch (%u)", grid.y);
         return BEM_ERR_INVALID_ARGUMENT;
     }
-#define BEM_LAUNCH(K, T)                                                                                         \
-    hipLaunchKernelGGL((regular_kernel<OP, NTS, NSS, QT, TM, MIXED, K, T, (T ? MT : 1), (T ? MS : 1), OutT, DUAL>), \
+#define BEM_LAUNCH(K, T)                                                                                              \
+    hipLaunchKernelGGL((regular_kernel<OP, NTS, NSS, QT, TM, MIXED, K, T, (T ? MT : 1), (T ? MS : 1), OutT, DUAL, \
+                                       SYM && T>),                                                                    \
                        grid, dim3(kRegThreads), 0, stream, a)
     if (tile) {
         if (MIXED && kahan) BEM_LAUNCH(true, true);
         else BEM_LAUNCH(false, true);
-    } else {
+    } else if constexpr (!SYM) {
         if (MIXED && kahan) BEM_LAUNCH(true, false);
         else BEM_LAUNCH(false, false);
     }
@@ -796,16 +969,23 @@ static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
     return check_launch("regular_kernel");
 }
 
-template <int OP, int NTS, int NSS, bool MIXED, typename OutT, bool DUAL = false>
+template <int OP, int NTS, int NSS, bool MIXED, typename OutT, bool DUAL = false, bool SYM = false>
 static int launch2(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
-    if (a.rule.n == 6) return launch3<OP, NTS, NSS, 6, MIXED, OutT, DUAL>(a, kahan, tile, stream);
-    return launch3<OP, NTS, NSS, 0, MIXED, OutT, DUAL>(a, kahan, tile, stream);
+    if (a.rule.n == 6) return launch3<OP, NTS, NSS, 6, MIXED, OutT, DUAL, SYM>(a, kahan, tile, stream);
+    return launch3<OP, NTS, NSS, 0, MIXED, OutT, DUAL, SYM>(a, kahan, tile, stream);
 }
 
-template <int OP, bool MIXED, typename OutT, bool DUAL = false>
+template <int OP, bool MIXED, typename OutT, bool DUAL = false, bool SYM = false>
 static int launch1(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
     if constexpr (OP == kMaxwellEFIE) {
         return launch2<OP, 3, 3, MIXED, OutT>(a, kahan, tile, stream);
+    } else if constexpr (SYM) {
+        // shared dofs (the tiled path) on one space: P1
+        if (nshape_of(a.test.shape) != 3 || nshape_of(a.trial.shape) != 3) {
+            set_error("the symmetric pass needs a space with three shape functions per element (P1)");
+            return BEM_ERR_INVALID_ARGUMENT;
+        }
+        return launch2<OP, 3, 3, MIXED, OutT, true, true>(a, kahan, tile, stream);
     } else {
         int nt = nshape_of(a.test.shape), ns = nshape_of(a.trial.shape);
         if (nt == 1 && ns == 1) return launch2<OP, 1, 1, MIXED, OutT, DUAL>(a, kahan, tile, stream);
@@ -815,24 +995,24 @@ static int launch1(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
     }
 }
 
-template <int OP, bool DUAL = false>
+template <int OP, bool DUAL = false, bool SYM = false>
 static int launch_op(const RegularArgs &a, int precision, int kahan, bool tile, int out_dtype, hipStream_t stream) {
     constexpr bool CPLX = OpTraits<OP>::is_complex;
     const bool mixed = precision == BEM_PRECISION_MIXED;
     if (CPLX) {
         if (out_dtype == BEM_C128)
-            return mixed ? launch1<OP, true, double, DUAL>(a, kahan, tile, stream)
-                         : launch1<OP, false, double, DUAL>(a, kahan, tile, stream);
+            return mixed ? launch1<OP, true, double, DUAL, SYM>(a, kahan, tile, stream)
+                         : launch1<OP, false, double, DUAL, SYM>(a, kahan, tile, stream);
         if (out_dtype == BEM_C64)
-            return mixed ? launch1<OP, true, float, DUAL>(a, kahan, tile, stream)
-                         : launch1<OP, false, float, DUAL>(a, kahan, tile, stream);
+            return mixed ? launch1<OP, true, float, DUAL, SYM>(a, kahan, tile, stream)
+                         : launch1<OP, false, float, DUAL, SYM>(a, kahan, tile, stream);
     } else {
         if (out_dtype == BEM_F64)
-            return mixed ? launch1<OP, true, double, DUAL>(a, kahan, tile, stream)
-                         : launch1<OP, false, double, DUAL>(a, kahan, tile, stream);
+            return mixed ? launch1<OP, true, double, DUAL, SYM>(a, kahan, tile, stream)
+                         : launch1<OP, false, double, DUAL, SYM>(a, kahan, tile, stream);
         if (out_dtype == BEM_F32)
-            return mixed ? launch1<OP, true, float, DUAL>(a, kahan, tile, stream)
-                         : launch1<OP, false, float, DUAL>(a, kahan, tile, stream);
+            return mixed ? launch1<OP, true, float, DUAL, SYM>(a, kahan, tile, stream)
+                         : launch1<OP, false, float, DUAL, SYM>(a, kahan, tile, stream);
     }
     set_error("output dtype %d does not match operator %d", out_dtype, OP);
     return BEM_ERR_INVALID_ARGUMENT;
@@ -853,5 +1033,8 @@ BEM_DECLARE_REGULAR_OP(6)
 // single layer + double layer in one pass (regular_dual<k>.hip, k = the single layer)
 int regular_launch_dual_0(const RegularArgs &a, int precision, int kahan, bool tile, int out_dtype, hipStream_t s);
 int regular_launch_dual_3(const RegularArgs &a, int precision, int kahan, bool tile, int out_dtype, hipStream_t s);
+// symmetric single + double layer pass (regular_sym<k>.hip)
+int regular_launch_sym_0(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
+int regular_launch_sym_3(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
 
 }  // namespace bem

@@ -1,0 +1,9 @@
+// Symmetric regular-pair kernels (every unordered pair once) assembling kHelmholtzSL and the
+// Helmholtz double layer in one pass (see regular_impl.h, SYM).
+#include "regular_impl.h"
+
+namespace bem {
+int regular_launch_sym_3(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s) {
+    return launch_op<kHelmholtzSL, true, true>(a, precision, kahan, true, out_dtype, s);
+}
+}  // namespace bem

@@ -5,10 +5,12 @@ Workload (one "step"): mixed-precision (FP32 far-field / FP64 near-field and sin
 complex64 output) dense Galerkin assembly of the Helmholtz single-layer and double-layer
 operators, P1 x P1 continuous, wavenumber k = 4, on the level-5 icosahedral unit sphere
 (20,480 flat triangles, 10,242 vertices): two 10,242 x 10,242 complex64 matrices in HBM,
-both assembled in one pass (shared point-pair geometry and e^{ikr}).
+both assembled in one symmetric pass (every unordered element pair once; shared point-pair
+geometry and e^{ikr} for the single layer, the double layer and the reversed pair).
 Reported value: matrix entries per second (2 * 10,242^2 per step). Outside the timed region:
-the two operators assembled separately, the FP64 assemblies (separate = the plain path, and
-fused) with the relative Frobenius error of the mixed result, a Kahan-summation variant, the
+the pass over ordered pairs, the two operators assembled separately, the FP64 assemblies
+(separate = the plain path, ordered fused, symmetric) with the relative Frobenius error of
+the mixed result, a Kahan-summation variant, the
 potential of a seeded complex Gaussian P1 density at 16,384 points on the sphere of radius 2,
 the frequency sweep (32 wavenumbers, DP0, level 4), the loss/gradient workload (level 6),
 kernel-level roofline figures (HIP events on the launch stream) and the CPU baseline (FP64
@@ -34,6 +36,10 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 # algorithmic FLOP per kernel evaluation (add/sub/mul 1, FMA 2, sqrt/rsqrt/rcp/sin/cos/exp 1)
 FLOP_PER_EVAL = {"helmholtz_single_layer": 35, "helmholtz_double_layer": 45}
+# FP32 VALU FLOP the symmetric mixed kernel executes per unordered point pair, counted from its
+# gfx950 assembly (inner loop over one test point and the 6 trial points: 205 FMA (2 FLOP),
+# 126 add/sub/mul, 24 sqrt/rcp/sin/cos -> 560 FLOP per 6 point pairs)
+EXEC_FLOP_PER_SYM_POINT_PAIR = 560.0 / 6.0
 FLOP_PER_POTENTIAL_EVAL = 23
 
 
@@ -508,9 +514,40 @@ def main():
     reg = float(np.mean([e["regular_start"].elapsed_time(e["regular_end"]) for e in ev]))
     sing = float(np.mean([e["regular_end"].elapsed_time(e["singular_end"]) for e in ev]))
     flop_pair = FLOP_PER_EVAL[SL] + FLOP_PER_EVAL[DL]
-    kern = {"fused_regular_ms": reg, "fused_near_singular_ms": sing,
-            "fused_regular_tflops": n_evals * flop_pair / (reg * 1e-3) / 1e12}
-    log(f"mixed fused step {ms_per_step:.2f} ms, {value / 1e9:.2f} G entries/s; kernels {kern}")
+    # the regular section is zero fill (both outputs), the symmetric kernel and the transpose-add
+    # of the single layer; the two helpers are timed alone on the same stream and subtracted
+    aux = [{key: torch.cuda.Event(enable_timing=True) for key in ("a", "b", "c")} for _ in range(3)]
+    scratch = torch.empty_like(outs[SL])
+    for e in aux:
+        e["a"].record()
+        hb.dense_zero(scratch)
+        hb.dense_zero(scratch)
+        e["b"].record()
+        hb.dense_add_transpose(scratch)
+        e["c"].record()
+    torch.cuda.synchronize()
+    zero_ms = float(np.mean([e["a"].elapsed_time(e["b"]) for e in aux]))
+    trans_ms = float(np.mean([e["b"].elapsed_time(e["c"]) for e in aux]))
+    del scratch
+    kern_ms = reg - zero_ms - trans_ms
+    kern = {"symmetric_regular_section_ms": reg, "symmetric_kernel_ms": kern_ms, "zero_fill_ms": zero_ms,
+            "add_transpose_ms": trans_ms, "symmetric_near_singular_ms": sing,
+            "symmetric_kernel_tflops": n_evals * flop_pair / (kern_ms * 1e-3) / 1e12,
+            "symmetric_kernel_executed_tflops": n_evals / 2 * EXEC_FLOP_PER_SYM_POINT_PAIR / (kern_ms * 1e-3) / 1e12}
+    log(f"mixed symmetric step {ms_per_step:.2f} ms, {value / 1e9:.2f} G entries/s; kernels {kern}")
+
+    # the same fused pass over ordered pairs (parameters.hip.symmetric = False)
+    params.hip.symmetric = False
+    step()
+    torch.cuda.synchronize()
+    oev = {key: torch.cuda.Event(enable_timing=True) for key in evkeys}
+    t = time.perf_counter()
+    step(oev)
+    torch.cuda.synchronize()
+    kern["ordered_fused_ms_per_step"] = 1e3 * (time.perf_counter() - t)
+    kern["ordered_fused_regular_ms"] = oev["regular_start"].elapsed_time(oev["regular_end"])
+    params.hip.symmetric = True
+    log(f"ordered fused step {kern['ordered_fused_ms_per_step']:.2f} ms (regular {kern['ordered_fused_regular_ms']:.2f})")
 
     # the same two operators assembled separately (one weak_form() each)
     sep_ev = {name: {key: torch.cuda.Event(enable_timing=True) for key in evkeys} for name in ops}
@@ -545,7 +582,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "Helmholtz single+double layer, P1xP1, icosphere level 5 (20480 triangles, 10242 vertices), "
-                        "k=4.0, mixed FP32/FP64 -> complex64, dense in HBM, both operators in one pass",
+                        "k=4.0, mixed FP32/FP64 -> complex64, dense in HBM, both operators in one symmetric pass",
             "triangles": F,
             "dofs": N,
             "wavenumber": k,
@@ -555,17 +592,20 @@ def main():
         },
     }
     result["roofline"] = {
-        "kernel": "regular_kernel<helmholtz_single_layer + double layer (fused), P1xP1, mixed>",
+        "kernel": "regular_kernel<helmholtz_single_layer + double layer, symmetric, P1xP1, mixed>",
         "bound": "valu",
-        "achieved": kern["fused_regular_tflops"],
+        "achieved": kern["symmetric_kernel_tflops"],
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
-        "frac": kern["fused_regular_tflops"] / PEAK_FP32_TFLOPS,
-        "traffic": traffic_from_profiles("regular_kernel_helmholtz_sl_dl_p1_level5_mixed"),
+        "frac": kern["symmetric_kernel_tflops"] / PEAK_FP32_TFLOPS,
+        "traffic": traffic_from_profiles("regular_kernel_helmholtz_sl_dl_sym_p1_level5_mixed"),
         "algorithmic_flop_per_launch": n_evals * flop_pair,
-        "flop_per_eval": f"{FLOP_PER_EVAL[SL]} (single layer) + {FLOP_PER_EVAL[DL]} (double layer)",
+        "flop_per_eval": f"{FLOP_PER_EVAL[SL]} (single layer) + {FLOP_PER_EVAL[DL]} (double layer) per ordered "
+                         f"point pair; the symmetric pass evaluates each unordered point pair once",
         "evals_per_launch": n_evals,
-        "launch_ms": reg,
+        "executed_flop_per_launch": n_evals / 2 * EXEC_FLOP_PER_SYM_POINT_PAIR,
+        "executed_tflops": kern["symmetric_kernel_executed_tflops"],
+        "launch_ms": kern_ms,
     }
     result["kernels"] = kern
 
@@ -580,21 +620,30 @@ def main():
             torch.cuda.synchronize()
             t64 += time.perf_counter() - t
             errs[name] = rel_fro(outs[name], a64[name])
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        f1, f2 = hip_dense_assemble_sl_dl(space, space, params, desc64[SL], desc64[DL])
-        torch.cuda.synchronize()
-        t64f = time.perf_counter() - t
-        fused64_err = max(rel_fro(f1, a64[SL]), rel_fro(f2, a64[DL]))
-        del f1, f2, a64
+        # FP64 fused over ordered pairs, then FP64 symmetric (the same algorithm as the step)
+        fp64 = {}
+        for label, sym in (("ordered_fused", False), ("symmetric", True)):
+            params.hip.symmetric = sym
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            f1, f2 = hip_dense_assemble_sl_dl(space, space, params, desc64[SL], desc64[DL])
+            torch.cuda.synchronize()
+            fp64[label] = (time.perf_counter() - t, max(rel_fro(f1, a64[SL]), rel_fro(f2, a64[DL])))
+            del f1, f2
+        params.hip.symmetric = True
+        del a64
         result["rel_err_vs_fp64"] = errs
         result["fp64_entries_per_s"] = entries_per_step / t64
         result["speedup_vs_fp64"] = value / result["fp64_entries_per_s"]
-        result["fp64_fused_entries_per_s"] = entries_per_step / t64f
-        result["speedup_vs_fp64_fused"] = value / result["fp64_fused_entries_per_s"]
-        result["fp64_fused_vs_separate_rel_err"] = fused64_err
+        for label, (tl, el) in fp64.items():
+            result[f"fp64_{label}_entries_per_s"] = entries_per_step / tl
+            result[f"speedup_vs_fp64_{label}"] = value / result[f"fp64_{label}_entries_per_s"]
+            result[f"fp64_{label}_vs_separate_rel_err"] = el
         result["speedup_separate_mixed_vs_fp64"] = t64 / (sep_ms * 1e-3)
-        log(f"fp64 separate {1e3 * t64:.1f} ms, fused {1e3 * t64f:.1f} ms, rel err {errs}")
+        result["speedup_ordered_fused_mixed_vs_fp64_ordered_fused"] = fp64["ordered_fused"][0] / (
+            kern["ordered_fused_ms_per_step"] * 1e-3)
+        log(f"fp64 separate {1e3 * t64:.1f} ms, ordered fused {1e3 * fp64['ordered_fused'][0]:.1f} ms, symmetric "
+            f"{1e3 * fp64['symmetric'][0]:.1f} ms, rel err {errs}")
 
         # the same step with Kahan-compensated FP32 sums (parameters.hip.kahan)
         from bempp_amd.api.utils.parameters import DefaultParameters

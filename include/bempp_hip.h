@@ -116,6 +116,11 @@ typedef struct bem_dof_blocks {
 /* test tiles of bem_regular_assemble_sl_dl with blocks (two LDS tiles) */
 #define BEM_TILE_PAIR_TEST_ELEMENTS 8
 #define BEM_TILE_PAIR_TEST_DOFS 12
+/* test tiles of bem_regular_assemble_sl_dl_symmetric (three LDS tiles) */
+#define BEM_TILE_SYM_TEST_ELEMENTS 8
+#define BEM_TILE_SYM_TEST_DOFS 12
+/* test elements of one work item of bem_regular_assemble_sl_dl_symmetric (its run of tiles) */
+#define BEM_TILE_SYM_RUN_ELEMENTS 48
 
 /*
  * Regular part of a dense Galerkin operator: all (test, trial) element pairs, except,
@@ -160,6 +165,30 @@ int bem_regular_assemble_sl_dl(int op, const bem_space_t *test, const bem_space_
                                int grids_identical, const bem_quad_rule_t *rule, double k_re, double k_im,
                                int precision, double mixed_near_factor, int kahan, void *out_single, void *out_double,
                                int64_t ld, int out_dtype, int accumulate, void *stream);
+
+/*
+ * The same pair of operators for one P1 space on one grid (test space == trial space), with
+ * every unordered element pair evaluated once (no reference counterpart; the reference
+ * computes each ordered pair, bempp_cl/core/numba_kernels.py:980-1074). The single layer of
+ * the pair (f, e) is the transpose of that of (e, f); the double layer of (f, e) shares the
+ * distance and e^{ikr} and only needs the other normal.
+ *
+ * trial_blocks: blocking of the space's element list (as for bem_regular_assemble_blocked,
+ * with incidences). test_tiles: a blocking of the same element list whose tiles (at most
+ * BEM_TILE_SYM_TEST_ELEMENTS elements, BEM_TILE_SYM_TEST_DOFS dofs) each lie inside one trial
+ * block, in list order. work: n_work triples (block b, first tile, end tile); together they must
+ * list, for every block b, each tile of blocks 0..b exactly once, and the tiles of one triple may
+ * hold at most BEM_TILE_SYM_RUN_ELEMENTS elements (one workgroup per triple keeps the block's
+ * trial elements in registers and the run's test elements in LDS). In a tile's own block only pairs
+ * with the trial element later in the list than the test element are evaluated. out_single and out_double (n_dofs x n_dofs,
+ * leading dimension ld) are OVERWRITTEN with the regular part: pairs sharing a vertex are
+ * excluded, and in MIXED precision so are the near pairs (bem_near_assemble_sl_dl adds them).
+ */
+int bem_regular_assemble_sl_dl_symmetric(int op, const bem_space_t *space, const bem_dof_blocks_t *test_tiles,
+                                         const bem_dof_blocks_t *trial_blocks, int n_work, const int32_t *work,
+                                         int64_t n_dofs, const bem_quad_rule_t *rule, double k_re, double k_im,
+                                         int precision, double mixed_near_factor, int kahan, void *out_single,
+                                         void *out_double, int64_t ld, int out_dtype, void *stream);
 
 /*
  * Near-field regular pairs of the MIXED mode, in FP64 (no reference counterpart: the
@@ -285,6 +314,14 @@ int bem_adjoint_potential_single_layer(const bem_space_t *space, const bem_quad_
  */
 int bem_dense_matvec(int trans, const void *A, int a_dtype, int64_t rows, int64_t cols, int64_t ld, const void *x,
                      void *y, void *stream);
+
+/* A <- A + A^T for a square n x n matrix (leading dimension ld, dtype BEM_F32/F64/C64/C128),
+ * in place (completes the single layer of bem_regular_assemble_sl_dl_symmetric). */
+int bem_dense_add_transpose(void *A, int dtype, int64_t n, int64_t ld, void *stream);
+
+/* A <- 0 for a square n x n matrix (leading dimension ld), with a fill kernel (no memset node,
+ * so the call can be captured in a HIP graph). */
+int bem_dense_zero(void *A, int dtype, int64_t n, int64_t ld, void *stream);
 
 /* Set `bytes` bytes at `ptr` to zero with a fill kernel (safe inside graph capture). */
 int bem_fill_zero(void *ptr, size_t bytes, void *stream);

@@ -60,3 +60,16 @@ def test_error_path_without_gpu():
     rc = L.bem_kbatch_assemble(ctypes.byref(sp), ctypes.byref(sp), 1, ctypes.byref(rule), 65, ks, 1, 7, 2.0,
                                ctypes.byref(cfg), ctypes.c_void_p(16), 1, 1, 2, 0, None, 0, None, None, None, None, None)
     assert rc != 0
+    # symmetric pass: missing outputs / blocks and a malformed work list are rejected on the host
+    tb = hip_backend.BemDofBlocks()
+    rc = L.bem_regular_assemble_sl_dl_symmetric(3, ctypes.byref(sp), ctypes.byref(tb), ctypes.byref(tb), 1, None, 4,
+                                                ctypes.byref(rule), 1.0, 0.0, 1, 2.0, 0, ctypes.c_void_p(16), None, 4,
+                                                2, None)
+    assert rc == 1
+    assert b"null" in L.bem_last_error()
+    rc = L.bem_regular_assemble_sl_dl_symmetric(3, ctypes.byref(sp), ctypes.byref(tb), ctypes.byref(tb), 0, None, 8,
+                                                ctypes.byref(rule), 1.0, 0.0, 1, 2.0, 0, ctypes.c_void_p(16),
+                                                ctypes.c_void_p(16), 4, 2, None)
+    assert rc == 1  # ld < n_dofs
+    rc = L.bem_dense_add_transpose(ctypes.c_void_p(16), 2, 8, 4, None)
+    assert rc == 1

@@ -1,0 +1,58 @@
+"""Host side of the symmetric single + double layer pass (no GPU): the nested blocking and
+its work list visit every unordered element pair exactly once."""
+
+import numpy as np
+import pytest
+
+
+@pytest.mark.parametrize("level,run", [(1, None), (2, None), (2, 3), (3, None), (3, 7)])
+def test_work_list_covers_each_unordered_pair_once(level, run):
+    from bempp_amd.api import function_space
+    from bempp_amd.api.grid.shapes import icosphere
+    from bempp_amd.core.blocking import SYM_TEST_TILE_DOFS, SYM_TEST_TILE_ELEMENTS, make_symmetric_blocks
+
+    grid = icosphere(level)
+    p1 = function_space(grid, "P", 1)
+    trial, tiles, owner, work = make_symmetric_blocks(p1, tiles_per_item=run)
+    n = len(trial.order)
+    assert np.array_equal(tiles.order, trial.order)
+    assert np.diff(tiles.start).max() <= SYM_TEST_TILE_ELEMENTS and tiles.ndofs.max() <= SYM_TEST_TILE_DOFS
+    # every tile lies inside the block it is assigned to
+    for t, b in enumerate(owner):
+        assert trial.start[b] <= tiles.start[t] and tiles.start[t + 1] <= trial.start[b + 1]
+    # the kernel's rule: in the tile's own block only trial positions after the test position
+    count = np.zeros((n, n), dtype=np.int32)
+    assert work.shape[1] == 3 and (work[:, 1] < work[:, 2]).all()
+    for b, ta, tb in work:
+        for t in range(ta, tb):
+            t0, t1 = tiles.start[t], tiles.start[t + 1]
+            s0, s1 = trial.start[b], trial.start[b + 1]
+            pe = np.arange(t0, t1)[:, None]
+            pf = np.arange(s0, s1)[None, :]
+            take = np.ones((t1 - t0, s1 - s0), dtype=bool)
+            if s0 <= t0 < s1:
+                take = pf > pe
+            np.add.at(count, (np.broadcast_to(pe, take.shape)[take], np.broadcast_to(pf, take.shape)[take]), 1)
+    both = count + count.T
+    off = ~np.eye(n, dtype=bool)
+    assert (both[off] == 1).all()
+    assert (np.diag(both) == 0).all()
+    # slots of the tiles name the dofs of their elements
+    l2g = p1.local2global
+    for t in range(tiles.n_blocks):
+        for pos in range(tiles.start[t], tiles.start[t + 1]):
+            for i in range(3):
+                assert tiles.dofs[t, tiles.slots[pos, i]] == l2g[trial.order[pos], i]
+
+
+def test_symmetric_pass_predicate():
+    from bempp_amd.api import function_space
+    from bempp_amd.api.grid.shapes import icosphere
+    from bempp_amd.core.dense_assembler import use_symmetric_pass
+
+    g = icosphere(1)
+    p1 = function_space(g, "P", 1)
+    assert use_symmetric_pass(p1, function_space(g, "P", 1))
+    assert not use_symmetric_pass(p1, function_space(g, "DP", 1))
+    assert not use_symmetric_pass(function_space(g, "DP", 0), function_space(g, "DP", 0))
+    assert not use_symmetric_pass(p1, function_space(icosphere(2), "P", 1))
