@@ -1,3 +1,4 @@
+from bempp_amd.api.utils.parameters import DEFAULT_NEAR_FIELD_FACTOR
 """Boundary operators with cached weak forms (bempp_cl/api/assembly/boundary_operator.py:4-131)."""
 
 
@@ -114,7 +115,7 @@ def _fusion_key(op):
         p.quadrature.regular,
         p.quadrature.singular,
         p.assembly.always_promote_to_double,
-        getattr(hip, "mixed_near_field_factor", 2.0),
+        getattr(hip, "mixed_near_field_factor", DEFAULT_NEAR_FIELD_FACTOR),
         getattr(hip, "kahan", False),
         op.assembler._device_interface,
     )

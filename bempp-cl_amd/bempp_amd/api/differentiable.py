@@ -17,6 +17,7 @@ coefficient per element. Gradients follow torch's convention for complex tensors
 loss L of a complex output y, the backward receives dL/d(conj y) times 2).
 """
 
+from bempp_amd.api.utils.parameters import DEFAULT_NEAR_FIELD_FACTOR
 import numpy as _np
 
 
@@ -93,7 +94,7 @@ class SingleLayerProblem(object):
 
     def near_factor(self):
         hip = getattr(self.parameters, "hip", None)
-        return getattr(hip, "mixed_near_field_factor", 2.0) if hip is not None else 2.0
+        return getattr(hip, "mixed_near_field_factor", DEFAULT_NEAR_FIELD_FACTOR) if hip is not None else DEFAULT_NEAR_FIELD_FACTOR
 
 
 def _full_coefficients(space, c):

@@ -20,6 +20,9 @@ SYM_TEST_TILE_DOFS = 12
 SYM_RUN_ELEMENTS = 48  # test elements per work item (BEM_TILE_SYM_RUN_ELEMENTS)
 TRIAL_BLOCK_ELEMENTS = 256
 TRIAL_BLOCK_DOFS = 160
+RWG_MIXED_TEST_TILE_ELEMENTS = 8  # mixed-precision EFIE (BEM_TILE_RWG_MIXED_*)
+RWG_MIXED_TEST_TILE_DOFS = 16
+RWG_MIXED_TRIAL_BLOCK_DOFS = 416
 MAX_INCIDENCE = 12
 
 _CACHE = {}
@@ -178,11 +181,16 @@ _DEVICE_CACHE = {}
 
 
 def device_blocks(space, role):
-    """Device blocking of a space as test ('test', 'test_pair' for the fused pair) or trial ('trial') side."""
+    """Device blocking of a space as test ('test', 'test_pair' for the fused pair, 'test_rwg_mixed')
+    or trial ('trial', 'trial_rwg_mixed') side."""
     if role == "test":
         me, md = TEST_TILE_ELEMENTS, TEST_TILE_DOFS
     elif role == "test_pair":
         me, md = PAIR_TEST_TILE_ELEMENTS, PAIR_TEST_TILE_DOFS
+    elif role == "test_rwg_mixed":
+        me, md = RWG_MIXED_TEST_TILE_ELEMENTS, RWG_MIXED_TEST_TILE_DOFS
+    elif role == "trial_rwg_mixed":
+        me, md = TRIAL_BLOCK_ELEMENTS, RWG_MIXED_TRIAL_BLOCK_DOFS
     else:
         me, md = TRIAL_BLOCK_ELEMENTS, TRIAL_BLOCK_DOFS
     key = (space.id, me, md)

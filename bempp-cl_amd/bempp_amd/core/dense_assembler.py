@@ -10,6 +10,7 @@ runs the MIXED mode (FP32 far pairs, FP64 near and singular pairs) and returns
 float32/complex64.
 """
 
+from bempp_amd.api.utils.parameters import DEFAULT_NEAR_FIELD_FACTOR
 import numpy as _np
 
 from bempp_amd.api.utils.helpers import get_type
@@ -122,7 +123,7 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     if not accumulate and not full_cover:
         hb.fill_zero(out)
     hip_opts = getattr(parameters, "hip", None)
-    near = getattr(hip_opts, "mixed_near_field_factor", 2.0) if hip_opts is not None else 2.0
+    near = getattr(hip_opts, "mixed_near_field_factor", DEFAULT_NEAR_FIELD_FACTOR) if hip_opts is not None else DEFAULT_NEAR_FIELD_FACTOR
     kahan = getattr(hip_opts, "kahan", False) if hip_opts is not None else False
     srule = None
     if grids_identical and singular:
@@ -138,8 +139,11 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     else:
         from bempp_amd.core.blocking import device_blocks
 
+        # the mixed EFIE kernel takes 256-element trial blocks of up to ~400 edges
+        rwg_mixed = op == "maxwell_electric_field" and mode == hb.PRECISION_MIXED
         hb.regular_assemble_blocked(
-            op, device_blocks(dual_to_range, "test"), device_blocks(domain, "trial"),
+            op, device_blocks(dual_to_range, "test_rwg_mixed" if rwg_mixed else "test"),
+            device_blocks(domain, "trial_rwg_mixed" if rwg_mixed else "trial"),
             grids_identical, rule, _wavenumber(operator_descriptor), mode, out, near_factor=near, kahan=kahan,
         )
     if events is not None:
@@ -227,7 +231,7 @@ def hip_dense_assemble_sl_dl(domain, dual_to_range, parameters, sl_descriptor, d
         and domain.number_of_support_elements * domain.number_of_shape_functions == cols
     )
     hip_opts = getattr(parameters, "hip", None)
-    near = getattr(hip_opts, "mixed_near_field_factor", 2.0) if hip_opts is not None else 2.0
+    near = getattr(hip_opts, "mixed_near_field_factor", DEFAULT_NEAR_FIELD_FACTOR) if hip_opts is not None else DEFAULT_NEAR_FIELD_FACTOR
     kahan = getattr(hip_opts, "kahan", False) if hip_opts is not None else False
     symmetric = (
         getattr(hip_opts, "symmetric", True) if hip_opts is not None else True
@@ -264,8 +268,12 @@ def hip_dense_assemble_sl_dl(domain, dual_to_range, parameters, sl_descriptor, d
     if mode == hb.PRECISION_MIXED:
         from bempp_amd.core.near_field import device_near_candidates
 
-        cand = device_near_candidates(dual_to_range, domain, near)
-        hb.near_assemble_sl_dl(op, test, trial, cand, grids_identical, rule, k, near, outs[0], outs[1])
+        if symmetric:
+            cand = device_near_candidates(dual_to_range, domain, near, unordered=True)
+            hb.near_assemble_sl_dl_symmetric(op, test, cand, rule, k, near, outs[0], outs[1])
+        else:
+            cand = device_near_candidates(dual_to_range, domain, near)
+            hb.near_assemble_sl_dl(op, test, trial, cand, grids_identical, rule, k, near, outs[0], outs[1])
     if grids_identical:
         srule = singular_rule(dual_to_range.grid, parameters.quadrature.singular, dual_to_range.support, domain.support)
         hb.singular_assemble_sl_dl(op, test, trial, srule.pairs, srule.test_points, srule.trial_points, srule.weights,

@@ -9,6 +9,7 @@ the mixed mode and the FP64 Sauter-Schwab singular pairs for all wavenumbers. Th
 shape comes from the autotuner (bempp_amd/core/autotune.py).
 """
 
+from bempp_amd.api.utils.parameters import DEFAULT_NEAR_FIELD_FACTOR
 import numpy as _np
 
 
@@ -66,7 +67,7 @@ def hip_kbatch_assemble(domain, dual_to_range, wavenumbers, precision="single", 
         raise ValueError("the frequency sweep uses the order-4 (6-point) regular rule")
     rule = hb.quad_rule(points, weights)
     hip_opts = getattr(parameters, "hip", None)
-    near = getattr(hip_opts, "mixed_near_field_factor", 2.0) if hip_opts is not None else 2.0
+    near = getattr(hip_opts, "mixed_near_field_factor", DEFAULT_NEAR_FIELD_FACTOR) if hip_opts is not None else DEFAULT_NEAR_FIELD_FACTOR
     cand = device_near_candidates(dual_to_range, domain, near) if mode == hb.PRECISION_MIXED else None
     srule = None
     if grids_identical:

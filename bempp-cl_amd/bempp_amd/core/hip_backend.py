@@ -6,6 +6,7 @@ and streams come from PyTorch-ROCm; all compute happens in the HIP library. Ther
 CPU fallback: if the library is missing or no GPU is visible, the calls raise.
 """
 
+from bempp_amd.api.utils.parameters import DEFAULT_NEAR_FIELD_FACTOR
 import ctypes
 import os
 
@@ -137,6 +138,8 @@ def lib():
         L.bem_dense_zero.restype = i
         L.bem_near_assemble_sl_dl.argtypes = [i, SP, SP, i, P, i, QP, d, d, d, P, P, i64, i, P]
         L.bem_near_assemble_sl_dl.restype = i
+        L.bem_near_assemble_sl_dl_symmetric.argtypes = [i, SP, i, P, QP, d, d, d, P, P, i64, i, P]
+        L.bem_near_assemble_sl_dl_symmetric.restype = i
         L.bem_singular_assemble_sl_dl.argtypes = [i, SP, SP, i, P, P, P, P, d, d, P, P, i64, i, P]
         L.bem_singular_assemble_sl_dl.restype = i
         _LIB = L
@@ -271,7 +274,7 @@ def _dtype_code(t):
 
 
 def regular_assemble(op, test, trial, grids_identical, rule, wavenumber, precision, out, accumulate=0,
-                     near_factor=2.0, kahan=True):
+                     near_factor=DEFAULT_NEAR_FIELD_FACTOR, kahan=True):
     """Regular pairs of localised spaces (DP0/DP1): direct stores into out."""
     k = complex(wavenumber)
     check(
@@ -284,7 +287,7 @@ def regular_assemble(op, test, trial, grids_identical, rule, wavenumber, precisi
 
 
 def regular_assemble_blocked(op, test_blocks, trial_blocks, grids_identical, rule, wavenumber, precision, out,
-                             near_factor=2.0, kahan=True):
+                             near_factor=DEFAULT_NEAR_FIELD_FACTOR, kahan=True):
     """Regular pairs of spaces with shared dofs (P1, RWG/SNC): LDS-tiled sums added to out."""
     k = complex(wavenumber)
     check(
@@ -412,7 +415,7 @@ def dense_matvec(A, x, trans=0, out=None):
 
 
 def regular_assemble_sl_dl(op, test, trial, test_blocks, trial_blocks, grids_identical, rule, wavenumber, precision,
-                           out_single, out_double, accumulate=0, near_factor=2.0, kahan=False):
+                           out_single, out_double, accumulate=0, near_factor=DEFAULT_NEAR_FIELD_FACTOR, kahan=False):
     """Single layer (op) and its double layer in one pass; blocks None for localised spaces.
 
     With blocks, the element lists of the blocks (their space_struct) replace those of test/trial.
@@ -431,7 +434,7 @@ def regular_assemble_sl_dl(op, test, trial, test_blocks, trial_blocks, grids_ide
     )
 
 
-def regular_assemble_sl_dl_symmetric(op, sym, rule, wavenumber, precision, out_single, out_double, near_factor=2.0,
+def regular_assemble_sl_dl_symmetric(op, sym, rule, wavenumber, precision, out_single, out_double, near_factor=DEFAULT_NEAR_FIELD_FACTOR,
                                      kahan=False):
     """Single layer (op) and its double layer on one P1 space, every unordered pair once.
 
@@ -475,6 +478,18 @@ def near_assemble_sl_dl(op, test, trial, pairs, grids_identical, rule, wavenumbe
             OPERATORS[op], ctypes.byref(test.struct), ctypes.byref(trial.struct), int(pairs.shape[0]), ptr(pairs),
             int(grids_identical), ctypes.byref(rule), k.real, k.imag, float(near_factor), ptr(out_single),
             ptr(out_double), out_single.shape[-1], _dtype_code(out_single), stream_handle(),
+        )
+    )
+
+
+def near_assemble_sl_dl_symmetric(op, space, pairs, rule, wavenumber, near_factor, out_single, out_double):
+    """Near pairs of the symmetric pass: each (e, f) of `pairs` once, both orientations added."""
+    k = complex(wavenumber)
+    check(
+        lib().bem_near_assemble_sl_dl_symmetric(
+            OPERATORS[op], ctypes.byref(space.struct), int(pairs.shape[0]), ptr(pairs), ctypes.byref(rule), k.real,
+            k.imag, float(near_factor), ptr(out_single), ptr(out_double), out_single.shape[-1],
+            _dtype_code(out_single), stream_handle(),
         )
     )
 

@@ -34,16 +34,45 @@ def near_candidates(test_space, trial_space, factor):
     return pairs
 
 
+def pruned_candidates(test_space, trial_space, factor, unordered=False):
+    """The candidates without the pairs the device would reject for certain.
+
+    Drops pairs that share a vertex (same grid: the singular kernel takes them) and pairs whose
+    centroid distance exceeds factor * (diam_t + diam_f) / 2 by more than a 5e-5 relative margin
+    (the device applies the exact test to what remains). unordered: keep each pair of one space
+    with itself once, as (e, f) with e < f (the symmetric pass computes both orientations).
+    """
+    key = ("pruned", test_space.id, trial_space.id, float(factor), bool(unordered))
+    if key in _CACHE:
+        return _CACHE[key]
+    pairs = near_candidates(test_space, trial_space, factor)
+    e, f = pairs[:, 0].astype(_np.int64), pairs[:, 1].astype(_np.int64)
+    tg, sg = test_space.grid, trial_space.grid
+    keep = _np.ones(len(pairs), dtype=bool)
+    if unordered:
+        keep &= e < f
+    if tg == sg:
+        ve, vf = tg.elements[:, e], tg.elements[:, f]
+        keep &= ~(ve[:, None, :] == vf[None, :, :]).any(axis=(0, 1))
+    d = tg.centroids[e] - sg.centroids[f]
+    lim = factor * 0.5 * (tg.diameters[e] + sg.diameters[f])
+    keep &= (d * d).sum(axis=1) < lim * lim * (1.0 + 1e-4)
+    out = _np.ascontiguousarray(pairs[keep])
+    _CACHE[key] = out
+    return out
+
+
 _DEVICE_CACHE = {}
 
 
-def device_near_candidates(test_space, trial_space, factor):
+def device_near_candidates(test_space, trial_space, factor, unordered=False):
     import torch
     from bempp_amd.core import hip_backend as hb
 
-    key = (test_space.id, trial_space.id, float(factor))
+    key = (test_space.id, trial_space.id, float(factor), bool(unordered))
     if key not in _DEVICE_CACHE:
-        _DEVICE_CACHE[key] = torch.from_numpy(near_candidates(test_space, trial_space, factor)).to(hb.device())
+        _DEVICE_CACHE[key] = torch.from_numpy(pruned_candidates(test_space, trial_space, factor, unordered)).to(
+            hb.device())
     return _DEVICE_CACHE[key]
 
 

@@ -58,6 +58,7 @@ struct BlockArgs {
 };
 
 constexpr int kMaxInc = 12;  // incidences per dof and block held in registers by the owner lane
+constexpr int kRwgMaxInc = 2;  // an RWG dof (edge) has two elements
 
 struct RegularArgs {
     SpaceArgs test, trial;
@@ -76,8 +77,9 @@ struct RegularArgs {
 };
 
 
-// NXP: 1 in MIXED mode, where the absolute FP64 test points are not needed; Q: points per element
-template <int TM, int NXP = TM, int Q = kMaxQ>
+// NXP: 1 in MIXED mode, where the absolute FP64 test points are not needed; Q: points per element;
+// EF: the FP32 EFIE epilogue's element data (edge vectors, edge lengths, 1 / J)
+template <int TM, int NXP = TM, int Q = kMaxQ, bool EF = false>
 struct TestTile {
     int elem[TM];
     int vid[TM][3];
@@ -90,6 +92,9 @@ struct TestTile {
     float aa[TM][Q];       // |a|^2
     float wp[TM][Q];       // w_p J_e
     float nx[TM][3];       // test normal (multiplied)
+    float ee[EF ? TM : 1][2][3];
+    float el[EF ? TM : 1][3];
+    float ej[EF ? TM : 1];
 };
 
 // test elements staged per SYM work item (a run of test tiles)
@@ -190,8 +195,8 @@ __device__ __forceinline__ void pair_fp64_geom(const double *__restrict__ gt, co
 
 // FP64 pair with test points from LDS and trial points from registers. DUAL: S2 = double
 // layer; SYM: S3 = double layer of the reversed pair.
-template <int OP, int NTS, int NSS, int QT, int TM, int NXP, bool DUAL = false, bool SYM = false>
-__device__ __forceinline__ void pair_fp64_regs(const TestTile<TM, NXP, QT> &T_, int t, const double *gt, double tnm,
+template <int OP, int NTS, int NSS, int QT, bool DUAL = false, bool SYM = false, typename TT>
+__device__ __forceinline__ void pair_fp64_regs(const TT &T_, int t, const double *gt, double tnm,
                                                const double (&yq)[QT > 0 ? QT : 1][3], const double (&wq)[QT > 0 ? QT : 1],
                                                const double *ny, const RuleArgs &rule, double kr, double ki,
                                                C2d (&S)[NTS][NSS], C2d (&S2)[NTS][NSS], C2d (&S3)[NTS][NSS]) {
@@ -242,9 +247,8 @@ __device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf
 // DUAL (OP a single layer): S2 receives the double layer of the same family, sharing the
 // distance, 1/r and e^{ik delta} of every point pair. SYM (with DUAL): S3 receives the double
 // layer of the reversed pair (normal of the test element), from the same values again.
-template <int OP, int NTS, int NSS, int QT, int TM, bool KAHAN, bool DAMP, bool DUAL, int NXP, bool SYM = false>
-__device__ __forceinline__ void pair_fp32(const TestTile<TM, NXP, (QT > 0 ? QT : kMaxQ)> &T_, int t, const TrialF32<QT> &F,
-                                          double d0, double d1,
+template <int OP, int NTS, int NSS, int QT, bool KAHAN, bool DAMP, bool DUAL, bool SYM = false, typename TT>
+__device__ __forceinline__ void pair_fp32(const TT &T_, int t, const TrialF32<QT> &F, double d0, double d1,
                                           double d2, double dist2, const RuleArgs &rule, double kr, double ki,
                                           C2f (&S)[NTS][NSS], C2f (&S2)[NTS][NSS], C2f (&S3)[NTS][NSS]) {
     constexpr bool HELM = OP >= kHelmholtzSL;
@@ -480,6 +484,70 @@ __device__ __forceinline__ void scalar_block_f32(const C2f (&S)[NTS][NSS], C2f (
     }
 }
 
+// Maxwell EFIE local block in FP32 from FP32 moments (the formula of local_value<kMaxwellEFIE>):
+//   V[i][j] = l_i l_j [ -ik sum_ab M_ab (S[1+a][1+b] - o_i[a] S[0][1+b] - o_j[b] S[1+a][0] + o_i[a] o_j[b] S00)
+//                       - 4 S00 / (ik) ] / (J_t J_f),   M = E_t^T E_f,  o = ((0, 1), (1, 0), (0, 0))
+// et/ef: edge vectors v1 - v0, v2 - v0; lt/lf: edge lengths; jti/jfi: 1 / J; (ikr, iki) = ik and
+// (invr, invi) = 1 / (ik). The moments already carry FP32 rounding; the combination has no
+// cancellation beyond the factor ~3 of (xi - o) weights.
+__device__ __forceinline__ void efie_block_f32(const C2f (&S)[3][3], const float (&et)[2][3], const float (&lt)[3],
+                                               float jti, const float (&ef)[2][3], const float (&lf)[3], float jfi,
+                                               float ikr, float iki, float invr, float invi, C2f (&V)[3][3]) {
+    float M[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) M[a][b] = et[a][0] * ef[b][0] + et[a][1] * ef[b][1] + et[a][2] * ef[b][2];
+    const float jj = jti * jfi;
+    const float s_re = 4.f * S[0][0].re * jj, s_im = 4.f * S[0][0].im * jj;
+    const float c_re = -(invr * s_re - invi * s_im), c_im = -(invr * s_im + invi * s_re);
+    constexpr float ox[3] = {0.f, 1.f, 0.f}, oy[3] = {1.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        // R[a][b'] = S[1 + a][b'] - o_i[a] S[0][b']
+        const float oi[2] = {ox[i], oy[i]};
+        C2f R[2][3];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                R[a][b].re = S[1 + a][b].re - oi[a] * S[0][b].re;
+                R[a][b].im = S[1 + a][b].im - oi[a] * S[0][b].im;
+            }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const float oj[2] = {ox[j], oy[j]};
+            float tr = 0.f, ti = 0.f;
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    tr += M[a][b] * (R[a][1 + b].re - oj[b] * R[a][0].re);
+                    ti += M[a][b] * (R[a][1 + b].im - oj[b] * R[a][0].im);
+                }
+            tr *= jj;
+            ti *= jj;
+            const float l = lt[i] * lf[j];
+            V[i][j].re = (-(ikr * tr - iki * ti) + c_re) * l;
+            V[i][j].im = (-(ikr * ti + iki * tr) + c_im) * l;
+        }
+    }
+}
+
+// Edge vectors, edge lengths (|v1 - v0|, |v2 - v0|, |v1 - v2|) and 1 / J of a geometry row, in FP32.
+__device__ __forceinline__ void efie_geom_f32(const double *g, float (&e)[2][3], float (&l)[3], float &jinv) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        e[0][d] = (float)g[3 + d];
+        e[1][d] = (float)g[6 + d];
+    }
+    l[0] = (float)sqrt(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]);
+    l[1] = (float)sqrt(g[6] * g[6] + g[7] * g[7] + g[8] * g[8]);
+    const double d0 = g[3] - g[6], d1 = g[4] - g[7], d2 = g[5] - g[8];
+    l[2] = (float)sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    jinv = (float)(1.0 / g[12]);
+}
+
 template <typename OutT, bool CPLX>
 __device__ __forceinline__ void store_direct(void *out, int64_t ld, int row, int col, C2d v, int accumulate) {
     OutT *o = reinterpret_cast<OutT *>(out);
@@ -515,9 +583,17 @@ __device__ __forceinline__ void store_direct(void *out, int64_t ld, int row, int
 // count. Each pair (e, f) adds its single-layer block S_ef to a.out at (e, f) only (the caller
 // adds the transpose afterwards: S_fe = S_ef^T), its double-layer block to a.out2 at (e, f)
 // and the double layer of the reversed pair (test normal) to a.out2 at (f, e).
+// waves per SIMD the launch bound asks for: the FP32 fused pass fits 3 (168 VGPRs, 52.9 KB LDS);
+// the symmetric pass and the mixed EFIE pass keep more state (2); the others leave it to the compiler
+template <int OP, bool MIXED, bool DUAL, bool SYM>
+constexpr int regular_waves() {
+    return !MIXED ? 1 : SYM ? 2 : DUAL ? 3 : OP == kMaxwellEFIE ? 2 : 1;
+}
+
 template <int OP, int NTS, int NSS, int QT, int TM, bool MIXED, bool KAHAN, bool TILE, int MAXU_T, int MAXU_S,
           typename OutT, bool DUAL = false, bool SYM = false>
-__global__ __launch_bounds__(kRegThreads, DUAL && MIXED ? (SYM ? 2 : 3) : 1) void regular_kernel(const RegularArgs a) {
+__global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>())) void regular_kernel(
+    const RegularArgs a) {
     static_assert(!SYM || (DUAL && TILE), "the symmetric pass is a tiled single + double layer pass");
     constexpr bool CPLX = OpTraits<OP>::is_complex;
     constexpr int NT = NTS, NS = NSS;
@@ -534,7 +610,9 @@ __global__ __launch_bounds__(kRegThreads, DUAL && MIXED ? (SYM ? 2 : 3) : 1) voi
     // test elements in LDS: one tile, or for SYM the whole run of tiles of the work item
     constexpr int TMC = SYM ? kSymRunElements : TM;
     constexpr int QC = QT > 0 ? QT : kMaxQ;
-    __shared__ TestTile<TMC, MIXED ? 1 : TMC, QC> T_;
+    // EFX: the MIXED EFIE block is formed in FP32 from staged edge data (efie_block_f32)
+    constexpr bool EFX = OP == kMaxwellEFIE && MIXED;
+    __shared__ TestTile<TMC, MIXED ? 1 : TMC, QC, EFX> T_;
     __shared__ AccT acc[TILE ? MAXU_T * MAXU_S * NC : 1];
     __shared__ AccT acc2[TILE && DUAL ? MAXU_T * MAXU_S * NC : 1];
     __shared__ AccT acc3[TILE && SYM ? MAXU_T * MAXU_S * NC : 1];
@@ -588,6 +666,7 @@ __global__ __launch_bounds__(kRegThreads, DUAL && MIXED ? (SYM ? 2 : 3) : 1) voi
                 T_.nx[el][0] = (float)(g[9] * nm);
                 T_.nx[el][1] = (float)(g[10] * nm);
                 T_.nx[el][2] = (float)(g[11] * nm);
+                if constexpr (EFX) efie_geom_f32(g, T_.ee[el], T_.el[el], T_.ej[el]);
             } else {
                 const int p = (task - n) - el * nq;
                 const double xi = a.rule.x[p], eta = a.rule.y[p];
@@ -619,6 +698,15 @@ __global__ __launch_bounds__(kRegThreads, DUAL && MIXED ? (SYM ? 2 : 3) : 1) voi
     int slot_f[NS];
     TrialF32<QT> F32;
     double yq[QT > 0 ? QT : 1][3], wq64[QT > 0 ? QT : 1], ny64[3] = {0, 0, 0};
+    float ef[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}}, lf[3] = {0.f, 0.f, 0.f}, jfi = 0.f;  // EFX
+    float ikr = 0.f, iki = 0.f, invr = 0.f, invi = 0.f;  // EFX: ik = -Im k + i Re k and 1 / (ik)
+    if constexpr (EFX) {
+        const double den = a.ki * a.ki + a.kr * a.kr;
+        ikr = (float)(-a.ki);
+        iki = (float)a.kr;
+        invr = (float)(-a.ki / den);
+        invi = (float)(-a.kr / den);
+    }
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
         mf[j] = 0.0;
@@ -645,6 +733,7 @@ __global__ __launch_bounds__(kRegThreads, DUAL && MIXED ? (SYM ? 2 : 3) : 1) voi
         ny64[0] = gf[9] * snm;
         ny64[1] = gf[10] * snm;
         ny64[2] = gf[11] * snm;
+        if constexpr (EFX) efie_geom_f32(gf, ef, lf, jfi);
         if constexpr (MIXED) {
             F32.ny[0] = (float)ny64[0];
             F32.ny[1] = (float)ny64[1];
@@ -672,18 +761,26 @@ __global__ __launch_bounds__(kRegThreads, DUAL && MIXED ? (SYM ? 2 : 3) : 1) voi
         }
     }
 
-    // owner lane state: the incidences (lane p, shape function j) of trial slot `tid` in this
-    // block, as stage offsets of row (j, i = 0) and column p; missing ones point at the zero column
-    int own_off[OWNER ? kMaxInc : 1];
+    // owner lane state: the incidences (lane p, shape function j) of trial slots tid (and
+    // tid + 256 when a block has more slots than lanes) in this block, as stage offsets of row
+    // (j, i = 0) and column p; missing ones point at the zero column
+    constexpr int OWNS = OWNER && MAXU_S > kRegThreads ? 2 : 1;
+    constexpr int KINC = OP == kMaxwellEFIE ? kRwgMaxInc : kMaxInc;
+    int own_off[OWNER ? OWNS : 1][OWNER ? KINC : 1];
     int own_nds = 0;
     if constexpr (OWNER) {
         own_nds = min(a.sb.ndofs[block_id], MAXU_S);
-        const int16_t *src = a.sb.inc + ((int64_t)block_id * a.sb.maxu + min(tid, a.sb.maxu - 1)) * a.sb.maxinc;
 #pragma unroll
-        for (int k = 0; k < kMaxInc; ++k) {
-            const int code = (tid < own_nds && k < a.sb.maxinc) ? (int)src[k] : -1;
-            const int p = code / 3, j = code - 3 * p;
-            own_off[k] = (code >= 0 && code < 3 * kRegThreads) ? (j * NT * SROW + p) * NC : kRegThreads * NC;
+        for (int o = 0; o < OWNS; ++o) {
+            const int sl = tid + o * kRegThreads;
+            const int16_t *src =
+                a.sb.inc + ((int64_t)block_id * a.sb.maxu + min(sl, a.sb.maxu - 1)) * a.sb.maxinc;
+#pragma unroll
+            for (int k = 0; k < KINC; ++k) {
+                const int code = (sl < own_nds && k < a.sb.maxinc) ? (int)src[k] : -1;
+                const int p = code / 3, j = code - 3 * p;
+                own_off[o][k] = (code >= 0 && code < 3 * kRegThreads) ? (j * NT * SROW + p) * NC : kRegThreads * NC;
+            }
         }
         if (tid < NT * NC) stage[(tid / NC) * SROW * NC + kRegThreads * NC + tid % NC] = 0.f;  // zero column
     }
@@ -743,7 +840,7 @@ __global__ __launch_bounds__(kRegThreads, DUAL && MIXED ? (SYM ? 2 : 3) : 1) voi
                 if constexpr (!MIXED) {
                     double tnm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
                     if constexpr (QT > 0)
-                        pair_fp64_regs<OP, NTS, NSS, QT, TMC, MIXED ? 1 : TMC, DUAL, SYM>(T_, te, gt, tnm, yq, wq64, ny64,
+                        pair_fp64_regs<OP, NTS, NSS, QT, DUAL, SYM>(T_, te, gt, tnm, yq, wq64, ny64,
                                                                                       a.rule, a.kr, a.ki, S, S2, S3);
                     else
                         pair_fp64_geom<OP, NTS, NSS, QT, DUAL, SYM>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2, S3);
@@ -754,15 +851,15 @@ __global__ __launch_bounds__(kRegThreads, DUAL && MIXED ? (SYM ? 2 : 3) : 1) voi
                     if (!skip) {
                         double d0 = cf0 - T_.c[te][0], d1 = cf1 - T_.c[te][1], d2 = cf2 - T_.c[te][2];
                         if (OpTraits<OP>::is_complex && a.ki != 0.0)
-                            pair_fp32<OP, NTS, NSS, QT, TMC, KAHAN, true, DUAL, 1, SYM>(T_, te, F32, d0, d1, d2, dist2, a.rule,
+                            pair_fp32<OP, NTS, NSS, QT, KAHAN, true, DUAL, SYM>(T_, te, F32, d0, d1, d2, dist2, a.rule,
                                                                                        a.kr, a.ki, Sf, Sf2, Sf3);
                         else
-                            pair_fp32<OP, NTS, NSS, QT, TMC, KAHAN, false, DUAL, 1, SYM>(T_, te, F32, d0, d1, d2, dist2,
+                            pair_fp32<OP, NTS, NSS, QT, KAHAN, false, DUAL, SYM>(T_, te, F32, d0, d1, d2, dist2,
                                                                                         a.rule, a.kr, a.ki, Sf, Sf2, Sf3);
                     }
                 }
             }
-            if constexpr (MIXED && (!OWNER || OP == kMaxwellEFIE)) {
+            if constexpr (MIXED && !OWNER) {
     #pragma unroll
                 for (int i = 0; i < NTS; ++i)
     #pragma unroll
@@ -783,9 +880,12 @@ __global__ __launch_bounds__(kRegThreads, DUAL && MIXED ? (SYM ? 2 : 3) : 1) voi
             if constexpr (OWNER) {
                 // stage this lane's block (zero for idle lanes, skipped pairs, zero multipliers), then
                 // the owner lane of every trial slot adds the entries of its incidences to its tile column
-                auto owner_pass = [&](const C2f(&Sm)[NTS][NSS], const C2d(&Sd)[NTS][NSS], AccT *tile) {
+                auto owner_pass = [&](const C2f(&Sm)[NTS][NSS], AccT *tile) {
                     C2f V[NTS][NSS];
-                    if constexpr (OP != kMaxwellEFIE) scalar_block_f32<NTS, NSS>(Sm, V);
+                    if constexpr (OP == kMaxwellEFIE)
+                        efie_block_f32(Sm, T_.ee[te], T_.el[te], T_.ej[te], ef, lf, jfi, ikr, iki, invr, invi, V);
+                    else
+                        scalar_block_f32<NTS, NSS>(Sm, V);
     #pragma unroll
                     for (int i = 0; i < NT; ++i)
     #pragma unroll
@@ -793,26 +893,23 @@ __global__ __launch_bounds__(kRegThreads, DUAL && MIXED ? (SYM ? 2 : 3) : 1) voi
                             float vr = 0.f, vi = 0.f;
                             const float m = (float)T_.mult[te][i] * (float)mf[j];
                             if (valid && !skip && m != 0.f) {
-                                if constexpr (OP == kMaxwellEFIE) {
-                                    C2d v = local_value<OP, NTS, NSS>(Sd, i, j, gt, gf, a.kr, a.ki);
-                                    vr = (float)v.re * m;
-                                    vi = (float)v.im * m;
-                                } else {
-                                    vr = V[i][j].re * m;
-                                    vi = V[i][j].im * m;
-                                }
+                                vr = V[i][j].re * m;
+                                vi = V[i][j].im * m;
                             }
                             float *dst = stage + ((j * NT + i) * SROW + tid) * NC;
                             if constexpr (CPLX) *reinterpret_cast<float2 *>(dst) = make_float2(vr, vi);
                             else dst[0] = vr;
                         }
                     __syncthreads();
-                    if (tid < own_nds) {
+    #pragma unroll
+                    for (int o = 0; o < OWNS; ++o) {
+                        const int sl = tid + o * kRegThreads;
+                        if (sl >= own_nds) break;
                         // all reads of a row are issued before any sum: one LDS round trip per row
                         auto gather = [&](int i, int k0, int k1, float &sr, float &si) {
     #pragma unroll
                             for (int k = k0; k < k1; ++k) {
-                                const float *src = stage + own_off[k] + i * SROW * NC;
+                                const float *src = stage + own_off[o][k] + i * SROW * NC;
                                 if constexpr (CPLX) {
                                     const float2 v = *reinterpret_cast<const float2 *>(src);
                                     sr += v.x;
@@ -822,24 +919,24 @@ __global__ __launch_bounds__(kRegThreads, DUAL && MIXED ? (SYM ? 2 : 3) : 1) voi
                                 }
                             }
                         };
-                        constexpr int KLO = kMaxInc < 6 ? kMaxInc : 6;
+                        constexpr int KLO = KINC < 6 ? KINC : 6;
                         const bool more = a.sb.maxinc > KLO;  // uniform: vertices of valence > 6
     #pragma unroll
                         for (int i = 0; i < NT; ++i) {
                             if (T_.mult[te][i] == 0.0) continue;
                             float sr = 0.f, si = 0.f;
                             gather(i, 0, KLO, sr, si);
-                            if (more) gather(i, KLO, kMaxInc, sr, si);
-                            AccT *dst = tile + NC * (T_.slot[te][i] * MAXU_S + tid);
+                            if (more) gather(i, KLO, KINC, sr, si);
+                            AccT *dst = tile + NC * (T_.slot[te][i] * MAXU_S + sl);
                             dst[0] += (AccT)sr;
                             if constexpr (CPLX) dst[1] += (AccT)si;
                         }
                     }
                     __syncthreads();
                 };
-                owner_pass(Sf, S, acc);
-                if constexpr (DUAL) owner_pass(Sf2, S2, acc2);
-                if constexpr (SYM) owner_pass(Sf3, S3, acc3);
+                owner_pass(Sf, acc);
+                if constexpr (DUAL) owner_pass(Sf2, acc2);
+                if constexpr (SYM) owner_pass(Sf3, acc3);
             } else if (valid) {
     #pragma unroll
                 for (int i = 0; i < NT; ++i) {
@@ -925,11 +1022,21 @@ template <int OP, int NTS, int NSS, int QT, bool MIXED, typename OutT, bool DUAL
 static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
     // DUAL keeps two LDS tiles and SYM three; their smaller test tiles keep the LDS footprint
     // at 3 workgroups per CU
-    constexpr int TM = SYM ? BEM_TILE_SYM_TEST_ELEMENTS : DUAL ? BEM_TILE_PAIR_TEST_ELEMENTS : BEM_TILE_TEST_ELEMENTS;
+    // mixed EFIE: trial blocks of up to BEM_TILE_RWG_MIXED_TRIAL_DOFS edges (a 256-element patch),
+    // owner lanes with two slots each, smaller test tiles to stay at 2 workgroups per CU
+    constexpr bool RWGM = OP == kMaxwellEFIE && MIXED;
+    constexpr int TM = SYM ? BEM_TILE_SYM_TEST_ELEMENTS
+                     : DUAL ? BEM_TILE_PAIR_TEST_ELEMENTS
+                     : RWGM ? BEM_TILE_RWG_MIXED_TEST_ELEMENTS
+                            : BEM_TILE_TEST_ELEMENTS;
     constexpr bool CPLX = OpTraits<OP>::is_complex;
     // LDS tile of distinct dofs: test slots x trial slots (BEM_TILE_TEST_DOFS x BEM_TILE_TRIAL_DOFS)
-    constexpr int MT = SYM ? BEM_TILE_SYM_TEST_DOFS : DUAL ? BEM_TILE_PAIR_TEST_DOFS : BEM_TILE_TEST_DOFS;
-    constexpr int MS = BEM_TILE_TRIAL_DOFS;
+    constexpr int MT = SYM ? BEM_TILE_SYM_TEST_DOFS
+                     : DUAL ? BEM_TILE_PAIR_TEST_DOFS
+                     : RWGM ? BEM_TILE_RWG_MIXED_TEST_DOFS
+                            : BEM_TILE_TEST_DOFS;
+    constexpr int MS = RWGM ? BEM_TILE_RWG_MIXED_TRIAL_DOFS : BEM_TILE_TRIAL_DOFS;
+    constexpr int KMAX = OP == kMaxwellEFIE ? kRwgMaxInc : kMaxInc;
     (void)CPLX;
     dim3 grid;
     if (tile) {
@@ -937,8 +1044,8 @@ static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
             set_error("dof blocks larger than the LDS tile (%d x %d > %d x %d)", a.tb.maxu, a.sb.maxu, MT, MS);
             return BEM_ERR_INVALID_ARGUMENT;
         }
-        if (MIXED && (a.sb.inc == nullptr || a.sb.maxinc < 1 || a.sb.maxinc > kMaxInc)) {
-            set_error("mixed precision needs trial incidence lists with 1..%d entries per dof", kMaxInc);
+        if (MIXED && (a.sb.inc == nullptr || a.sb.maxinc < 1 || a.sb.maxinc > KMAX)) {
+            set_error("mixed precision needs trial incidence lists with 1..%d entries per dof", KMAX);
             return BEM_ERR_INVALID_ARGUMENT;
         }
         grid = SYM ? dim3(a.nwork) : dim3(a.sb.nblocks, a.tb.nblocks);

@@ -116,6 +116,11 @@ typedef struct bem_dof_blocks {
 /* test tiles of bem_regular_assemble_sl_dl with blocks (two LDS tiles) */
 #define BEM_TILE_PAIR_TEST_ELEMENTS 8
 #define BEM_TILE_PAIR_TEST_DOFS 12
+/* blocks of the mixed-precision EFIE (RWG/SNC) in bem_regular_assemble_blocked: a 256-element
+ * trial block holds up to ~400 edges; test tiles are smaller so that the LDS tile stays small */
+#define BEM_TILE_RWG_MIXED_TEST_ELEMENTS 8
+#define BEM_TILE_RWG_MIXED_TEST_DOFS 16
+#define BEM_TILE_RWG_MIXED_TRIAL_DOFS 416
 /* test tiles of bem_regular_assemble_sl_dl_symmetric (three LDS tiles) */
 #define BEM_TILE_SYM_TEST_ELEMENTS 8
 #define BEM_TILE_SYM_TEST_DOFS 12
@@ -206,6 +211,13 @@ int bem_near_assemble_sl_dl(int op, const bem_space_t *test, const bem_space_t *
                             const int32_t *pairs, int grids_identical, const bem_quad_rule_t *rule, double k_re,
                             double k_im, double near_factor, void *out_single, void *out_double, int64_t ld,
                             int out_dtype, void *stream);
+/* The near pairs of bem_regular_assemble_sl_dl_symmetric (one space, MIXED precision): each pair
+ * (e, f) of the list stands for both orientations and must appear once (e.g. e < f). It adds the
+ * single-layer block at (e, f) and, transposed, at (f, e); the double layer at (e, f) and the
+ * double layer of the reversed pair at (f, e). No reference counterpart. */
+int bem_near_assemble_sl_dl_symmetric(int op, const bem_space_t *space, int n_pairs, const int32_t *pairs,
+                                      const bem_quad_rule_t *rule, double k_re, double k_im, double near_factor,
+                                      void *out_single, void *out_double, int64_t ld, int out_dtype, void *stream);
 
 /*
  * Singular (Sauter-Schwab) part, always FP64: for each of n_pairs pairs p,

@@ -109,3 +109,23 @@ def test_add_transpose(dtype, n, hip_available):
     hb.dense_add_transpose(a)
     torch.cuda.synchronize()
     assert torch.equal(a, ref)
+
+
+@pytest.mark.parametrize("boundary", [False, True])
+def test_symmetric_partial_support(boundary, default_parameters, hip_available):
+    """P1 on half of a perturbed sphere (dofs on the cut excluded or kept, zero multipliers
+    where a shape function has no dof): symmetric == ordered pass, FP64 vs the oracle."""
+    from bempp_amd.api import function_space
+    from oracle import oracle as O
+
+    grid = perturbed_sphere(3)
+    upper = np.nonzero(grid.centroids[:, 2] > 0.1)[0]
+    p1 = function_space(grid, "P", 1, support_elements=upper, include_boundary_dofs=boundary)
+    o1, o2 = fused(p1, "helmholtz", 3.0, "double", False, default_parameters)
+    s1, s2 = fused(p1, "helmholtz", 3.0, "double", True, default_parameters)
+    print(f"partial support (boundary dofs {boundary}): {rel(s1, o1):.2e} {rel(s2, o2):.2e}", flush=True)
+    assert rel(s1, o1) < 1e-13 and rel(s2, o2) < 1e-13
+    ref = O.assemble("helmholtz_double_layer", grid.vertices, grid.elements, p1, p1, 3.0)
+    assert np.linalg.norm(s2.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-12
+    m1, m2 = fused(p1, "helmholtz", 3.0, "single", True, default_parameters)
+    assert rel(m1, o1) < 1e-6 and rel(m2, o2) < 1e-6
