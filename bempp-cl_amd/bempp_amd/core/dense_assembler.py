@@ -129,9 +129,19 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     if grids_identical and singular:
         srule = singular_rule(dual_to_range.grid, parameters.quadrature.singular, dual_to_range.support, domain.support)
 
+    hip_opts_ = getattr(parameters, "hip", None)
+    upper = (
+        direct
+        and not accumulate
+        and use_upper_triangle(domain, dual_to_range, op)
+        and (getattr(hip_opts_, "symmetric", True) if hip_opts_ is not None else True)
+    )
     if events is not None:
         events["regular_start"].record()
-    if direct:
+    if upper:
+        hb.regular_assemble_symmetric(op, test, rule, _wavenumber(operator_descriptor), mode, out, near_factor=near,
+                                      kahan=kahan)
+    elif direct:
         hb.regular_assemble(
             op, test, trial, grids_identical, rule, _wavenumber(operator_descriptor), mode, out,
             accumulate=int(accumulate), near_factor=near, kahan=kahan,
@@ -183,6 +193,21 @@ def can_fuse_sl_dl(first, second):
         and dom1 == dom2
         and dual1 == dual2
     )
+
+
+_SYMMETRIC_SINGLE_LAYERS = ("laplace_single_layer", "helmholtz_single_layer")
+
+
+def use_upper_triangle(domain, dual_to_range, op):
+    """True if a single layer on one DP0 space can be assembled as its upper triangle plus a
+    mirror copy: the operator is symmetric and the dof of list position p is p."""
+    if op not in _SYMMETRIC_SINGLE_LAYERS or domain != dual_to_range or domain.number_of_shape_functions != 1:
+        return False
+    if not _is_localised_injective(domain):
+        return False
+    n = domain.number_of_support_elements
+    dofs = domain.local2global[domain.support_elements, 0]
+    return domain.global_dof_count == n and bool(_np.array_equal(dofs, _np.arange(n)))
 
 
 def use_symmetric_pass(domain, dual_to_range):

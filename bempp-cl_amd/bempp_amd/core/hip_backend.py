@@ -136,6 +136,8 @@ def lib():
         L.bem_dense_add_transpose.restype = i
         L.bem_dense_zero.argtypes = [P, i, i64, i64, P]
         L.bem_dense_zero.restype = i
+        L.bem_regular_assemble_symmetric.argtypes = [i, SP, QP, d, d, i, d, i, P, i64, i, P]
+        L.bem_regular_assemble_symmetric.restype = i
         L.bem_near_assemble_sl_dl.argtypes = [i, SP, SP, i, P, i, QP, d, d, d, P, P, i64, i, P]
         L.bem_near_assemble_sl_dl.restype = i
         L.bem_near_assemble_sl_dl_symmetric.argtypes = [i, SP, i, P, QP, d, d, d, P, P, i64, i, P]
@@ -452,6 +454,22 @@ def regular_assemble_sl_dl_symmetric(op, sym, rule, wavenumber, precision, out_s
             ctypes.byref(sym.trial.struct), int(sym.n_work), ptr(sym.work), n, ctypes.byref(rule), k.real, k.imag,
             precision, float(near_factor), int(bool(kahan)), ptr(out_single), ptr(out_double), n,
             _dtype_code(out_single), stream_handle(),
+        )
+    )
+
+
+def regular_assemble_symmetric(op, space, rule, wavenumber, precision, out, near_factor=DEFAULT_NEAR_FIELD_FACTOR,
+                               kahan=False):
+    """Regular part of a symmetric single layer on one DP0 space (dof = list position): the upper
+    triangle is evaluated, the lower one copied. `out` (n x n, contiguous) is overwritten."""
+    k = complex(wavenumber)
+    n = out.shape[0]
+    if out.dim() != 2 or out.shape[1] != n or not out.is_contiguous():
+        raise ValueError("expected a contiguous square output")
+    check(
+        lib().bem_regular_assemble_symmetric(
+            OPERATORS[op], ctypes.byref(space.struct), ctypes.byref(rule), k.real, k.imag, precision,
+            float(near_factor), int(bool(kahan)), ptr(out), n, _dtype_code(out), stream_handle(),
         )
     )
 

@@ -64,7 +64,8 @@ int check_space(const bem_space_t *s, const char *which);
 int check_launch(const char *what);
 
 // Square n x n matrix (leading dimension ld, dtype BEM_F32/F64/C64/C128) on stream s:
-// what = 0 sets it to zero, what = 1 replaces A by A + A^T (matvec.hip).
+// what = 0 sets it to zero, what = 1 replaces A by A + A^T, what = 2 copies the upper triangle
+// into the lower one and zeroes the diagonal (matvec.hip).
 int square_matrix_op(int what, void *A, int dtype, int64_t n, int64_t ld, hipStream_t s);
 
 }  // namespace bem

@@ -166,6 +166,27 @@ __global__ __launch_bounds__(256) void add_transpose_kernel(E *A, int64_t n, int
     }
 }
 
+// Lower triangle from the upper one, zero diagonal: A[i][j] = A[j][i] for i > j, A[i][i] = 0. The
+// workgroup of destination tile (bi, bj), bi >= bj, reads source tile (bj, bi) into LDS first.
+template <typename E>
+__global__ __launch_bounds__(256) void mirror_upper_kernel(E *A, int64_t n, int64_t ld) {
+    const int64_t bi = blockIdx.y, bj = blockIdx.x;
+    if (bi < bj) return;  // uniform per workgroup, before the barrier
+    __shared__ E X[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int64_t r0 = bi * 32, c0 = bj * 32;
+    for (int r = ty; r < 32; r += 8)
+        if (c0 + r < n && r0 + tx < n) X[r][tx] = A[(c0 + r) * ld + r0 + tx];
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t i = r0 + r, j = c0 + tx;  // A[i][j] = A[j][i] = X[tx][r]
+        if (i < n && j < n) {
+            if (i > j) A[i * ld + j] = X[tx][r];
+            else if (i == j) A[i * ld + j] = E{};
+        }
+    }
+}
+
 template <typename E>
 __global__ __launch_bounds__(256) void zero_square_kernel(E *A, int64_t n, int64_t ld) {
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -182,6 +203,10 @@ static int square_launch(int what, void *A, int64_t n, int64_t ld, hipStream_t s
         return check_launch("zero_square_kernel");
     }
     const unsigned nb = (unsigned)((n + 31) / 32);
+    if (what == 2) {
+        hipLaunchKernelGGL((mirror_upper_kernel<E>), dim3(nb, nb), dim3(256), 0, s, (E *)A, n, ld);
+        return check_launch("mirror_upper_kernel");
+    }
     hipLaunchKernelGGL((add_transpose_kernel<E>), dim3(nb, nb), dim3(256), 0, s, (E *)A, n, ld);
     return check_launch("add_transpose_kernel");
 }

@@ -8,7 +8,7 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
                          const bem_dof_blocks_t *sblocks, int grids_identical, const bem_quad_rule_t *rule,
                          double k_re, double k_im, int precision, double mixed_near_factor, int kahan, void *out,
                          void *out2, int64_t ld, int out_dtype, int accumulate, void *stream,
-                         const int32_t *work = nullptr, int nwork = 0) {
+                         const int32_t *work = nullptr, int nwork = 0, bool upper = false) {
     clear_error();
     if (!test || !trial || !rule || !out) {
         set_error("null argument");
@@ -65,6 +65,12 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
                          sblocks->n_blocks, sblocks->max_dofs, sblocks->max_incidence};
     }
     hipStream_t s = (hipStream_t)stream;
+    if (upper) {
+        if (op == BEM_LAPLACE_SINGLE_LAYER) return regular_launch_upper_0(a, precision, kahan, out_dtype, s);
+        if (op == BEM_HELMHOLTZ_SINGLE_LAYER) return regular_launch_upper_3(a, precision, kahan, out_dtype, s);
+        set_error("the symmetric DP0 pass needs a single-layer operator, got %d", op);
+        return BEM_ERR_UNSUPPORTED;
+    }
     if (work) {
         if (op == BEM_LAPLACE_SINGLE_LAYER) return regular_launch_sym_0(a, precision, kahan, out_dtype, s);
         if (op == BEM_HELMHOLTZ_SINGLE_LAYER) return regular_launch_sym_3(a, precision, kahan, out_dtype, s);
@@ -162,4 +168,27 @@ extern "C" int bem_regular_assemble_sl_dl_symmetric(int op, const bem_space_t *s
         if (rc) return rc;
     }
     return square_matrix_op(1, out_single, out_dtype, n_dofs, ld, s);
+}
+
+extern "C" int bem_regular_assemble_symmetric(int op, const bem_space_t *space, const bem_quad_rule_t *rule,
+                                              double k_re, double k_im, int precision, double mixed_near_factor,
+                                              int kahan, void *out, int64_t ld, int out_dtype, void *stream) {
+    clear_error();
+    if (!space || !out) {
+        set_error("null argument");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (space->shapeset != BEM_SHAPE_P0) {
+        set_error("the symmetric DP0 pass needs a piecewise constant space");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (ld < space->n_elements) {
+        set_error("leading dimension smaller than the number of dofs");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    // upper triangle (overwritten), then its mirror image below a zero diagonal
+    int rc = regular_entry(op, space, space, nullptr, nullptr, 1, rule, k_re, k_im, precision, mixed_near_factor,
+                           kahan, out, nullptr, ld, out_dtype, 0, stream, nullptr, 0, true);
+    if (rc) return rc;
+    return square_matrix_op(2, out, out_dtype, space->n_elements, ld, (hipStream_t)stream);
 }

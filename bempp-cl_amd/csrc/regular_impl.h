@@ -590,11 +590,15 @@ constexpr int regular_waves() {
     return !MIXED ? 1 : SYM ? 2 : DUAL ? 3 : OP == kMaxwellEFIE ? 2 : 1;
 }
 
+// UPPER (direct DP0 stores, a symmetric operator on one space): only the pairs whose trial element
+// comes later in the list than the test element are evaluated and stored; the caller mirrors the
+// upper triangle into the lower one (bem_regular_assemble_symmetric).
 template <int OP, int NTS, int NSS, int QT, int TM, bool MIXED, bool KAHAN, bool TILE, int MAXU_T, int MAXU_S,
-          typename OutT, bool DUAL = false, bool SYM = false>
+          typename OutT, bool DUAL = false, bool SYM = false, bool UPPER = false>
 __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>())) void regular_kernel(
     const RegularArgs a) {
     static_assert(!SYM || (DUAL && TILE), "the symmetric pass is a tiled single + double layer pass");
+    static_assert(!UPPER || (!TILE && !DUAL && NTS == 1 && NSS == 1), "UPPER stores DP0 blocks directly");
     constexpr bool CPLX = OpTraits<OP>::is_complex;
     constexpr int NT = NTS, NS = NSS;
     constexpr int NC = CPLX ? 2 : 1;
@@ -644,6 +648,11 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
     s1 = min(s1, s0 + kRegThreads);
     const int jf = s0 + tid;
     const bool valid = jf < s1;
+    // UPPER: a trial block that ends at or before the test tile holds no pair of the upper
+    // triangle (uniform per workgroup, before any barrier)
+    if constexpr (UPPER) {
+        if (s1 <= (int)blockIdx.y * TM + 1) return;
+    }
 
     // stage the test elements at list positions [pos0, pos0 + n) into T_[0, n): all lanes share
     // the element tasks and the (element, quadrature point) tasks
@@ -820,6 +829,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                       v2 == fv1 || v2 == fv2;
             }
             if constexpr (SYM) adj = adj || (diag && jf <= t0 + t);  // the pair belongs to the other orientation
+            if constexpr (UPPER) adj = adj || jf <= t0 + t;            // lower triangle: mirrored afterwards
             C2d S[NTS][NSS], S2[NTS][NSS], S3[NTS][NSS];     // FP64 moments (FP64 mode; MIXED paths that need the FP64 epilogue)
             C2f Sf[NTS][NSS], Sf2[NTS][NSS], Sf3[NTS][NSS];  // FP32 moments (MIXED)
     #pragma unroll
@@ -1018,7 +1028,8 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
 }
 
 // ------------------------------------------------------------------ dispatch
-template <int OP, int NTS, int NSS, int QT, bool MIXED, typename OutT, bool DUAL = false, bool SYM = false>
+template <int OP, int NTS, int NSS, int QT, bool MIXED, typename OutT, bool DUAL = false, bool SYM = false,
+          bool UPPER = false>
 static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
     // DUAL keeps two LDS tiles and SYM three; their smaller test tiles keep the LDS footprint
     // at 3 workgroups per CU
@@ -1054,6 +1065,10 @@ static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
             set_error("the symmetric pass needs dof blocks");
             return BEM_ERR_INVALID_ARGUMENT;
         }
+        if (UPPER && a.test.n != a.trial.n) {
+            set_error("the symmetric DP0 pass needs one space on both sides");
+            return BEM_ERR_INVALID_ARGUMENT;
+        }
         grid = dim3((a.trial.n + kRegThreads - 1) / kRegThreads, (a.test.n + TM - 1) / TM);
     }
     if (grid.x == 0 || grid.y == 0) return BEM_OK;
@@ -1063,9 +1078,13 @@ static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
     }
 #define BEM_LAUNCH(K, T)                                                                                              \
     hipLaunchKernelGGL((regular_kernel<OP, NTS, NSS, QT, TM, MIXED, K, T, (T ? MT : 1), (T ? MS : 1), OutT, DUAL, \
-                                       SYM && T>),                                                                    \
+                                       SYM && T, UPPER && !T>),                                                       \
                        grid, dim3(kRegThreads), 0, stream, a)
     if (tile) {
+        if constexpr (UPPER) {
+            set_error("the symmetric DP0 pass stores directly (no dof blocks)");
+            return BEM_ERR_INVALID_ARGUMENT;
+        }
         if (MIXED && kahan) BEM_LAUNCH(true, true);
         else BEM_LAUNCH(false, true);
     } else if constexpr (!SYM) {
@@ -1076,16 +1095,22 @@ static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
     return check_launch("regular_kernel");
 }
 
-template <int OP, int NTS, int NSS, bool MIXED, typename OutT, bool DUAL = false, bool SYM = false>
+template <int OP, int NTS, int NSS, bool MIXED, typename OutT, bool DUAL = false, bool SYM = false, bool UPPER = false>
 static int launch2(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
-    if (a.rule.n == 6) return launch3<OP, NTS, NSS, 6, MIXED, OutT, DUAL, SYM>(a, kahan, tile, stream);
-    return launch3<OP, NTS, NSS, 0, MIXED, OutT, DUAL, SYM>(a, kahan, tile, stream);
+    if (a.rule.n == 6) return launch3<OP, NTS, NSS, 6, MIXED, OutT, DUAL, SYM, UPPER>(a, kahan, tile, stream);
+    return launch3<OP, NTS, NSS, 0, MIXED, OutT, DUAL, SYM, UPPER>(a, kahan, tile, stream);
 }
 
-template <int OP, bool MIXED, typename OutT, bool DUAL = false, bool SYM = false>
+template <int OP, bool MIXED, typename OutT, bool DUAL = false, bool SYM = false, bool UPPER = false>
 static int launch1(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
     if constexpr (OP == kMaxwellEFIE) {
         return launch2<OP, 3, 3, MIXED, OutT>(a, kahan, tile, stream);
+    } else if constexpr (UPPER) {
+        if (nshape_of(a.test.shape) != 1 || nshape_of(a.trial.shape) != 1) {
+            set_error("the symmetric DP0 pass needs piecewise constant spaces");
+            return BEM_ERR_INVALID_ARGUMENT;
+        }
+        return launch2<OP, 1, 1, MIXED, OutT, false, false, true>(a, kahan, tile, stream);
     } else if constexpr (SYM) {
         // shared dofs (the tiled path) on one space: P1
         if (nshape_of(a.test.shape) != 3 || nshape_of(a.trial.shape) != 3) {
@@ -1102,24 +1127,24 @@ static int launch1(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
     }
 }
 
-template <int OP, bool DUAL = false, bool SYM = false>
+template <int OP, bool DUAL = false, bool SYM = false, bool UPPER = false>
 static int launch_op(const RegularArgs &a, int precision, int kahan, bool tile, int out_dtype, hipStream_t stream) {
     constexpr bool CPLX = OpTraits<OP>::is_complex;
     const bool mixed = precision == BEM_PRECISION_MIXED;
     if (CPLX) {
         if (out_dtype == BEM_C128)
-            return mixed ? launch1<OP, true, double, DUAL, SYM>(a, kahan, tile, stream)
-                         : launch1<OP, false, double, DUAL, SYM>(a, kahan, tile, stream);
+            return mixed ? launch1<OP, true, double, DUAL, SYM, UPPER>(a, kahan, tile, stream)
+                         : launch1<OP, false, double, DUAL, SYM, UPPER>(a, kahan, tile, stream);
         if (out_dtype == BEM_C64)
-            return mixed ? launch1<OP, true, float, DUAL, SYM>(a, kahan, tile, stream)
-                         : launch1<OP, false, float, DUAL, SYM>(a, kahan, tile, stream);
+            return mixed ? launch1<OP, true, float, DUAL, SYM, UPPER>(a, kahan, tile, stream)
+                         : launch1<OP, false, float, DUAL, SYM, UPPER>(a, kahan, tile, stream);
     } else {
         if (out_dtype == BEM_F64)
-            return mixed ? launch1<OP, true, double, DUAL, SYM>(a, kahan, tile, stream)
-                         : launch1<OP, false, double, DUAL, SYM>(a, kahan, tile, stream);
+            return mixed ? launch1<OP, true, double, DUAL, SYM, UPPER>(a, kahan, tile, stream)
+                         : launch1<OP, false, double, DUAL, SYM, UPPER>(a, kahan, tile, stream);
         if (out_dtype == BEM_F32)
-            return mixed ? launch1<OP, true, float, DUAL, SYM>(a, kahan, tile, stream)
-                         : launch1<OP, false, float, DUAL, SYM>(a, kahan, tile, stream);
+            return mixed ? launch1<OP, true, float, DUAL, SYM, UPPER>(a, kahan, tile, stream)
+                         : launch1<OP, false, float, DUAL, SYM, UPPER>(a, kahan, tile, stream);
     }
     set_error("output dtype %d does not match operator %d", out_dtype, OP);
     return BEM_ERR_INVALID_ARGUMENT;
@@ -1143,5 +1168,8 @@ int regular_launch_dual_3(const RegularArgs &a, int precision, int kahan, bool t
 // symmetric single + double layer pass (regular_sym<k>.hip)
 int regular_launch_sym_0(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
 int regular_launch_sym_3(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
+// symmetric DP0 single layer, upper triangle (regular_upper.hip)
+int regular_launch_upper_0(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
+int regular_launch_upper_3(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
 
 }  // namespace bem

@@ -327,6 +327,19 @@ int bem_adjoint_potential_single_layer(const bem_space_t *space, const bem_quad_
 int bem_dense_matvec(int trans, const void *A, int a_dtype, int64_t rows, int64_t cols, int64_t ld, const void *x,
                      void *y, void *stream);
 
+/*
+ * Regular part of a symmetric single layer (BEM_LAPLACE_SINGLE_LAYER, BEM_HELMHOLTZ_SINGLE_LAYER)
+ * on one piecewise constant space whose dof of list position p is p (no reference counterpart:
+ * the reference computes every ordered pair, bempp_cl/core/numba_kernels.py:980-1074). Only the
+ * pairs with the trial element later in the list are evaluated; the upper triangle is then copied
+ * into the lower one. out (n x n, n = space->n_elements, leading dimension ld) is OVERWRITTEN,
+ * with zeros on the diagonal and at the pairs that share a vertex (and, in MIXED precision, at
+ * the near pairs): add bem_near_assemble and bem_singular_assemble afterwards.
+ */
+int bem_regular_assemble_symmetric(int op, const bem_space_t *space, const bem_quad_rule_t *rule, double k_re,
+                                   double k_im, int precision, double mixed_near_factor, int kahan, void *out,
+                                   int64_t ld, int out_dtype, void *stream);
+
 /* A <- A + A^T for a square n x n matrix (leading dimension ld, dtype BEM_F32/F64/C64/C128),
  * in place (completes the single layer of bem_regular_assemble_sl_dl_symmetric). */
 int bem_dense_add_transpose(void *A, int dtype, int64_t n, int64_t ld, void *stream);

@@ -129,3 +129,33 @@ def test_symmetric_partial_support(boundary, default_parameters, hip_available):
     assert np.linalg.norm(s2.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-12
     m1, m2 = fused(p1, "helmholtz", 3.0, "single", True, default_parameters)
     assert rel(m1, o1) < 1e-6 and rel(m2, o2) < 1e-6
+
+
+@pytest.mark.parametrize("fam,k", [("helmholtz", 3.0), ("helmholtz", 2.0 + 0.4j), ("laplace", None)])
+def test_dp0_single_layer_upper_triangle(fam, k, default_parameters, hip_available):
+    """DP0 single layer on one space: upper triangle + mirror copy == all ordered pairs."""
+    import bempp_amd.api.operators.boundary as B
+    from bempp_amd.api import function_space
+    from bempp_amd.core.dense_assembler import hip_dense_assemble, use_upper_triangle
+    from oracle import oracle as O
+
+    grid = perturbed_sphere(3)
+    dp0 = function_space(grid, "DP", 0)
+    assert use_upper_triangle(dp0, dp0, f"{fam}_single_layer")
+    mod = getattr(B, fam)
+    args = (dp0, dp0, dp0) if k is None else (dp0, dp0, dp0, k)
+    res = {}
+    for precision in ("double", "single"):
+        d = mod.single_layer(*args, parameters=default_parameters, precision=precision).descriptor
+        for sym in (False, True):
+            default_parameters.hip.symmetric = sym
+            try:
+                res[precision, sym] = hip_dense_assemble(dp0, dp0, default_parameters, d)
+            finally:
+                default_parameters.hip.symmetric = True
+    print(f"{fam} k={k}: upper vs ordered FP64 {rel(res['double', True], res['double', False]):.2e}", flush=True)
+    assert rel(res["double", True], res["double", False]) < 1e-13
+    assert rel(res["single", True], res["double", False]) < 1e-6
+    ref = O.assemble(f"{fam}_single_layer", grid.vertices, grid.elements, dp0, dp0, 0.0 if k is None else k)
+    assert np.linalg.norm(res["double", True].cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-12
+

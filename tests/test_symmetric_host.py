@@ -56,3 +56,17 @@ def test_symmetric_pass_predicate():
     assert not use_symmetric_pass(p1, function_space(g, "DP", 1))
     assert not use_symmetric_pass(function_space(g, "DP", 0), function_space(g, "DP", 0))
     assert not use_symmetric_pass(p1, function_space(icosphere(2), "P", 1))
+
+
+def test_upper_triangle_predicate():
+    from bempp_amd.api import function_space
+    from bempp_amd.core.dense_assembler import use_upper_triangle
+
+    from bempp_amd.api.grid.shapes import icosphere
+
+    g = icosphere(1)
+    dp0, p1 = function_space(g, "DP", 0), function_space(g, "P", 1)
+    assert use_upper_triangle(dp0, dp0, "helmholtz_single_layer")
+    assert not use_upper_triangle(dp0, dp0, "helmholtz_double_layer")
+    assert not use_upper_triangle(p1, p1, "laplace_single_layer")
+    assert not use_upper_triangle(dp0, function_space(g, "DP", 0, support_elements=np.arange(10)), "laplace_single_layer")
