@@ -11,12 +11,14 @@
 //   dV_ef/dJ_f = sum_pq w_p J_e w_q G,   dJ/dv1 = e2 x n, dJ/dv2 = n x e1, dJ/dv0 = -(dJ/dv1 + dJ/dv2).
 // The regular part uses the symmetry V_ef = V_fe of the single layer on one space: the
 // derivative with respect to the test element's vertices equals the trial-side derivative of
-// the transposed pair, so one lane per trial element f accumulates, over all test elements e,
-// the trial-side derivative with the weight conj(x_e) y_f + conj(x_f) y_e, and no cross-lane
-// reductions are needed. Singular pairs (one wavefront per pair, FP64) differentiate both
-// elements with the weight conj(x_e) y_f, since their quadrature is not symmetric.
-// MIXED mode: far pairs in FP32 with the reference-distance split (far_split.h), near pairs
-// (the ones the mixed forward kernels compute in FP64) in FP64; FP64 mode: all in FP64.
+// the transposed pair, so both orientations carry the weight conj(x_e) y_f + conj(x_f) y_e.
+// FP64 mode: one lane per trial element f accumulates, over all test elements e, the
+// trial-side derivative, and no cross-lane reductions are needed. MIXED mode: far pairs in
+// FP32 with the reference-distance split (far_split.h), each unordered pair once, both sides'
+// derivatives from one evaluation (adjoint_far_kernel<TM, true>); near pairs (the ones the
+// mixed forward kernels compute in FP64) in FP64, all ordered pairs. Singular pairs (one
+// wavefront per pair, FP64) differentiate both elements with the weight conj(x_e) y_f, since
+// their quadrature is not symmetric.
 //
 // Potential. For u_p = sum_e m_e c_e sum_q w_q J_e G(|x_p - y_q|) (bempp_cl/core/numba_kernels.py:
 // 1788-1849) the potential adjoint adds (P^H g)_e to grad_c and the vertex derivative of
@@ -91,13 +93,29 @@ struct TileF32 {
     float xr[TM], xi[TM], yr[TM], yi[TM];  // m_e x_e, m_e y_e
 };
 
-template <int TM>
-__global__ __launch_bounds__(kThreads) void adjoint_far_kernel(const Args a) {
+// UPPER: every unordered pair once. The lane of trial element f takes only the test elements
+// earlier in the list; besides f's trial-side derivative it forms the test-side derivative of e
+// from the same point pairs (d/dx_p = -d/dy_q of G(|y_q - x_p|), the Jacobian term J_f / J_e
+// times the pair sum). Since phi_0 = 1 - xi - eta, e's vertex gradient needs only three moments
+// of the point derivatives P_p = sum_q d/dy_q: sum_p P_p (= the pair's sum over q), sum_p xi_p P_p
+// and sum_p eta_p P_p. Those 9 values and the pair sum are summed over the wave (DPP), over the
+// workgroup in LDS, and folded into e's vertex gradient once per test tile.
+template <int TM, bool UPPER = false>
+__global__ __launch_bounds__(kThreads, UPPER ? 3 : 1) void adjoint_far_kernel(const Args a) {
     __shared__ TileF32<TM> T_;
+    __shared__ float tB[UPPER ? TM : 1][3][3];  // UPPER: J_f * (sum_p, sum_p xi_p, sum_p eta_p) P_p per tile element
+    __shared__ float tS[UPPER ? TM : 1];        // UPPER: J_f * (pair sum) per tile element
     const int tid = threadIdx.x;
     const int jf = blockIdx.x * kThreads + tid;
     const bool valid = jf < a.sp.n;
-    const int e0 = blockIdx.y * a.range, e1 = min(e0 + a.range, a.sp.n);
+    const int e0 = blockIdx.y * a.range;
+    int e1 = min(e0 + a.range, a.sp.n);
+    if constexpr (UPPER) {
+        // test positions must lie below the largest trial position of the block (uniform, before any barrier)
+        const int last = min((int)(blockIdx.x * kThreads + kThreads), a.sp.n) - 1;
+        if (e0 >= last) return;
+        e1 = min(e1, last);
+    }
     const float kf = (float)a.k;
     const float krev = (float)(a.k * kInv2Pi);
 
@@ -132,6 +150,8 @@ __global__ __launch_bounds__(kThreads) void adjoint_far_kernel(const Args a) {
 #pragma unroll
     for (int q = 0; q < kQ; ++q) A[q][0] = A[q][1] = A[q][2] = 0.0;
 
+    const float jf32 = (float)gf[12];  // UPPER: J_f (test-side weights)
+    const int lane = tid & 63;
     for (int t0 = e0; t0 < e1; t0 += TM) {
         const int ntile = min(TM, e1 - t0);
         __syncthreads();
@@ -156,6 +176,12 @@ __global__ __launch_bounds__(kThreads) void adjoint_far_kernel(const Args a) {
             T_.yr[tid] = (float)(m * a.y[2 * dof]);
             T_.yi[tid] = (float)(m * a.y[2 * dof + 1]);
         }
+        if constexpr (UPPER) {
+            for (int i = tid; i < TM * 10; i += kThreads) {
+                if (i < TM * 9) (&tB[0][0][0])[i] = 0.f;
+                else tS[i - TM * 9] = 0.f;
+            }
+        }
         __syncthreads();
         for (int t = 0; t < ntile; ++t) {
             bool skip = !valid;
@@ -166,15 +192,28 @@ __global__ __launch_bounds__(kThreads) void adjoint_far_kernel(const Args a) {
                 double dist2;
                 if (!skip) skip = pair_is_near(T_.c[t], cf, T_.diam[t], diam_f, a.near_factor, &dist2);
             }
-            if (skip) continue;
+            if constexpr (UPPER) {
+                skip = skip || t0 + t >= jf;
+                // the test-side sums are reduced over the wave, so a lane that skips stays converged
+                // and computes a harmless far pair with zero weight, unless the whole wave skips
+                if (__all(skip)) continue;
+            } else {
+                if (skip) continue;
+            }
             // W = conj(x_e) y_f + conj(x_f) y_e
-            const float Wr = T_.xr[t] * yfr + T_.xi[t] * yfi + xfr * T_.yr[t] + xfi * T_.yi[t];
-            const float Wi = T_.xr[t] * yfi - T_.xi[t] * yfr + xfr * T_.yi[t] - xfi * T_.yr[t];
-            const FarPair P = far_pair_setup(T_.c[t], cf);
+            float Wr = T_.xr[t] * yfr + T_.xi[t] * yfi + xfr * T_.yr[t] + xfi * T_.yi[t];
+            float Wi = T_.xr[t] * yfi - T_.xi[t] * yfr + xfr * T_.yi[t] - xfi * T_.yr[t];
+            double cpair[3] = {cf[0], cf[1], cf[2]};
+            if (UPPER && skip) {
+                Wr = Wi = 0.f;
+                cpair[0] = T_.c[t][0] + 1.0e3 * (1.0 + T_.diam[t]);  // far from every point of e
+            }
+            const FarPair P = far_pair_setup(T_.c[t], cpair);
             float cs, sn;
             phase_rev_f32(a.k, (double)P.r0 * kInv2Pi, cs, sn);
             const float Er = Wr * cs - Wi * sn, Ei = Wr * sn + Wi * cs;  // W e^{i k r0}
             float Aq[kQ][3], sj = 0.f;
+            float M1[3] = {0.f, 0.f, 0.f}, M2[3] = {0.f, 0.f, 0.f};  // UPPER: sum_p xi_p P_p, sum_p eta_p P_p
 #pragma unroll
             for (int q = 0; q < kQ; ++q) Aq[q][0] = Aq[q][1] = Aq[q][2] = 0.f;
 #pragma unroll 1
@@ -183,6 +222,7 @@ __global__ __launch_bounds__(kThreads) void adjoint_far_kernel(const Args a) {
                 const float Ap = P.resid - 2.f * (P.fd0 * T_.a[t][p][0] + P.fd1 * T_.a[t][p][1] + P.fd2 * T_.a[t][p][2]) +
                                  T_.aa[t][p];
                 const float wp = T_.wp[t][p];
+                float P0 = 0.f, P1 = 0.f, P2 = 0.f;  // UPPER: P_p = sum_q tq (y_q - x_p)
 #pragma unroll
                 for (int q = 0; q < kQ; ++q) {
                     const float tt = Ap + bb[q] + 2.f * (w0 * b[q][0] + w1 * b[q][1] + w2 * b[q][2]);
@@ -195,15 +235,70 @@ __global__ __launch_bounds__(kThreads) void adjoint_far_kernel(const Args a) {
                     const float amp = rinv * wq[q] * wp;
                     sj += amp * Zr;
                     const float tq = amp * (-Zr - kf * r * Zi) * rinv * rinv;
-                    Aq[q][0] += tq * (w0 + b[q][0]);
-                    Aq[q][1] += tq * (w1 + b[q][1]);
-                    Aq[q][2] += tq * (w2 + b[q][2]);
+                    const float d0 = tq * (w0 + b[q][0]), d1 = tq * (w1 + b[q][1]), d2 = tq * (w2 + b[q][2]);
+                    Aq[q][0] += d0;
+                    Aq[q][1] += d1;
+                    Aq[q][2] += d2;
+                    if constexpr (UPPER) {
+                        P0 += d0;
+                        P1 += d1;
+                        P2 += d2;
+                    }
+                }
+                if constexpr (UPPER) {
+                    const float xi = (float)a.rule.x[p], eta = (float)a.rule.y[p];
+                    M1[0] += xi * P0;
+                    M1[1] += xi * P1;
+                    M1[2] += xi * P2;
+                    M2[0] += eta * P0;
+                    M2[1] += eta * P1;
+                    M2[2] += eta * P2;
                 }
             }
 #pragma unroll
             for (int q = 0; q < kQ; ++q)
                 for (int d = 0; d < 3; ++d) A[q][d] += (double)Aq[q][d];
             S += (double)sj;
+            if constexpr (UPPER) {
+                // test side: d/dx_p = -P_p J_f (tq lacks J_f); J_e S_e = J_f * (pair sum)
+                float M0[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    M0[d] = Aq[0][d];
+#pragma unroll
+                    for (int q = 1; q < kQ; ++q) M0[d] += Aq[q][d];
+                }
+                float R[10];
+                wave_total2_dpp(M0[0] * jf32, M0[1] * jf32, R[0], R[1]);
+                wave_total2_dpp(M0[2] * jf32, M1[0] * jf32, R[2], R[3]);
+                wave_total2_dpp(M1[1] * jf32, M1[2] * jf32, R[4], R[5]);
+                wave_total2_dpp(M2[0] * jf32, M2[1] * jf32, R[6], R[7]);
+                wave_total2_dpp(M2[2] * jf32, sj * jf32, R[8], R[9]);
+                if (lane == 0) {
+#pragma unroll
+                    for (int m = 0; m < 9; ++m) atomicAdd(&tB[t][m / 3][m % 3], R[m]);
+                    atomicAdd(&tS[t], R[9]);
+                }
+            }
+        }
+        if constexpr (UPPER) {
+            // fold the test side of this tile: G_e[j] = S_e dJ_e/dv_j - sum_p phi_j(xi_p) J_f P_p
+            __syncthreads();
+            if (tid < ntile) {
+                const int e = T_.elem[tid];
+                const double *g = a.sp.geom + (int64_t)e * BEM_GEOM_STRIDE;
+                double dJ[3][3], G[3][3];
+                jacobian_gradient(g, dJ);
+                const double Se = (double)tS[tid] / g[12];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double m0 = tB[tid][0][d], m1 = tB[tid][1][d], m2 = tB[tid][2][d];
+                    G[0][d] = Se * dJ[0][d] - (m0 - m1 - m2);
+                    G[1][d] = Se * dJ[1][d] - m1;
+                    G[2][d] = Se * dJ[2][d] - m2;
+                }
+                atomic_vertex_grad(a.grad, a.sp.vids, e, G);
+            }
         }
     }
     if (valid) {
@@ -577,7 +672,8 @@ extern "C" int bem_adjoint_single_layer(const bem_space_t *space, const bem_quad
         a.range = range;
         a.nty = (a.sp.n + range - 1) / range;
         const dim3 grid(ntx, a.nty);
-        if (mixed) hipLaunchKernelGGL((adj::adjoint_far_kernel<32>), grid, dim3(adj::kThreads), 0, s, a);
+        // mixed: each unordered pair once (both sides' derivatives from one evaluation)
+        if (mixed) hipLaunchKernelGGL((adj::adjoint_far_kernel<32, true>), grid, dim3(adj::kThreads), 0, s, a);
         else hipLaunchKernelGGL((adj::adjoint_fp64_kernel<32>), grid, dim3(adj::kThreads), 0, s, a);
         if ((rc = check_launch("adjoint_regular_kernel"))) return rc;
     }

@@ -26,6 +26,43 @@ __device__ __forceinline__ T wave_sum64(T v) {
     return v;
 }
 
+// Sum over the 64 lanes of a converged wave, on the VALU (DPP) instead of LDS permutes: row
+// shifts by 1, 2, 4, 8 leave each 16-lane row's sum in its last lane, two row broadcasts carry
+// rows 0 + 1 and rows 2 + 3 into lane 63, which is read back as a wave-uniform value.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_moved(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWS, 0xf, true));
+}
+
+__device__ __forceinline__ float wave_total_dpp(float v) {
+    v += dpp_moved<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_moved<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_moved<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_moved<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_moved<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_moved<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+
+// Two wave sums at once: permlane32_swap leaves a's upper half beside b's lower half, so one
+// add gives a summed over lanes i and i + 32 in lanes 0-31 and b likewise in lanes 32-63; row
+// shifts and one row broadcast then finish a in lane 31 and b in lane 63.
+__device__ __forceinline__ void wave_total2_dpp(float a, float b, float &sa, float &sb) {
+    const auto sw = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b),
+                                                     false, false);
+    // copy the two halves out before the bit casts: a bit cast applied to sw[1] directly reads
+    // element 0 with this compiler
+    const unsigned lo = sw[0], hi = sw[1];
+    float v = __builtin_bit_cast(float, lo) + __builtin_bit_cast(float, hi);
+    v += dpp_moved<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_moved<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_moved<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_moved<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_moved<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    sa = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 31));
+    sb = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+
 struct FarPair {
     float fd0, fd1, fd2;  // centroid offset d = c_f - c_e (FP32)
     float r0, r0sq, resid;
