@@ -1,4 +1,4 @@
-"""Helmholtz boundary operators (bempp_cl/api/operators/boundary/helmholtz.py:9-131).
+"""Helmholtz boundary operators (bempp_cl/api/operators/boundary/helmholtz.py:9-178).
 
 The options array is [Re k, Im k] (helmholtz.py:41). The reference redirects Re k == 0 to
 modified Helmholtz operators (helmholtz.py:22-32), which this library does not provide.
@@ -15,11 +15,11 @@ def _check_wavenumber(wavenumber):
 
 
 def _make(identifier, kernel, domain, range_, dual_to_range, wavenumber, parameters, assembler, device_interface,
-          precision):
+          precision, assembly_type="default_scalar"):
     _check_wavenumber(wavenumber)
     return _common.create_operator(
         identifier, domain, range_, dual_to_range, parameters, assembler,
-        [float(_np.real(wavenumber)), float(_np.imag(wavenumber))], kernel, "default_scalar",
+        [float(_np.real(wavenumber)), float(_np.imag(wavenumber))], kernel, assembly_type,
         device_interface, precision, True,
     )
 
@@ -43,6 +43,17 @@ def adjoint_double_layer(domain, range_, dual_to_range, wavenumber, parameters=N
     """Helmholtz adjoint double-layer boundary operator."""
     return _make("helmholtz_adjoint_double_layer_boundary", "helmholtz_adjoint_double_layer", domain, range_,
                  dual_to_range, wavenumber, parameters, assembler, device_interface, precision)
+
+
+def hypersingular(domain, range_, dual_to_range, wavenumber, parameters=None, assembler="default_nonlocal",
+                  device_interface=None, precision=None):
+    """Helmholtz hypersingular boundary operator (bempp_cl/api/operators/boundary/helmholtz.py:134-178)."""
+    if domain.shapeset.identifier != "p1_discontinuous":
+        raise ValueError("Domain shapeset must be of type 'p1_discontinuous'.")
+    if dual_to_range.shapeset.identifier != "p1_discontinuous":
+        raise ValueError("Dual to range shapeset must be of type 'p1_discontinuous'.")
+    return _make("helmholtz_hypersingular_boundary", "helmholtz_single_layer", domain, range_, dual_to_range,
+                 wavenumber, parameters, assembler, device_interface, precision, "helmholtz_hypersingular")
 
 
 def single_layer_frequency_sweep(domain, range_, dual_to_range, wavenumbers, parameters=None, device_interface=None,

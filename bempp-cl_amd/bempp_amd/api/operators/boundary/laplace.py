@@ -1,4 +1,4 @@
-"""Laplace boundary operators (bempp_cl/api/operators/boundary/laplace.py:6-81)."""
+"""Laplace boundary operators (bempp_cl/api/operators/boundary/laplace.py:6-113)."""
 
 from bempp_amd.api.operators.boundary import common as _common
 
@@ -27,4 +27,17 @@ def adjoint_double_layer(domain, range_, dual_to_range, parameters=None, assembl
     return _common.create_operator(
         "laplace_adjoint_double_layer_boundary", domain, range_, dual_to_range, parameters, assembler, [],
         "laplace_adjoint_double_layer", "default_scalar", device_interface, precision, False,
+    )
+
+
+def hypersingular(domain, range_, dual_to_range, parameters=None, assembler="default_nonlocal", device_interface=None,
+                  precision=None):
+    """Laplace hypersingular boundary operator (bempp_cl/api/operators/boundary/laplace.py:84-113)."""
+    if domain.shapeset.identifier != "p1_discontinuous":
+        raise ValueError("Domain shapeset must be of type 'p1_discontinuous'.")
+    if dual_to_range.shapeset.identifier != "p1_discontinuous":
+        raise ValueError("Dual to range shapeset must be of type 'p1_discontinuous'.")
+    return _common.create_operator(
+        "laplace_hypersingular_boundary", domain, range_, dual_to_range, parameters, assembler, [],
+        "laplace_single_layer", "laplace_hypersingular", device_interface, precision, False,
     )

@@ -58,8 +58,8 @@ class DenseAssembler(object):
 
 
 def _kernel_name(descriptor):
-    if descriptor.assembly_type == "maxwell_electric_field":
-        return "maxwell_electric_field"
+    if descriptor.assembly_type in ("maxwell_electric_field", "laplace_hypersingular", "helmholtz_hypersingular"):
+        return descriptor.assembly_type
     if descriptor.assembly_type != "default_scalar":
         raise KeyError(f"Unsupported assembly type {descriptor.assembly_type}")
     return descriptor.kernel_type

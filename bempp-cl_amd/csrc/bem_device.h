@@ -27,11 +27,25 @@ enum Op {
     kHelmholtzDL = BEM_HELMHOLTZ_DOUBLE_LAYER,
     kHelmholtzADL = BEM_HELMHOLTZ_ADJOINT_DOUBLE_LAYER,
     kMaxwellEFIE = BEM_MAXWELL_ELECTRIC_FIELD,
+    kLaplaceHS = BEM_LAPLACE_HYPERSINGULAR,
+    kHelmholtzHS = BEM_HELMHOLTZ_HYPERSINGULAR,
 };
+
+// hypersingular operators: moments of their single layer, combined with surface curls in the epilogue
+template <int OP>
+constexpr bool is_hypersingular() {
+    return OP == kLaplaceHS || OP == kHelmholtzHS;
+}
+
+// the operator whose Green's function the point-pair sums of OP evaluate
+template <int OP>
+constexpr int green_op() {
+    return OP == kLaplaceHS ? kLaplaceSL : OP == kHelmholtzHS ? kHelmholtzSL : OP;
+}
 
 template <int OP>
 struct OpTraits {
-    static constexpr bool is_complex = OP >= kHelmholtzSL;
+    static constexpr bool is_complex = OP >= kHelmholtzSL && OP != kLaplaceHS;
     static constexpr bool needs_trial_normal = OP == kLaplaceDL || OP == kHelmholtzDL;
     static constexpr bool needs_test_normal = OP == kLaplaceADL || OP == kHelmholtzADL;
 };
@@ -241,12 +255,42 @@ __device__ __forceinline__ bool pair_is_near(const double *ct, const double *cf,
 }
 
 // ------------------------------------------------------------------ epilogue
-// Local block value (i, j) from the moment block S, for scalar shapesets or EFIE.
+// Surface curl of the P1 shape function i of a flat triangle times its Jacobian J:
+// curl phi = n x grad phi gives curl phi_1 = e2 / J, curl phi_2 = -e1 / J and
+// curl phi_0 = (e1 - e2) / J (e1 = v1 - v0, e2 = v2 - v0), i.e. the coefficients of (e1, e2).
+__host__ __device__ constexpr double curl_coef(int i, int a) {
+    return i == 0 ? (a == 0 ? 1.0 : -1.0) : i == 1 ? (a == 0 ? 0.0 : 1.0) : (a == 0 ? -1.0 : 0.0);
+}
+
+// Local block value (i, j) from the moment block S, for scalar shapesets, EFIE and the
+// hypersingular operators. nmp: product of the two elements' normal multipliers (used by the
+// hypersingular operators, whose curls and normals both carry them).
 template <int OP, int NTS, int NSS>
 __device__ __forceinline__ C2d local_value(const C2d (&S)[NTS][NSS], int i, int j, const double *gt,
-                                                  const double *gf, double kr, double ki) {
+                                                  const double *gf, double kr, double ki, double nmp = 1.0) {
     C2d v;
-    if constexpr (OP == kMaxwellEFIE) {
+    if constexpr (is_hypersingular<OP>()) {
+        // curl_i . curl_j S00 - k^2 (n_t . n_f) phi_i psi_j-moment (bempp_cl/core/numba_kernels.py:1284-1299)
+        static_assert(NTS == 3 && NSS == 3, "hypersingular operators need P1 shape functions on both sides");
+        const C2d sl = local_value<kLaplaceSL, NTS, NSS>(S, i, j, gt, gf, kr, ki);
+        double c = 0.0;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const double m = gt[3 + 3 * a] * gf[3 + 3 * b] + gt[4 + 3 * a] * gf[4 + 3 * b] + gt[5 + 3 * a] * gf[5 + 3 * b];
+                c += curl_coef(i, a) * curl_coef(j, b) * m;
+            }
+        c *= nmp / (gt[12] * gf[12]);
+        v.re = c * S[0][0].re;
+        v.im = c * S[0][0].im;
+        if constexpr (OP == kHelmholtzHS) {
+            const double nn = nmp * (gt[9] * gf[9] + gt[10] * gf[10] + gt[11] * gf[11]);
+            const double k2r = (kr * kr - ki * ki) * nn, k2i = 2.0 * kr * ki * nn;
+            v.re -= k2r * sl.re - k2i * sl.im;
+            v.im -= k2r * sl.im + k2i * sl.re;
+        }
+    } else if constexpr (OP == kMaxwellEFIE) {
         // M = E_t^T E_f, E = [v1 - v0, v2 - v0]
         double M[2][2];
 #pragma unroll

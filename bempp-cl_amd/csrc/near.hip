@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void near_kernel(const NearArgs a) {
     double tnm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
     double snm = a.trial.nmult ? (double)a.trial.nmult[f] : 1.0;
     C2d S[NTS][NSS], S2[NTS][NSS], S3[NTS][NSS];
-    pair_fp64_geom<OP, NTS, NSS, QT, DUAL, SYM>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2, S3);
+    pair_fp64_geom<green_op<OP>(), NTS, NSS, QT, DUAL, SYM>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2, S3);
     // to: 0 adds block (i, j) at (e_i, f_j), 1 at (f_j, e_i), 2 at both
     auto scatter = [&](const C2d(&Sm)[NTS][NSS], auto op_tag, void *out, int to) {
         constexpr int OPX = decltype(op_tag)::value;
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void near_kernel(const NearArgs a) {
             for (int j = 0; j < NSS; ++j) {
                 double m = a.test.mult[(int64_t)e * NTS + i] * a.trial.mult[(int64_t)f * NSS + j];
                 if (m == 0.0) continue;
-                C2d v = local_value<OPX, NTS, NSS>(Sm, i, j, gt, gf, a.kr, a.ki);
+                C2d v = local_value<OPX, NTS, NSS>(Sm, i, j, gt, gf, a.kr, a.ki, tnm * snm);
                 const int64_t r = a.test.l2g[(int64_t)e * NTS + i], c = a.trial.l2g[(int64_t)f * NSS + j];
                 if (to != 1) add(r * a.ld + c, v, m);
                 if (to != 0) add(c * a.ld + r, v, m);
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void near_group_kernel(const NearArgs a) {
                     green_sl_dl_rev<CPLX, double, SM>(dx, dy, dz, r2, nx, ny, a.kr, a.ki, v[0][0], v[0][1], v[1][0],
                                                       v[1][1], v[2][0], v[2][1]);
                 else
-                    green2<OP, DUAL, double, SM>(dx, dy, dz, r2, nx, ny, a.kr, a.ki, v[0][0], v[0][1], v[1][0],
+                    green2<green_op<OP>(), DUAL, double, SM>(dx, dy, dz, r2, nx, ny, a.kr, a.ki, v[0][0], v[0][1], v[1][0],
                                                  v[1][1]);
                 const double wq = a.rule.w[q] * gf[12];
 #pragma unroll
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void near_group_kernel(const NearArgs a) {
         if (m == 0.0) continue;
         // compile-time block indices (a runtime index would put S in scratch)
         C2d v;
-        if (o == 0) v = local_value<OP, NTS, NSS>(S[0], i, j, gt, gf, a.kr, a.ki);
+        if (o == 0) v = local_value<OP, NTS, NSS>(S[0], i, j, gt, gf, a.kr, a.ki, tnm * snm);
         else if (o == 1) v = local_value<(DUAL ? OP + 1 : OP), NTS, NSS>(S[DUAL ? 1 : 0], i, j, gt, gf, a.kr, a.ki);
         else v = local_value<(DUAL ? OP + 1 : OP), NTS, NSS>(S[NOPS - 1], i, j, gt, gf, a.kr, a.ki);
         const int64_t r = a.test.l2g[(int64_t)e * NTS + i], c = a.trial.l2g[(int64_t)f * NSS + j];
@@ -215,6 +215,12 @@ static int near_launch3(const NearArgs &a, hipStream_t s) {
 template <int OP, typename OutT, bool DUAL, bool SYM = false>
 static int near_launch2(const NearArgs &a, hipStream_t s) {
     if constexpr (OP == kMaxwellEFIE) {
+        return near_launch3<OP, 3, 3, OutT, false>(a, s);
+    } else if constexpr (is_hypersingular<OP>()) {
+        if (nshape_of(a.test.shape) != 3 || nshape_of(a.trial.shape) != 3) {
+            set_error("hypersingular operators need P1 shape functions on both sides");
+            return BEM_ERR_INVALID_ARGUMENT;
+        }
         return near_launch3<OP, 3, 3, OutT, false>(a, s);
     } else if constexpr (SYM) {
         const int nt = nshape_of(a.test.shape);
@@ -293,6 +299,8 @@ static int near_entry(int op, const bem_space_t *test, const bem_space_t *trial,
     case BEM_HELMHOLTZ_DOUBLE_LAYER: return near_launch_op<kHelmholtzDL>(a, out_dtype, s);
     case BEM_HELMHOLTZ_ADJOINT_DOUBLE_LAYER: return near_launch_op<kHelmholtzADL>(a, out_dtype, s);
     case BEM_MAXWELL_ELECTRIC_FIELD: return near_launch_op<kMaxwellEFIE>(a, out_dtype, s);
+    case BEM_LAPLACE_HYPERSINGULAR: return near_launch_op<kLaplaceHS>(a, out_dtype, s);
+    case BEM_HELMHOLTZ_HYPERSINGULAR: return near_launch_op<kHelmholtzHS>(a, out_dtype, s);
     default: set_error("unsupported operator %d", op); return BEM_ERR_UNSUPPORTED;
     }
 }

@@ -91,6 +91,8 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
     case 4: return regular_launch_4(a, precision, kahan, tile, out_dtype, s);
     case 5: return regular_launch_5(a, precision, kahan, tile, out_dtype, s);
     case 6: return regular_launch_6(a, precision, kahan, tile, out_dtype, s);
+    case 7: return regular_launch_7(a, precision, kahan, tile, out_dtype, s);
+    case 8: return regular_launch_8(a, precision, kahan, tile, out_dtype, s);
     default: set_error("unsupported operator %d", op); return BEM_ERR_UNSUPPORTED;
     }
 }

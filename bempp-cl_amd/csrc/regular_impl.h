@@ -534,6 +534,44 @@ __device__ __forceinline__ void efie_block_f32(const C2f (&S)[3][3], const float
     }
 }
 
+// Hypersingular local block in FP32 from FP32 moments (the formula of local_value for
+// kLaplaceHS / kHelmholtzHS): V[i][j] = C_ij S00 - k^2 (n_t . n_f) SL_ij with the curl products
+//   C_ij = (nm_t / J_t) (nm_f / J_f) sum_ab c_i[a] c_j[b] (e_a^t . e_b^f),  c = curl_coef,
+// SL the P1 single-layer block of the same moments. et/ef: edge vectors; sjt/sjf: nm / J;
+// nt/nf: normals times nm; (k2r, k2i) = k^2 (zero for Laplace).
+template <bool CPLX>
+__device__ __forceinline__ void hs_block_f32(const C2f (&S)[3][3], const float (&et)[2][3], float sjt,
+                                             const float (&nt)[3], const float (&ef)[2][3], float sjf,
+                                             const float (&nf)[3], float k2r, float k2i, C2f (&V)[3][3]) {
+    float M[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) M[a][b] = et[a][0] * ef[b][0] + et[a][1] * ef[b][1] + et[a][2] * ef[b][2];
+    const float jj = sjt * sjf;
+    C2f SL[3][3];
+    scalar_block_f32<3, 3>(S, SL);
+    const float nn = nt[0] * nf[0] + nt[1] * nf[1] + nt[2] * nf[2];
+    const float ar = k2r * nn, ai = k2i * nn;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            float c = 0.f;
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) c += (float)(curl_coef(i, a) * curl_coef(j, b)) * M[a][b];
+            c *= jj;
+            V[i][j].re = c * S[0][0].re;
+            V[i][j].im = CPLX ? c * S[0][0].im : 0.f;
+            if constexpr (CPLX) {
+                V[i][j].re -= ar * SL[i][j].re - ai * SL[i][j].im;
+                V[i][j].im -= ar * SL[i][j].im + ai * SL[i][j].re;
+            }
+        }
+}
+
 // Edge vectors, edge lengths (|v1 - v0|, |v2 - v0|, |v1 - v2|) and 1 / J of a geometry row, in FP32.
 __device__ __forceinline__ void efie_geom_f32(const double *g, float (&e)[2][3], float (&l)[3], float &jinv) {
 #pragma unroll
@@ -603,6 +641,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
     constexpr int NT = NTS, NS = NSS;
     constexpr int NC = CPLX ? 2 : 1;
     constexpr int OP2 = OP + 1;  // the double layer of the family (DUAL only)
+    constexpr int GOP = green_op<OP>();  // the point-pair sums use this operator's Green's function
     // OWNER: the trial-side reduction of the tile runs without atomics. For every test
     // element each lane stores its local block to LDS; after a barrier the lane owning trial
     // slot s sums the block entries of the (element, j) incidences of s and updates its own
@@ -616,7 +655,9 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
     constexpr int QC = QT > 0 ? QT : kMaxQ;
     // EFX: the MIXED EFIE block is formed in FP32 from staged edge data (efie_block_f32)
     constexpr bool EFX = OP == kMaxwellEFIE && MIXED;
-    __shared__ TestTile<TMC, MIXED ? 1 : TMC, QC, EFX> T_;
+    // HSX: the MIXED hypersingular block is formed in FP32 from staged edge vectors and nm / J
+    constexpr bool HSX = is_hypersingular<OP>() && MIXED;
+    __shared__ TestTile<TMC, MIXED ? 1 : TMC, QC, EFX || HSX> T_;
     __shared__ AccT acc[TILE ? MAXU_T * MAXU_S * NC : 1];
     __shared__ AccT acc2[TILE && DUAL ? MAXU_T * MAXU_S * NC : 1];
     __shared__ AccT acc3[TILE && SYM ? MAXU_T * MAXU_S * NC : 1];
@@ -675,7 +716,8 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                 T_.nx[el][0] = (float)(g[9] * nm);
                 T_.nx[el][1] = (float)(g[10] * nm);
                 T_.nx[el][2] = (float)(g[11] * nm);
-                if constexpr (EFX) efie_geom_f32(g, T_.ee[el], T_.el[el], T_.ej[el]);
+                if constexpr (EFX || HSX) efie_geom_f32(g, T_.ee[el], T_.el[el], T_.ej[el]);
+                if constexpr (HSX) T_.ej[el] *= (float)nm;
             } else {
                 const int p = (task - n) - el * nq;
                 const double xi = a.rule.x[p], eta = a.rule.y[p];
@@ -709,6 +751,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
     double yq[QT > 0 ? QT : 1][3], wq64[QT > 0 ? QT : 1], ny64[3] = {0, 0, 0};
     float ef[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}}, lf[3] = {0.f, 0.f, 0.f}, jfi = 0.f;  // EFX
     float ikr = 0.f, iki = 0.f, invr = 0.f, invi = 0.f;  // EFX: ik = -Im k + i Re k and 1 / (ik)
+    const float k2r = (float)(a.kr * a.kr - a.ki * a.ki), k2i = (float)(2.0 * a.kr * a.ki);  // HSX: k^2
     if constexpr (EFX) {
         const double den = a.ki * a.ki + a.kr * a.kr;
         ikr = (float)(-a.ki);
@@ -742,7 +785,8 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
         ny64[0] = gf[9] * snm;
         ny64[1] = gf[10] * snm;
         ny64[2] = gf[11] * snm;
-        if constexpr (EFX) efie_geom_f32(gf, ef, lf, jfi);
+        if constexpr (EFX || HSX) efie_geom_f32(gf, ef, lf, jfi);
+        if constexpr (HSX) jfi *= (float)snm;
         if constexpr (MIXED) {
             F32.ny[0] = (float)ny64[0];
             F32.ny[1] = (float)ny64[1];
@@ -846,14 +890,16 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
 
             // MIXED: near pairs are left to the FP64 near-field kernel (bem_near_assemble)
             bool skip = adj;
+            // normal multipliers of the pair (the hypersingular epilogue)
+            const double nmp = (a.test.nmult ? (double)a.test.nmult[e] : 1.0) * snm;
             if (valid && !adj) {
                 if constexpr (!MIXED) {
                     double tnm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
                     if constexpr (QT > 0)
-                        pair_fp64_regs<OP, NTS, NSS, QT, DUAL, SYM>(T_, te, gt, tnm, yq, wq64, ny64,
+                        pair_fp64_regs<GOP, NTS, NSS, QT, DUAL, SYM>(T_, te, gt, tnm, yq, wq64, ny64,
                                                                                       a.rule, a.kr, a.ki, S, S2, S3);
                     else
-                        pair_fp64_geom<OP, NTS, NSS, QT, DUAL, SYM>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2, S3);
+                        pair_fp64_geom<GOP, NTS, NSS, QT, DUAL, SYM>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2, S3);
                 } else {
                     const double cfv[3] = {cf0, cf1, cf2};
                     double dist2;
@@ -861,10 +907,10 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                     if (!skip) {
                         double d0 = cf0 - T_.c[te][0], d1 = cf1 - T_.c[te][1], d2 = cf2 - T_.c[te][2];
                         if (OpTraits<OP>::is_complex && a.ki != 0.0)
-                            pair_fp32<OP, NTS, NSS, QT, KAHAN, true, DUAL, SYM>(T_, te, F32, d0, d1, d2, dist2, a.rule,
+                            pair_fp32<GOP, NTS, NSS, QT, KAHAN, true, DUAL, SYM>(T_, te, F32, d0, d1, d2, dist2, a.rule,
                                                                                        a.kr, a.ki, Sf, Sf2, Sf3);
                         else
-                            pair_fp32<OP, NTS, NSS, QT, KAHAN, false, DUAL, SYM>(T_, te, F32, d0, d1, d2, dist2,
+                            pair_fp32<GOP, NTS, NSS, QT, KAHAN, false, DUAL, SYM>(T_, te, F32, d0, d1, d2, dist2,
                                                                                         a.rule, a.kr, a.ki, Sf, Sf2, Sf3);
                     }
                 }
@@ -894,6 +940,8 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                     C2f V[NTS][NSS];
                     if constexpr (OP == kMaxwellEFIE)
                         efie_block_f32(Sm, T_.ee[te], T_.el[te], T_.ej[te], ef, lf, jfi, ikr, iki, invr, invi, V);
+                    else if constexpr (HSX)
+                        hs_block_f32<CPLX>(Sm, T_.ee[te], T_.ej[te], T_.nx[te], ef, jfi, F32.ny, k2r, k2i, V);
                     else
                         scalar_block_f32<NTS, NSS>(Sm, V);
     #pragma unroll
@@ -956,7 +1004,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                         const double m = mi * mf[j];
                         if constexpr (TILE) {
                             if (m != 0.0 && !skip) {
-                                C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
+                                C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki, nmp);
                                 const int off = (CPLX ? 2 : 1) * (T_.slot[te][i] * MAXU_S + slot_f[j]);
                                 unsafeAtomicAdd(acc + off, (AccT)(v.re * m));
                                 if constexpr (CPLX) unsafeAtomicAdd(acc + off + 1, (AccT)(v.im * m));
@@ -972,7 +1020,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                                 }
                             }
                         } else {
-                            C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
+                            C2d v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki, nmp);
                             v.re *= m;
                             v.im *= m;
                             int row = a.test.l2g[(int64_t)e * NT + i];
@@ -1105,6 +1153,12 @@ template <int OP, bool MIXED, typename OutT, bool DUAL = false, bool SYM = false
 static int launch1(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
     if constexpr (OP == kMaxwellEFIE) {
         return launch2<OP, 3, 3, MIXED, OutT>(a, kahan, tile, stream);
+    } else if constexpr (is_hypersingular<OP>()) {
+        if (nshape_of(a.test.shape) != 3 || nshape_of(a.trial.shape) != 3) {
+            set_error("hypersingular operators need P1 shape functions on both sides");
+            return BEM_ERR_INVALID_ARGUMENT;
+        }
+        return launch2<OP, 3, 3, MIXED, OutT>(a, kahan, tile, stream);
     } else if constexpr (UPPER) {
         if (nshape_of(a.test.shape) != 1 || nshape_of(a.trial.shape) != 1) {
             set_error("the symmetric DP0 pass needs piecewise constant spaces");
@@ -1161,6 +1215,8 @@ BEM_DECLARE_REGULAR_OP(3)
 BEM_DECLARE_REGULAR_OP(4)
 BEM_DECLARE_REGULAR_OP(5)
 BEM_DECLARE_REGULAR_OP(6)
+BEM_DECLARE_REGULAR_OP(7)
+BEM_DECLARE_REGULAR_OP(8)
 #undef BEM_DECLARE_REGULAR_OP
 // single layer + double layer in one pass (regular_dual<k>.hip, k = the single layer)
 int regular_launch_dual_0(const RegularArgs &a, int precision, int kahan, bool tile, int out_dtype, hipStream_t s);

@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
             double dz = (gf[2] + gf[5] * et0 + gf[8] * et1) - (gt[2] + gt[5] * xi0 + gt[8] * xi1);
             double r2 = dx * dx + dy * dy + dz * dz;
             double gre, gim, g2re, g2im;
-            green2<OP, DUAL, double, SMALL>(dx, dy, dz, r2, nx, ny, a.kr, a.ki, gre, gim, g2re, g2im);
+            green2<green_op<OP>(), DUAL, double, SMALL>(dx, dy, dz, r2, nx, ny, a.kr, a.ki, gre, gim, g2re, g2im);
             gre *= w;
             gim *= w;
             g2re *= w;
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
                 v = second ? local_value<OP + 1, NTS, NSS>(S2, i, j, gt, gf, a.kr, a.ki)
                            : local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
             } else {
-                v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki);
+                v = local_value<OP, NTS, NSS>(S, i, j, gt, gf, a.kr, a.ki, tnm * snm);
             }
             int row = a.test.l2g[(int64_t)e * NTS + i];
             int col = a.trial.l2g[(int64_t)f * NSS + j];
@@ -140,6 +140,12 @@ static int launch_sing(const SingularArgs &a, hipStream_t s) {
     dim3 grid((a.n_pairs + 3) / 4);
     if (a.n_pairs == 0) return BEM_OK;
     if constexpr (OP == kMaxwellEFIE) {
+        hipLaunchKernelGGL((singular_kernel<OP, 3, 3, OutT>), grid, dim3(256), 0, s, a);
+    } else if constexpr (is_hypersingular<OP>()) {
+        if (nshape_of(a.test.shape) != 3 || nshape_of(a.trial.shape) != 3) {
+            set_error("hypersingular operators need P1 shape functions on both sides");
+            return BEM_ERR_INVALID_ARGUMENT;
+        }
         hipLaunchKernelGGL((singular_kernel<OP, 3, 3, OutT>), grid, dim3(256), 0, s, a);
     } else {
         int nt = nshape_of(a.test.shape), ns = nshape_of(a.trial.shape);
@@ -209,6 +215,8 @@ static int singular_entry(int op, const bem_space_t *test, const bem_space_t *tr
     case BEM_HELMHOLTZ_DOUBLE_LAYER: return launch_sing_op<kHelmholtzDL>(a, out_dtype, s);
     case BEM_HELMHOLTZ_ADJOINT_DOUBLE_LAYER: return launch_sing_op<kHelmholtzADL>(a, out_dtype, s);
     case BEM_MAXWELL_ELECTRIC_FIELD: return launch_sing_op<kMaxwellEFIE>(a, out_dtype, s);
+    case BEM_LAPLACE_HYPERSINGULAR: return launch_sing_op<kLaplaceHS>(a, out_dtype, s);
+    case BEM_HELMHOLTZ_HYPERSINGULAR: return launch_sing_op<kHelmholtzHS>(a, out_dtype, s);
     default: set_error("unsupported operator %d", op); return BEM_ERR_UNSUPPORTED;
     }
 }

@@ -43,7 +43,11 @@ enum bem_operator {
     BEM_HELMHOLTZ_SINGLE_LAYER = 3,
     BEM_HELMHOLTZ_DOUBLE_LAYER = 4,
     BEM_HELMHOLTZ_ADJOINT_DOUBLE_LAYER = 5,
-    BEM_MAXWELL_ELECTRIC_FIELD = 6
+    BEM_MAXWELL_ELECTRIC_FIELD = 6,
+    /* P1 x P1 only: the single layer's Green's function with the surface-curl weights of
+       bempp_cl/core/numba_kernels.py:1078-1333 (regular) and 1530-1697 (singular) */
+    BEM_LAPLACE_HYPERSINGULAR = 7,
+    BEM_HELMHOLTZ_HYPERSINGULAR = 8
 };
 
 /* reference shapesets (bempp_cl/api/space/shapesets.py:46-158) */

@@ -14,8 +14,10 @@
  *   geometry            bempp_cl/api/grid/grid.py:552-590
  *   adjacency ordering  bempp_cl/api/grid/grid.py:1015-1098
  *   kernels             bempp_cl/core/numba_kernels.py:176-398 (regular), 453-551 (singular)
- *   regular assembly    bempp_cl/core/numba_kernels.py:980-1074 (scalar), 1853-1969 (EFIE)
+ *   regular assembly    bempp_cl/core/numba_kernels.py:980-1074 (scalar), 1853-1969 (EFIE),
+ *                       1078-1333 (Laplace / Helmholtz hypersingular)
  *   singular assembly   bempp_cl/core/numba_kernels.py:1471-1526 (scalar), 1973-2061 (EFIE),
+ *                       1530-1697 (hypersingular),
  *                       bempp_cl/core/singular_assembler.py:69-448, dense_assembler.py:78-98
  *   potentials          bempp_cl/core/numba_kernels.py:1788-1849 (scalar), 2281-2359 (EFIE)
  *   Piola / edge length bempp_cl/core/numba_kernels.py:96-133
@@ -32,7 +34,8 @@
 #include <omp.h>
 #endif
 
-enum { OP_LAPLACE_SL = 0, OP_LAPLACE_DL, OP_LAPLACE_ADL, OP_HELMHOLTZ_SL, OP_HELMHOLTZ_DL, OP_HELMHOLTZ_ADL, OP_MAXWELL_EFIE };
+enum { OP_LAPLACE_SL = 0, OP_LAPLACE_DL, OP_LAPLACE_ADL, OP_HELMHOLTZ_SL, OP_HELMHOLTZ_DL, OP_HELMHOLTZ_ADL, OP_MAXWELL_EFIE,
+       OP_LAPLACE_HS, OP_HELMHOLTZ_HS };
 enum { SHAPE_P0 = 0, SHAPE_P1 = 1, SHAPE_RWG = 2 };
 
 #define M_INV_4PI_ 0.079577471545947667884
@@ -90,6 +93,43 @@ static inline void rwg_piola(const Elem *g, double x, double y, double out[3][3]
         for (int d = 0; d < 3; ++d) out[i][d] = (g->e1[d] * ref[i][0] + g->e2[d] * ref[i][1]) / g->intel;
 }
 
+/* surface curls n x grad(phi_i) of the three P1 shape functions, times the normal multiplier:
+ * grad = J (J^T J)^{-1} [[-1, 1, 0], [-1, 0, 1]] (bempp_cl/api/grid/grid.py:585-590,
+ * bempp_cl/core/numba_kernels.py:1120-1147) */
+static void surface_curls(const Elem *g, double nm, double curl[3][3]) {
+    double a = 0, b = 0, c = 0;
+    for (int d = 0; d < 3; ++d) { a += g->e1[d] * g->e1[d]; b += g->e1[d] * g->e2[d]; c += g->e2[d] * g->e2[d]; }
+    double det = a * c - b * b;
+    double i00 = c / det, i01 = -b / det, i11 = a / det;
+    double jit[3][2];
+    for (int d = 0; d < 3; ++d) {
+        jit[d][0] = g->e1[d] * i00 + g->e2[d] * i01;
+        jit[d][1] = g->e1[d] * i01 + g->e2[d] * i11;
+    }
+    static const double rg[3][2] = {{-1, -1}, {1, 0}, {0, 1}};
+    for (int i = 0; i < 3; ++i) {
+        double gr[3];
+        for (int d = 0; d < 3; ++d) gr[d] = jit[d][0] * rg[i][0] + jit[d][1] * rg[i][1];
+        curl[i][0] = (g->n[1] * gr[2] - g->n[2] * gr[1]) * nm;
+        curl[i][1] = (g->n[2] * gr[0] - g->n[0] * gr[2]) * nm;
+        curl[i][2] = (g->n[0] * gr[1] - g->n[1] * gr[0]) * nm;
+    }
+}
+
+/* hypersingular weight of (i, j) at a point pair: curl_i . curl_j - k^2 phi_i psi_j (n_x . n_y)
+ * (bempp_cl/core/numba_kernels.py:1284-1299); Laplace: the curl product only */
+static inline cplx hs_weight(int op, double ct[3][3], double cs[3][3], const double *tn, const double *sn,
+                             const double *phi, const double *psi, int i, int j, double kr, double ki) {
+    cplx w = {ct[i][0] * cs[j][0] + ct[i][1] * cs[j][1] + ct[i][2] * cs[j][2], 0.0};
+    if (op == OP_HELMHOLTZ_HS) {
+        double nn = tn[0] * sn[0] + tn[1] * sn[1] + tn[2] * sn[2];
+        double pp = phi[i] * psi[j] * nn;
+        w.re -= (kr * kr - ki * ki) * pp;
+        w.im -= 2.0 * kr * ki * pp;
+    }
+    return w;
+}
+
 /* ------------------------------------------------------------------ kernels */
 /* G(x, y) for the operator. nx / ny are the (multiplied) test / trial normals. */
 static inline cplx kernel_eval(int op, const double *x, const double *y, const double *nx, const double *ny, double kr, double ki) {
@@ -98,10 +138,12 @@ static inline cplx kernel_eval(int op, const double *x, const double *y, const d
     double r = sqrt(r2);
     cplx out = {0.0, 0.0};
     switch (op) {
-    case OP_LAPLACE_SL: out.re = M_INV_4PI_ / r; break;
+    case OP_LAPLACE_SL:
+    case OP_LAPLACE_HS: out.re = M_INV_4PI_ / r; break;
     case OP_LAPLACE_DL: out.re = -M_INV_4PI_ * (diff[0] * ny[0] + diff[1] * ny[1] + diff[2] * ny[2]) / (r * r * r); break;
     case OP_LAPLACE_ADL: out.re = M_INV_4PI_ * (diff[0] * nx[0] + diff[1] * nx[1] + diff[2] * nx[2]) / (r * r * r); break;
     case OP_HELMHOLTZ_SL:
+    case OP_HELMHOLTZ_HS:
     case OP_MAXWELL_EFIE: {
         double damp = exp(-ki * r);
         out.re = cos(kr * r) * M_INV_4PI_ / r * damp;
@@ -298,6 +340,9 @@ static void regular_pair(int op, const Elem *te, const Elem *se, double tnm, dou
     double sn[3] = {se->n[0] * snm, se->n[1] * snm, se->n[2] * snm};
     for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) loc[i][j].re = loc[i][j].im = 0;
     cplx ik = {-ki, kr};
+    const int hs = op == OP_LAPLACE_HS || op == OP_HELMHOLTZ_HS;
+    double ct[3][3], cs[3][3];
+    if (hs) { surface_curls(te, tnm, ct); surface_curls(se, snm, cs); }
     for (int p = 0; p < nq; ++p) {
         double x[3]; map_point(te, qx[p], qy[p], x);
         double phi[3] = {0, 0, 0}, tv[3][3] = {{0}};
@@ -317,6 +362,12 @@ static void regular_pair(int op, const Elem *te, const Elem *se, double tnm, dou
                     double dot = tv[i][0] * sv[j][0] + tv[i][1] * sv[j][1] + tv[i][2] * sv[j][2];
                     cplx fac = {-ik.re * dot - divp * inv_ik.re, -ik.im * dot - divp * inv_ik.im};
                     cplx c = cmul(g, fac);
+                    loc[i][j].re += c.re; loc[i][j].im += c.im;
+                }
+            } else if (hs) {
+                double psi[3]; scalar_shape(sshape, qx[q], qy[q], psi);
+                for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+                    cplx c = cmul(g, hs_weight(op, ct, cs, tn, sn, phi, psi, i, j, kr, ki));
                     loc[i][j].re += c.re; loc[i][j].im += c.im;
                 }
             } else {
@@ -341,6 +392,9 @@ static void singular_pair(int op, const Elem *te, const Elem *se, double tnm, do
     cplx ik = {-ki, kr};
     double den = ik.re * ik.re + ik.im * ik.im;
     cplx inv_ik = {ik.re / den, -ik.im / den};
+    const int hs = op == OP_LAPLACE_HS || op == OP_HELMHOLTZ_HS;
+    double ct[3][3], cs[3][3];
+    if (hs) { surface_curls(te, tnm, ct); surface_curls(se, snm, cs); }
     for (int k = 0; k < np; ++k) {
         double x[3], y[3];
         map_point(te, tx[k], ty[k], x);
@@ -355,6 +409,14 @@ static void singular_pair(int op, const Elem *te, const Elem *se, double tnm, do
                 double dot = tv[i][0] * sv[j][0] + tv[i][1] * sv[j][1] + tv[i][2] * sv[j][2];
                 cplx fac = {-ik.re * dot - divp * inv_ik.re, -ik.im * dot - divp * inv_ik.im};
                 cplx c = cmul(g, fac);
+                loc[i][j].re += c.re; loc[i][j].im += c.im;
+            }
+        } else if (hs) {
+            double phi[3] = {0, 0, 0}, psi[3] = {0, 0, 0};
+            scalar_shape(tshape, tx[k], ty[k], phi);
+            scalar_shape(sshape, sx[k], sy[k], psi);
+            for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+                cplx c = cmul(g, hs_weight(op, ct, cs, tn, sn, phi, psi, i, j, kr, ki));
                 loc[i][j].re += c.re; loc[i][j].im += c.im;
             }
         } else {
@@ -426,6 +488,7 @@ int bem_oracle_assemble(int op, int nv, const double *vtx, int nf, const int32_t
     build_elements(nf, vtx, elm, G);
     Spaces S = {op, tshape, sshape, nshape_of(tshape), nshape_of(sshape), tl2g, sl2g, tmult, smult, tnm, snm, tsup, ssup};
     if (op == OP_MAXWELL_EFIE) { S.nt = 3; S.ns = 3; }
+    if ((op == OP_LAPLACE_HS || op == OP_HELMHOLTZ_HS) && (tshape != SHAPE_P1 || sshape != SHAPE_P1)) return -2;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif

@@ -26,6 +26,8 @@ OPS = {
     "helmholtz_double_layer": 4,
     "helmholtz_adjoint_double_layer": 5,
     "maxwell_electric_field": 6,
+    "laplace_hypersingular": 7,
+    "helmholtz_hypersingular": 8,
 }
 
 SHAPES = {"p0_discontinuous": 0, "p1_discontinuous": 1, "rwg0": 2, "snc0": 2}

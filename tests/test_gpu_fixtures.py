@@ -41,6 +41,9 @@ CASES = [
     ("helmholtz_complex_single_layer_boundary", "helmholtz.single_layer", ("P", 1), ("P", 1), 2.5 + 1j),
     ("helmholtz_complex_double_layer_boundary", "helmholtz.double_layer", ("P", 1), ("P", 1), 2.5 + 1j),
     ("helmholtz_complex_adj_double_layer_boundary", "helmholtz.adjoint_double_layer", ("P", 1), ("P", 1), 2.5 + 1j),
+    ("laplace_hypersingular_boundary", "laplace.hypersingular", ("P", 1), ("P", 1), None),
+    ("helmholtz_hypersingular_boundary", "helmholtz.hypersingular", ("P", 1), ("P", 1), 2.5),
+    ("helmholtz_complex_hypersingular_boundary", "helmholtz.hypersingular", ("P", 1), ("P", 1), 2.5 + 1j),
 ]
 
 

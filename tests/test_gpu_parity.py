@@ -29,6 +29,10 @@ CASES = [
     ("helmholtz", "double_layer", ("P", 1), ("P", 1), 4.0),
     ("helmholtz", "adjoint_double_layer", ("P", 1), ("P", 1), 3.0 + 0.5j),
     ("helmholtz", "single_layer", ("DP", 0), ("P", 1), 2.0),
+    ("laplace", "hypersingular", ("P", 1), ("P", 1), None),
+    ("helmholtz", "hypersingular", ("P", 1), ("P", 1), 4.0),
+    ("helmholtz", "hypersingular", ("DP", 1), ("DP", 1), 3.0 + 0.5j),
+    ("helmholtz", "hypersingular", ("DP", 1), ("P", 1), 2.0),
 ]
 
 
@@ -45,6 +49,27 @@ def test_scalar_operators_vs_oracle(case, ico2, default_parameters, hip_availabl
     args = (dom, dom, dual) if k is None else (dom, dom, dual, k)
     A64 = factory(*args, parameters=default_parameters, precision="double").weak_form().to_dense()
     ref = O.assemble(f"{mod}_{fn}", ico2.vertices, ico2.elements, dom, dual, 0.0 if k is None else k)
+    assert rel_fro(A64, ref) < 1e-12
+    A32 = factory(*args, parameters=default_parameters, precision="single").weak_form().to_dense()
+    assert rel_fro(A32.astype(A64.dtype), A64) < 1e-6
+
+
+@pytest.mark.parametrize("fam,k", [("laplace", None), ("helmholtz", 3.0 + 0.5j)])
+def test_hypersingular_swapped_normals_vs_oracle(fam, k, ico2, default_parameters, hip_available):
+    """Normal multipliers -1 on one segment: the curls and the normal product both carry them."""
+    import bempp_amd.api.operators.boundary as B
+    from bempp_amd.api import function_space
+    from bempp_amd.api.grid.grid import Grid
+    from oracle import oracle as O
+
+    upper = (ico2.centroids[:, 2] > 0.0).astype(np.uint32)
+    grid = Grid(ico2.vertices, ico2.elements, domain_indices=upper)
+    p1 = function_space(grid, "P", 1, swapped_normals=[1])
+    assert (p1.normal_multipliers == -1).any() and (p1.normal_multipliers == 1).any()
+    factory = getattr(B, fam).hypersingular
+    args = (p1, p1, p1) if k is None else (p1, p1, p1, k)
+    A64 = factory(*args, parameters=default_parameters, precision="double").weak_form().to_dense()
+    ref = O.assemble(f"{fam}_hypersingular", grid.vertices, grid.elements, p1, p1, 0.0 if k is None else k)
     assert rel_fro(A64, ref) < 1e-12
     A32 = factory(*args, parameters=default_parameters, precision="single").weak_form().to_dense()
     assert rel_fro(A32.astype(A64.dtype), A64) < 1e-6

@@ -36,6 +36,9 @@ BOUNDARY = [
     ("helmholtz_complex_adj_double_layer_boundary", "helmholtz_adjoint_double_layer", ("P", 1), ("P", 1), 2.5 + 1j),
     ("maxwell_electric_field_boundary", "maxwell_electric_field", ("RWG", 0), ("SNC", 0), 2.5),
     ("maxwell_electric_field_complex_boundary", "maxwell_electric_field", ("RWG", 0), ("SNC", 0), 2.5 + 1j),
+    ("laplace_hypersingular_boundary", "laplace_hypersingular", ("P", 1), ("P", 1), 0.0),
+    ("helmholtz_hypersingular_boundary", "helmholtz_hypersingular", ("P", 1), ("P", 1), 2.5),
+    ("helmholtz_complex_hypersingular_boundary", "helmholtz_hypersingular", ("P", 1), ("P", 1), 2.5 + 1j),
 ]
 
 
