@@ -1,7 +1,8 @@
 """bempp_amd.api: the bempp-cl style API of the MI355X dense BEM library.
 
-Mirrors the parts of bempp_cl.api (bempp_cl/api/__init__.py) used by dense assembly and
-dense potentials: grids, function spaces, boundary and potential operators, parameters.
+Mirrors the parts of bempp_cl.api (bempp_cl/api/__init__.py) used by dense assembly, dense
+potentials and solves on the assembled operators: grids, function spaces, grid functions,
+boundary (and blocked) and potential operators, the L2 identity, GMRES / CG / LU, parameters.
 The only device interface is "hip" (hand-written HIP kernels in libbempp_hip.so).
 """
 
@@ -12,7 +13,12 @@ from bempp_amd.api.utils.parameters import DefaultParameters
 from bempp_amd.api.grid import Grid, import_grid, export, shapes  # noqa: F401
 from bempp_amd.api.space import function_space  # noqa: F401
 from bempp_amd.api.assembly.grid_function import GridFunction  # noqa: F401
+from bempp_amd.api.assembly.grid_function import real_callable, complex_callable, callable  # noqa: F401,A004
+from bempp_amd.api.assembly.blocked_operator import BlockedOperator  # noqa: F401
+from bempp_amd.api.assembly.discrete_boundary_operator import as_matrix  # noqa: F401
 from bempp_amd.api import operators  # noqa: F401
+from bempp_amd.api import linalg  # noqa: F401
+from bempp_amd.api.linalg import lu, compute_lu_factors, gmres, cg  # noqa: F401
 
 LOGGER = _logging.getLogger("bempp_amd")
 LOGGER.addHandler(_logging.NullHandler())

@@ -1,16 +1,20 @@
 """Assembler selection (bempp_cl/api/assembly/assembler.py:4-186).
 
 "dense" and "default_nonlocal" select the dense assembler, as in the reference
-(assembler.py:24-27); other assembler types of the reference (sparse, fmm, singular-only,
-diagonal) are outside this library and raise ValueError.
+(assembler.py:24-27), and "sparse" the host assembler of the L2 identity; other assembler
+types of the reference (fmm, singular-only, diagonal) are outside this library and raise
+ValueError.
 """
 
 
 def _create_assembler(domain, dual_to_range, identifier, parameters, device_interface=None):
     from bempp_amd.core.dense_assembler import DenseAssembler
+    from bempp_amd.core.sparse_assembler import SparseAssembler
 
     if identifier in ("dense", "default_nonlocal"):
         return DenseAssembler(domain, dual_to_range, parameters)
+    if identifier == "sparse":
+        return SparseAssembler(domain, dual_to_range, parameters)
     raise ValueError("Unknown assembler type.")
 
 

@@ -1,3 +1,3 @@
-"""Boundary operators: Laplace, Helmholtz and Maxwell (electric field)."""
+"""Boundary operators: Laplace, Helmholtz, Maxwell (electric field) and the sparse identity."""
 
-from . import laplace, helmholtz, maxwell  # noqa: F401
+from . import laplace, helmholtz, maxwell, sparse  # noqa: F401

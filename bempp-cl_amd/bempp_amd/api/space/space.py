@@ -203,6 +203,19 @@ class FunctionSpace(object):
     def is_compatible(self, other):
         return self == other
 
+    def evaluate(self, element, local_coordinates):
+        """Basis values (codomain, nshape, npoints) on one element with its local multipliers (scalar spaces)."""
+        if self.codomain_dimension != 1:
+            raise NotImplementedError("Evaluation is provided for scalar spaces (DP0, DP1, P1).")
+        vals = self._shapeset.evaluate(local_coordinates)
+        return vals * self.local_multipliers[element][None, :, None]
+
+    def mass_matrix(self):
+        """Mass matrix of the space with itself (scipy CSR)."""
+        from bempp_amd.core.sparse_assembler import mass_matrix
+
+        return mass_matrix(self, self)
+
     def __eq__(self, other):
         if self is other:
             return True
