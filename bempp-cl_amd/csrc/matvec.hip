@@ -7,7 +7,10 @@
 // discrete_boundary_operator.py:85-110, `self.A @ x`) and, for the adjoint, `A^H x`.
 //
 // The products stream the matrix once (HBM bound):
-//  * op = N: one wavefront per row; lanes read consecutive columns (coalesced), x from L2.
+//  * op = N, 16-byte aligned rows: workgroups of 32 rows x 1024 columns; the x chunk is staged in
+//    LDS once per workgroup, each wavefront streams 8 rows with 16-byte loads (8 independent loads
+//    in flight per lane), and the column chunks' FP64 sums are combined with FP64 atomics.
+//  * op = N otherwise: one wavefront per row; lanes read consecutive columns, x from L2.
 //  * op = T/C: each workgroup owns 256 consecutive columns and a band of rows; every lane
 //    walks its column down the band, and the band sums are added to y with FP64 atomics.
 #include "bem_device.h"
@@ -54,6 +57,119 @@ __device__ __forceinline__ T wsum(T v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
+}
+
+// 16-byte loads of the matrix: E elements per load
+template <typename AT>
+struct Pack16;
+template <>
+struct Pack16<float2> {
+    static constexpr int E = 2;
+    using V = float4;
+    static __device__ __forceinline__ void get(const V &v, int e, double &re, double &im) {
+        re = e ? v.z : v.x;
+        im = e ? v.w : v.y;
+    }
+};
+template <>
+struct Pack16<double2> {
+    static constexpr int E = 1;
+    using V = double2;
+    static __device__ __forceinline__ void get(const V &v, int, double &re, double &im) {
+        re = v.x;
+        im = v.y;
+    }
+};
+template <>
+struct Pack16<float> {
+    static constexpr int E = 4;
+    using V = float4;
+    static __device__ __forceinline__ void get(const V &v, int e, double &re, double &im) {
+        re = e == 0 ? v.x : e == 1 ? v.y : e == 2 ? v.z : v.w;
+        im = 0.0;
+    }
+};
+template <>
+struct Pack16<double> {
+    static constexpr int E = 2;
+    using V = double2;
+    static __device__ __forceinline__ void get(const V &v, int e, double &re, double &im) {
+        re = e ? v.y : v.x;
+        im = 0.0;
+    }
+};
+
+constexpr int kMvRows = 32;          // rows per workgroup
+constexpr int kMvRowsPerWave = 8;    // rows per wavefront
+constexpr int kMvCols = 1024;        // columns per workgroup (x chunk: 16 KB of LDS)
+
+// y_i += sum_{j in the chunk} A_ij x_j (y zeroed beforehand); rows start 16-byte aligned
+template <typename AT>
+__global__ __launch_bounds__(256, sizeof(AT) == 4 ? 3 : 4) void matvec_n_tiled_kernel(const void *A, int64_t rows, int64_t cols, int64_t ld,
+                                                             const double *x, double *y) {
+    using P = Pack16<AT>;
+    constexpr int E = P::E;
+    __shared__ double2 xs[kMvCols];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t c0 = (int64_t)blockIdx.y * kMvCols;
+    const int nc = (int)min((int64_t)kMvCols, cols - c0);
+    for (int j = tid; j < kMvCols; j += 256)
+        xs[j] = j < nc ? make_double2(x[2 * (c0 + j)], x[2 * (c0 + j) + 1]) : make_double2(0.0, 0.0);
+    __syncthreads();
+    const int64_t r0 = (int64_t)blockIdx.x * kMvRows + wave * kMvRowsPerWave;
+    const int nr = (int)max((int64_t)0, min((int64_t)kMvRowsPerWave, rows - r0));  // uniform per wavefront
+    double sr[kMvRowsPerWave], si[kMvRowsPerWave];
+#pragma unroll
+    for (int r = 0; r < kMvRowsPerWave; ++r) sr[r] = si[r] = 0.0;
+    if (nr == 0) return;  // uniform per wavefront, after the only barrier
+    // rows past the end read the last row again (in bounds); their sums are never stored, so
+    // every step issues its kMvRowsPerWave loads back to back with no branches
+    const AT *rowp[kMvRowsPerWave];
+#pragma unroll
+    for (int r = 0; r < kMvRowsPerWave; ++r)
+        rowp[r] = reinterpret_cast<const AT *>(A) + (r0 + min(r, nr - 1)) * ld + c0;
+    const int nfull = nc - nc % E;  // columns covered by whole 16-byte loads
+    for (int j = lane * E; j < nfull; j += 64 * E) {
+        double2 xv[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) xv[e] = xs[j + e];
+        typename P::V v[kMvRowsPerWave];
+#pragma unroll
+        for (int r = 0; r < kMvRowsPerWave; ++r) v[r] = *reinterpret_cast<const typename P::V *>(rowp[r] + j);
+#pragma unroll
+        for (int r = 0; r < kMvRowsPerWave; ++r) {
+            double tr = 0.0, ti = 0.0;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                double ar, ai;
+                P::get(v[r], e, ar, ai);
+                tr += ar * xv[e].x - ai * xv[e].y;
+                ti += ar * xv[e].y + ai * xv[e].x;
+            }
+            sr[r] += tr;
+            si[r] += ti;
+        }
+    }
+    // the chunk's last nc % E columns (the row's last columns): element loads
+    for (int j = nfull + lane; j < nc; j += 64) {
+        const double2 xv = xs[j];
+#pragma unroll
+        for (int r = 0; r < kMvRowsPerWave; ++r) {
+            double ar, ai;
+            Elem<AT>::get(rowp[r], j, ar, ai);
+            sr[r] += ar * xv.x - ai * xv.y;
+            si[r] += ar * xv.y + ai * xv.x;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kMvRowsPerWave; ++r) {
+        if (r >= nr) break;
+        const double a = wsum(sr[r]), b = wsum(si[r]);
+        if (lane == 0) {
+            atomic_add(y + 2 * (r0 + r), a);
+            atomic_add(y + 2 * (r0 + r) + 1, b);
+        }
+    }
 }
 
 // y_i = sum_j A_ij x_j, one wavefront per row
@@ -110,6 +226,14 @@ __global__ void zero_kernel(double *y, int64_t n) {
 template <typename AT>
 int launch(int trans, const void *A, int64_t rows, int64_t cols, int64_t ld, const double *x, double *y,
            hipStream_t s) {
+    const bool aligned = reinterpret_cast<uintptr_t>(A) % 16 == 0 && (ld * (int64_t)sizeof(AT)) % 16 == 0;
+    const int64_t nchunks = (cols + kMvCols - 1) / kMvCols;
+    if (trans == 0 && aligned && nchunks <= 65535) {
+        hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((2 * rows + 255) / 256)), dim3(256), 0, s, y, 2 * rows);
+        const dim3 grid((unsigned)((rows + kMvRows - 1) / kMvRows), (unsigned)nchunks);
+        hipLaunchKernelGGL(matvec_n_tiled_kernel<AT>, grid, dim3(256), 0, s, A, rows, cols, ld, x, y);
+        return check_launch("matvec_n_tiled_kernel");
+    }
     if (trans == 0) {
         hipLaunchKernelGGL(matvec_n_kernel<AT>, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s, A, rows, cols, ld, x,
                            y);

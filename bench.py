@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--no-sweep", action="store_true", help="skip the frequency-sweep measurement")
     ap.add_argument("--no-efie", action="store_true", help="skip the Maxwell EFIE measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the loss/gradient measurement")
+    ap.add_argument("--no-solve", action="store_true", help="skip the GMRES solve on the assembled operator")
+    ap.add_argument("--no-hypersingular", action="store_true", help="skip the hypersingular assembly")
     ap.add_argument("--adjoint-level", type=int, default=6)
     ap.add_argument("--adjoint-k", type=float, default=8.0)
     ap.add_argument("--fd-vertices", type=int, default=32)
@@ -395,6 +397,94 @@ def frequency_sweep_bench(level, reps):
     }
 
 
+def solve_bench(space, A, k, tol=1e-5):
+    """GMRES (restart 20, rtol 1e-5: the reference's defaults) on the assembled mixed single layer.
+
+    The right-hand side is the projection of the plane wave e^{ik x} onto the P1 space; the
+    matrix is the complex64 operator the assembly step left in HBM. Also times the HIP product
+    alone (bem_dense_matvec, one complex64 matrix read per product) for its HBM roofline.
+    """
+    import numpy as np
+    import torch
+
+    import bempp_amd.api as bempp
+    from bempp_amd.api.linalg.iterative_solvers import IterationCounter, device_gmres
+    from bempp_amd.core import hip_backend as hb
+
+    @bempp.complex_callable
+    def plane_wave(x, n, domain_index, res):
+        res[0] = np.exp(1j * k * x[0])
+
+    b = bempp.GridFunction(space, fun=plane_wave).projections()
+    counter = IterationCounter(True)
+    device_gmres(A, b, tol, None, None, counter)  # warm-up (library paths, rocBLAS handles)
+    torch.cuda.synchronize()
+    counter = IterationCounter(True)
+    t = time.perf_counter()
+    x, info = device_gmres(A, b, tol, None, None, counter)
+    torch.cuda.synchronize()
+    t_solve = time.perf_counter() - t
+    xh = x.cpu().numpy()
+    # true relative residual with the FP64 product on the device
+    r = torch.as_tensor(b, device=A.device) - hb.dense_matvec(A, x)
+    rel_res = float(torch.linalg.vector_norm(r)) / float(np.linalg.norm(b))
+    # the product alone, HIP events on the current stream
+    v = torch.as_tensor(xh, device=A.device)
+    reps = 20
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    hb.dense_matvec(A, v)
+    e0.record()
+    for _ in range(reps):
+        hb.dense_matvec(A, v)
+    e1.record()
+    torch.cuda.synchronize()
+    mv_ms = e0.elapsed_time(e1) / reps
+    n = A.shape[0]
+    bytes_mv = n * n * A.element_size()
+    return {
+        "workload": f"GMRES(20), rtol {tol}, Helmholtz single layer P1 {n} dofs (complex64, mixed), plane-wave rhs",
+        "iterations": counter.count,
+        "info": int(info),
+        "seconds": t_solve,
+        "ms_per_iteration": 1e3 * t_solve / max(counter.count, 1),
+        "final_rel_residual": rel_res,
+        "matvec_ms": mv_ms,
+        "matvec_hbm_gb_s": bytes_mv / (mv_ms * 1e-3) / 1e9,
+        "matvec_hbm_frac": bytes_mv / (mv_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+    }
+
+
+def hypersingular_bench(space, k, reps=3):
+    """Helmholtz hypersingular P1 x P1 on the same space: mixed vs FP64 (time and error)."""
+    import torch
+
+    from bempp_amd.api.operators.boundary import helmholtz
+
+    def run(precision):
+        op = helmholtz.hypersingular(space, space, space, k, precision=precision)
+        from bempp_amd.core.dense_assembler import hip_dense_assemble
+
+        out = hip_dense_assemble(space, space, op.assembler.parameters, op.descriptor)  # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            out = hip_dense_assemble(space, space, op.assembler.parameters, op.descriptor, out=out, accumulate=False)
+        torch.cuda.synchronize()
+        return out, (time.perf_counter() - t) / reps
+
+    m32, t32 = run("single")
+    m64, t64 = run("double")
+    n = space.global_dof_count
+    return {
+        "workload": f"Helmholtz hypersingular P1 x P1, {n} dofs, k={k}",
+        "mixed_ms": 1e3 * t32,
+        "fp64_ms": 1e3 * t64,
+        "mixed_entries_per_s": n * n / t32,
+        "speedup_vs_fp64": t64 / t32,
+        "rel_err_vs_fp64": rel_fro(m32, m64),
+    }
+
+
 def traffic_from_profiles(kernel_key):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if recorded."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -697,6 +787,14 @@ def main():
             "rel_err_vs_fp64": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
         }
         log(f"potential {result['potential']}")
+
+    if not args.no_solve:
+        result["solve"] = solve_bench(space, outs[SL], k)
+        log(f"solve {result['solve']}")
+
+    if not args.no_hypersingular:
+        result["hypersingular"] = hypersingular_bench(space, k)
+        log(f"hypersingular {result['hypersingular']}")
 
     if not args.no_efie:
         result["efie"] = efie_bench(4, 2.0, 3)

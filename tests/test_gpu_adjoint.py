@@ -28,20 +28,27 @@ def _complex(rng, n):
     return rng.standard_normal(n) + 1j * rng.standard_normal(n)
 
 
+# (rows, cols, leading dimension): one partial column chunk and a column tail; several chunks of
+# the tiled product; a leading dimension whose rows are not 16-byte aligned (row-per-wave product)
+MATVEC_SHAPES = [(300, 257, 280), (100, 4500, 4500), (70, 2301, 2303)]
+
+
+@pytest.mark.parametrize("shape", MATVEC_SHAPES, ids=[f"{r}x{c}_ld{ld}" for r, c, ld in MATVEC_SHAPES])
 @pytest.mark.parametrize("dtype", ["float32", "float64", "complex64", "complex128"])
-def test_dense_matvec(dtype, hip_available):
+def test_dense_matvec(dtype, shape, hip_available):
     import torch
     from bempp_amd.core import hip_backend as hb
 
+    rows, cols, ld = shape
     g = torch.Generator().manual_seed(3)
     dt = getattr(torch, dtype)
-    full = torch.randn(300, 280, generator=g, dtype=torch.float64)
+    full = torch.randn(rows, ld, generator=g, dtype=torch.float64)
     if dt.is_complex:
-        full = full + 1j * torch.randn(300, 280, generator=g, dtype=torch.float64)
-    A = full.to(dt).cuda()[:, :257]  # leading dimension 280 > cols
+        full = full + 1j * torch.randn(rows, ld, generator=g, dtype=torch.float64)
+    A = full.to(dt).cuda()[:, :cols]  # leading dimension ld >= cols
     A64 = A.to(torch.complex128)
-    x = (torch.randn(257, dtype=torch.float64, generator=g) + 1j * torch.randn(257, dtype=torch.float64, generator=g)).cuda()
-    xt = (torch.randn(300, dtype=torch.float64, generator=g) + 1j * torch.randn(300, dtype=torch.float64, generator=g)).cuda()
+    x = (torch.randn(cols, dtype=torch.float64, generator=g) + 1j * torch.randn(cols, dtype=torch.float64, generator=g)).cuda()
+    xt = (torch.randn(rows, dtype=torch.float64, generator=g) + 1j * torch.randn(rows, dtype=torch.float64, generator=g)).cuda()
     ref = [A64 @ x, A64.T @ xt, A64.conj().T @ xt]
     got = [hb.dense_matvec(A, x, 0), hb.dense_matvec(A, xt, 1), hb.dense_matvec(A, xt, 2)]
     for r, y in zip(ref, got):
