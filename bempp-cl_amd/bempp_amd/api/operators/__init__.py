@@ -9,3 +9,4 @@ OperatorDescriptor = _collections.namedtuple(
 
 from . import boundary  # noqa: E402,F401
 from . import potential  # noqa: E402,F401
+from . import far_field  # noqa: E402,F401

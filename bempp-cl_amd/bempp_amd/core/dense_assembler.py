@@ -58,7 +58,8 @@ class DenseAssembler(object):
 
 
 def _kernel_name(descriptor):
-    if descriptor.assembly_type in ("maxwell_electric_field", "laplace_hypersingular", "helmholtz_hypersingular"):
+    if descriptor.assembly_type in ("maxwell_electric_field", "laplace_hypersingular", "helmholtz_hypersingular",
+                                    "maxwell_electric_far_field", "maxwell_magnetic_far_field"):
         return descriptor.assembly_type
     if descriptor.assembly_type != "default_scalar":
         raise KeyError(f"Unsupported assembly type {descriptor.assembly_type}")

@@ -1,4 +1,4 @@
-"""Dense potential evaluation on the device (bempp_cl/core/dense_potential_assembler.py:4-40).
+"""Dense potential and far-field evaluation on the device (bempp_cl/core/dense_potential_assembler.py:4-40).
 
 The evaluator maps space coefficients to the full grid (space.map_to_full_grid @ x, which
 applies the local multipliers, space.py:376-391), uploads them and the evaluation points,
@@ -31,7 +31,8 @@ def hip_potential_assembler(space, operator_descriptor, points, parameters):
 
     op = _kernel_name(operator_descriptor)
     if op not in ("laplace_single_layer", "laplace_double_layer", "helmholtz_single_layer",
-                  "helmholtz_double_layer", "maxwell_electric_field"):
+                  "helmholtz_double_layer", "maxwell_electric_field", "helmholtz_far_field_single_layer",
+                  "helmholtz_far_field_double_layer", "maxwell_electric_far_field", "maxwell_magnetic_far_field"):
         raise KeyError(f"Unsupported potential kernel {op}")
     dev = hb.device()
     dspace = hb.device_space(space)

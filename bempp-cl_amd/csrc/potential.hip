@@ -10,6 +10,9 @@
 //      staged through LDS in tiles of 256 and read as broadcasts; partial sums per chunk go
 //      to the workspace;
 //   3. reduce: sums the chunk partials in a fixed order (deterministic result).
+// Far fields (numba_kernels.py:402-449, 2437-2570): the targets are unit directions xh and
+// the kernel is K = e^{-i Re(k) xh.y} / (4 pi) (the reference puts Re k in the exponent); they
+// are evaluated in FP64 in both precision modes (far_field_eval_fp64).
 // MIXED: per (target, tile) the distance r0 = |x - c| to the tile centre c and e^{ik r0}
 // are formed in FP64; per source only r - r0 = u.(2v + u) / (r + r0), with v = x - c and
 // u = c - y ... small vectors, is needed, so the FP32 phase error stays ~k * eps * tile size
@@ -343,6 +346,94 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed(const PotArg
     }
 }
 
+// Far-field integrands per source (sources from potential_prep_kernel: c for scalar spaces,
+// t1 and t2 of the EFIE for RWG):
+//   Helmholtz single layer  K c
+//   Helmholtz double layer  -i Re(k) (xh . n_y) K c
+//   Maxwell electric field  K (ik t1 - xh t2)
+//   Maxwell magnetic field  xh x (K ik t1), the cross product taken once per direction
+enum FarField { kFarSL = 0, kFarDL = 1, kFarE = 2, kFarH = 3 };
+
+template <int FF>
+__global__ __launch_bounds__(kPotThreads) void far_field_eval_fp64(const PotArgs a) {
+    constexpr bool MAXWELL = FF == kFarE || FF == kFarH;
+    constexpr int KD = MAXWELL ? 3 : 1;
+    constexpr int NF = MAXWELL ? kSrcFields : 8;
+    __shared__ double sh[NF][kPotTile];
+    const int m = blockIdx.x * kPotThreads + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int s0 = chunk * a.chunk_len;
+    const int s1 = min(s0 + a.chunk_len, a.nsrc_pad);
+    double x0 = 0, x1 = 0, x2 = 0;
+    if (m < a.npts) {
+        x0 = a.pts[3 * (int64_t)m];
+        x1 = a.pts[3 * (int64_t)m + 1];
+        x2 = a.pts[3 * (int64_t)m + 2];
+    }
+    const double xh[3] = {x0, x1, x2};
+    double accr[KD], acci[KD];
+    for (int d = 0; d < KD; ++d) accr[d] = acci[d] = 0.0;
+    const double kr = a.kr, ki = a.ki;
+    for (int t0 = s0; t0 < s1; t0 += kPotTile) {
+        __syncthreads();
+        {
+            const int s = t0 + threadIdx.x;
+            // sources past the chunk: at the origin with zero weights
+            for (int k = 0; k < NF; ++k) sh[k][threadIdx.x] = s < s1 ? a.src[(int64_t)k * a.nsrc_pad + s] : 0.0;
+        }
+        __syncthreads();
+        const int nt = min(kPotTile, s1 - t0);
+        for (int j = 0; j < nt; ++j) {
+            // padding sources of the prep sit at 1e30 with zero weights: the phase stays finite
+            const double dot = x0 * sh[0][j] + x1 * sh[1][j] + x2 * sh[2][j];
+            double sn, cs;
+            sincos(-kr * dot, &sn, &cs);
+            const double gre = kInv4Pi * cs, gim = kInv4Pi * sn;
+            if constexpr (FF == kFarSL || FF == kFarDL) {
+                double vr = gre, vi = gim;
+                if constexpr (FF == kFarDL) {
+                    const double f = -kr * (x0 * sh[3][j] + x1 * sh[4][j] + x2 * sh[5][j]);
+                    vr = -f * gim;  // i f K
+                    vi = f * gre;
+                }
+                const double cr = sh[6][j], ci = sh[7][j];
+                accr[0] += vr * cr - vi * ci;
+                acci[0] += vr * ci + vi * cr;
+            } else {
+                const double t2r = sh[6][j], t2i = sh[7][j];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double t1r = sh[8 + 2 * d][j], t1i = sh[9 + 2 * d][j];
+                    // ik t1 with ik = -ki + i kr
+                    double tr = -ki * t1r - kr * t1i, ti = -ki * t1i + kr * t1r;
+                    if constexpr (FF == kFarE) {
+                        tr -= xh[d] * t2r;
+                        ti -= xh[d] * t2i;
+                    }
+                    accr[d] += gre * tr - gim * ti;
+                    acci[d] += gre * ti + gim * tr;
+                }
+            }
+        }
+    }
+    if (m < a.npts) {
+        if constexpr (FF == kFarH) {
+            const double vr[3] = {accr[0], accr[1], accr[2]}, vi[3] = {acci[0], acci[1], acci[2]};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+                accr[d] = xh[d1] * vr[d2] - xh[d2] * vr[d1];
+                acci[d] = xh[d1] * vi[d2] - xh[d2] * vi[d1];
+            }
+        }
+        for (int d = 0; d < KD; ++d) {
+            double *p = a.partial + 2 * (((int64_t)chunk * KD + d) * a.npts + m);
+            p[0] = accr[d];
+            p[1] = acci[d];
+        }
+    }
+}
+
 template <typename OutT>
 __global__ void potential_reduce_kernel(const PotArgs a, int kd) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -371,19 +462,32 @@ static void plan(int npts, int nsrc, int *nchunk, int *chunk_len, int *nsrc_pad)
     *nchunk = (*nsrc_pad + *chunk_len - 1) / *chunk_len;
 }
 
-static bool is_potential_op(int op) {
-    return op == BEM_LAPLACE_SINGLE_LAYER || op == BEM_LAPLACE_DOUBLE_LAYER || op == BEM_HELMHOLTZ_SINGLE_LAYER ||
-           op == BEM_HELMHOLTZ_DOUBLE_LAYER || op == BEM_MAXWELL_ELECTRIC_FIELD;
+static bool is_far_field_op(int op) {
+    return op == BEM_HELMHOLTZ_FAR_FIELD_SINGLE_LAYER || op == BEM_HELMHOLTZ_FAR_FIELD_DOUBLE_LAYER ||
+           op == BEM_MAXWELL_ELECTRIC_FAR_FIELD || op == BEM_MAXWELL_MAGNETIC_FAR_FIELD;
 }
 
-template <int OP>
+static bool is_potential_op(int op) {
+    return op == BEM_LAPLACE_SINGLE_LAYER || op == BEM_LAPLACE_DOUBLE_LAYER || op == BEM_HELMHOLTZ_SINGLE_LAYER ||
+           op == BEM_HELMHOLTZ_DOUBLE_LAYER || op == BEM_MAXWELL_ELECTRIC_FIELD || is_far_field_op(op);
+}
+
+// the operators whose sources are RWG fields (t1, t2) and whose values are 3-vectors
+static bool is_vector_potential_op(int op) {
+    return op == BEM_MAXWELL_ELECTRIC_FIELD || op == BEM_MAXWELL_ELECTRIC_FAR_FIELD || op == BEM_MAXWELL_MAGNETIC_FAR_FIELD;
+}
+
+// FF < 0: the potential of OP; FF >= 0: far field FF (sources prepared as for OP)
+template <int OP, int FF = -1>
 static int run_potential(PotArgs &a, int precision, int out_dtype, hipStream_t s) {
     int kd = OP == kMaxwellEFIE ? 3 : 1;
     hipLaunchKernelGGL((potential_prep_kernel<OP>), dim3((a.nsrc_pad + 255) / 256), dim3(256), 0, s, a);
     int rc = check_launch("potential_prep_kernel");
     if (rc) return rc;
     dim3 grid((a.npts + kPotThreads - 1) / kPotThreads, a.nchunk);
-    if (precision == BEM_PRECISION_MIXED)
+    if constexpr (FF >= 0)
+        hipLaunchKernelGGL((far_field_eval_fp64<FF>), grid, dim3(kPotThreads), 0, s, a);
+    else if (precision == BEM_PRECISION_MIXED)
         hipLaunchKernelGGL((potential_eval_mixed<OP>), grid, dim3(kPotThreads), 0, s, a);
     else
         hipLaunchKernelGGL((potential_eval_fp64<OP>), grid, dim3(kPotThreads), 0, s, a);
@@ -406,7 +510,7 @@ extern "C" size_t bem_potential_workspace_size(int op, const bem_space_t *space,
     if (!space || !rule || n_points < 0) return 0;
     int nsrc = space->n_elements * rule->n_points, nchunk, chunk_len, nsrc_pad;
     plan(n_points, nsrc, &nchunk, &chunk_len, &nsrc_pad);
-    int kd = op == BEM_MAXWELL_ELECTRIC_FIELD ? 3 : 1;
+    int kd = is_vector_potential_op(op) ? 3 : 1;
     return sizeof(double) * ((size_t)kSrcFields * nsrc_pad + (size_t)2 * nchunk * kd * n_points) + 256;
 }
 
@@ -425,8 +529,8 @@ extern "C" int bem_potential_evaluate(int op, const bem_space_t *space, const vo
     }
     int rc;
     if ((rc = check_space(space, "potential"))) return rc;
-    if ((op == BEM_MAXWELL_ELECTRIC_FIELD) != (space->shapeset == BEM_SHAPE_RWG)) {
-        set_error("the electric field potential needs an RWG space, scalar potentials a scalar space");
+    if (is_vector_potential_op(op) != (space->shapeset == BEM_SHAPE_RWG)) {
+        set_error("the Maxwell potentials and far fields need an RWG space, scalar ones a scalar space");
         return BEM_ERR_INVALID_ARGUMENT;
     }
     if (out_dtype != BEM_C128 && out_dtype != BEM_C64) {
@@ -463,6 +567,10 @@ extern "C" int bem_potential_evaluate(int op, const bem_space_t *space, const vo
     case BEM_LAPLACE_DOUBLE_LAYER: return run_potential<kLaplaceDL>(a, precision, out_dtype, s);
     case BEM_HELMHOLTZ_SINGLE_LAYER: return run_potential<kHelmholtzSL>(a, precision, out_dtype, s);
     case BEM_HELMHOLTZ_DOUBLE_LAYER: return run_potential<kHelmholtzDL>(a, precision, out_dtype, s);
+    case BEM_HELMHOLTZ_FAR_FIELD_SINGLE_LAYER: return run_potential<kHelmholtzSL, kFarSL>(a, precision, out_dtype, s);
+    case BEM_HELMHOLTZ_FAR_FIELD_DOUBLE_LAYER: return run_potential<kHelmholtzSL, kFarDL>(a, precision, out_dtype, s);
+    case BEM_MAXWELL_ELECTRIC_FAR_FIELD: return run_potential<kMaxwellEFIE, kFarE>(a, precision, out_dtype, s);
+    case BEM_MAXWELL_MAGNETIC_FAR_FIELD: return run_potential<kMaxwellEFIE, kFarH>(a, precision, out_dtype, s);
     default: return run_potential<kMaxwellEFIE>(a, precision, out_dtype, s);
     }
 }

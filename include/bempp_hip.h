@@ -47,7 +47,13 @@ enum bem_operator {
     /* P1 x P1 only: the single layer's Green's function with the surface-curl weights of
        bempp_cl/core/numba_kernels.py:1078-1333 (regular) and 1530-1697 (singular) */
     BEM_LAPLACE_HYPERSINGULAR = 7,
-    BEM_HELMHOLTZ_HYPERSINGULAR = 8
+    BEM_HELMHOLTZ_HYPERSINGULAR = 8,
+    /* far fields, for bem_potential_evaluate only (the points are unit directions):
+       bempp_cl/core/numba_kernels.py:402-449 (Helmholtz) and 2437-2570 (Maxwell) */
+    BEM_HELMHOLTZ_FAR_FIELD_SINGLE_LAYER = 9,
+    BEM_HELMHOLTZ_FAR_FIELD_DOUBLE_LAYER = 10,
+    BEM_MAXWELL_ELECTRIC_FAR_FIELD = 11,
+    BEM_MAXWELL_MAGNETIC_FAR_FIELD = 12
 };
 
 /* reference shapesets (bempp_cl/api/space/shapesets.py:46-158) */

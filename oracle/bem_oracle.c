@@ -20,6 +20,7 @@
  *                       1530-1697 (hypersingular),
  *                       bempp_cl/core/singular_assembler.py:69-448, dense_assembler.py:78-98
  *   potentials          bempp_cl/core/numba_kernels.py:1788-1849 (scalar), 2281-2359 (EFIE)
+ *   far fields          bempp_cl/core/numba_kernels.py:402-449 (Helmholtz), 2437-2570 (Maxwell)
  *   Piola / edge length bempp_cl/core/numba_kernels.py:96-133
  *   colouring           bempp_cl/api/space/space.py:704-733 (per-colour parallel loop,
  *                       bempp_cl/core/numba_assemblers.py:90-112)
@@ -35,7 +36,7 @@
 #endif
 
 enum { OP_LAPLACE_SL = 0, OP_LAPLACE_DL, OP_LAPLACE_ADL, OP_HELMHOLTZ_SL, OP_HELMHOLTZ_DL, OP_HELMHOLTZ_ADL, OP_MAXWELL_EFIE,
-       OP_LAPLACE_HS, OP_HELMHOLTZ_HS };
+       OP_LAPLACE_HS, OP_HELMHOLTZ_HS, OP_HELMHOLTZ_FF_SL, OP_HELMHOLTZ_FF_DL, OP_MAXWELL_E_FF, OP_MAXWELL_H_FF };
 enum { SHAPE_P0 = 0, SHAPE_P1 = 1, SHAPE_RWG = 2 };
 
 #define M_INV_4PI_ 0.079577471545947667884
@@ -365,7 +366,7 @@ static void regular_pair(int op, const Elem *te, const Elem *se, double tnm, dou
                     loc[i][j].re += c.re; loc[i][j].im += c.im;
                 }
             } else if (hs) {
-                double psi[3]; scalar_shape(sshape, qx[q], qy[q], psi);
+                double psi[3] = {0, 0, 0}; scalar_shape(sshape, qx[q], qy[q], psi);
                 for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
                     cplx c = cmul(g, hs_weight(op, ct, cs, tn, sn, phi, psi, i, j, kr, ki));
                     loc[i][j].re += c.re; loc[i][j].im += c.im;
@@ -567,8 +568,10 @@ int bem_oracle_potential(int op, int nv, const double *vtx, int nf, const int32_
     if (nq < 0) return -1;
     Elem *G = malloc(sizeof(Elem) * nf);
     build_elements(nf, vtx, elm, G);
-    int nsh = op == OP_MAXWELL_EFIE ? 3 : nshape_of(shape);
-    int kdim = op == OP_MAXWELL_EFIE ? 3 : 1;
+    const int maxwell = op == OP_MAXWELL_EFIE || op == OP_MAXWELL_E_FF || op == OP_MAXWELL_H_FF;
+    const int far = op == OP_HELMHOLTZ_FF_SL || op == OP_HELMHOLTZ_FF_DL || op == OP_MAXWELL_E_FF || op == OP_MAXWELL_H_FF;
+    int nsh = maxwell ? 3 : nshape_of(shape);
+    int kdim = maxwell ? 3 : 1;
     int nsrc = 0;
     for (int e = 0; e < nf; ++e) if (sup[e]) ++nsrc;
     nsrc *= nq;
@@ -582,7 +585,7 @@ int bem_oracle_potential(int op, int nv, const double *vtx, int nf, const int32_
             for (int d = 0; d < 3; ++d) sn[3 * s + d] = G[e].n[d] * nm[e];
             t2[s].re = t2[s].im = 0;
             for (int d = 0; d < 3; ++d) t1[3 * s + d].re = t1[3 * s + d].im = 0;
-            if (op == OP_MAXWELL_EFIE) {
+            if (maxwell) {
                 double bv[3][3]; rwg_piola(G + e, qx[q], qy[q], bv);
                 for (int i = 0; i < 3; ++i) {
                     double f = qw[q] * G[e].len[i];
@@ -610,6 +613,39 @@ int bem_oracle_potential(int op, int nv, const double *vtx, int nf, const int32_
         const double *x = pts + 3 * m;
         cplx acc[3] = {{0, 0}, {0, 0}, {0, 0}};
         for (int j = 0; j < nsrc; ++j) {
+            if (far) {
+                /* K = e^{-i Re(k) x.y} / (4 pi), x a unit direction (numba_kernels.py:402-420) */
+                const double *y = sp + 3 * j;
+                double dot = x[0] * y[0] + x[1] * y[1] + x[2] * y[2];
+                cplx K = {M_INV_4PI_ * cos(-kr * dot), M_INV_4PI_ * sin(-kr * dot)};
+                if (op == OP_HELMHOLTZ_FF_SL) {
+                    cplx c = cmul(K, t2[j]);
+                    acc[0].re += c.re; acc[0].im += c.im;
+                } else if (op == OP_HELMHOLTZ_FF_DL) {
+                    /* factor = -Re(k) x.n_y; value = (-factor sin(-k x.y), factor cos(-k x.y)) / (4 pi) (424-449) */
+                    const double *n = sn + 3 * j;
+                    double factor = -kr * (x[0] * n[0] + x[1] * n[1] + x[2] * n[2]);
+                    cplx v = {-factor * M_INV_4PI_ * sin(-kr * dot), factor * M_INV_4PI_ * cos(-kr * dot)};
+                    cplx c = cmul(v, t2[j]);
+                    acc[0].re += c.re; acc[0].im += c.im;
+                } else if (op == OP_MAXWELL_E_FF) {
+                    /* K (ik t1 - x t2) (2437-2500) */
+                    for (int d = 0; d < 3; ++d) {
+                        cplx a = cmul(ik, t1[3 * j + d]);
+                        cplx term = {a.re - x[d] * t2[j].re, a.im - x[d] * t2[j].im};
+                        cplx c = cmul(K, term);
+                        acc[d].re += c.re; acc[d].im += c.im;
+                    }
+                } else {
+                    /* x cross (K ik t1) (2504-2570) */
+                    cplx v[3];
+                    for (int d = 0; d < 3; ++d) v[d] = cmul(K, cmul(ik, t1[3 * j + d]));
+                    acc[0].re += x[1] * v[2].re - x[2] * v[1].re; acc[0].im += x[1] * v[2].im - x[2] * v[1].im;
+                    acc[1].re += x[2] * v[0].re - x[0] * v[2].re; acc[1].im += x[2] * v[0].im - x[0] * v[2].im;
+                    acc[2].re += x[0] * v[1].re - x[1] * v[0].re; acc[2].im += x[0] * v[1].im - x[1] * v[0].im;
+                }
+                continue;
+            }
             cplx g = kernel_eval(op, x, sp + 3 * j, zero, sn + 3 * j, kr, ki);
             if (op == OP_MAXWELL_EFIE) {
                 double diff[3] = {x[0] - sp[3 * j], x[1] - sp[3 * j + 1], x[2] - sp[3 * j + 2]};

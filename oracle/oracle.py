@@ -28,6 +28,10 @@ OPS = {
     "maxwell_electric_field": 6,
     "laplace_hypersingular": 7,
     "helmholtz_hypersingular": 8,
+    "helmholtz_far_field_single_layer": 9,
+    "helmholtz_far_field_double_layer": 10,
+    "maxwell_electric_far_field": 11,
+    "maxwell_magnetic_far_field": 12,
 }
 
 SHAPES = {"p0_discontinuous": 0, "p1_discontinuous": 1, "rwg0": 2, "snc0": 2}
@@ -132,7 +136,7 @@ def potential(op, vertices, elements, space, coefficients, points, wavenumber=0.
     full[sup] = coeffs[space.local2global[sup]] * space.local_multipliers[sup]
     cbuf = np.ascontiguousarray(np.stack([full.real, full.imag], axis=-1))
     pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).T)
-    kdim = 3 if op == "maxwell_electric_field" else 1
+    kdim = 3 if op.startswith("maxwell") else 1
     out = np.zeros((kdim, pts.shape[0], 2))
     k = complex(wavenumber)
     rc = lib().bem_oracle_potential(

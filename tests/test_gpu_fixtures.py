@@ -106,6 +106,51 @@ def test_potential_fixture(case, precision, helpers, default_parameters, sphere_
     np.testing.assert_allclose(actual, data["result"], rtol=helpers.default_tolerance(precision))
 
 
+FAR_FIELDS = [
+    ("helmholtz_single_layer_far_field_p1", "helmholtz.single_layer", ("P", 1)),
+    ("helmholtz_double_layer_far_field_p1", "helmholtz.double_layer", ("P", 1)),
+    ("maxwell_electric_far_field", "maxwell.electric_field", ("RWG", 0)),
+    ("maxwell_magnetic_far_field", "maxwell.magnetic_field", ("RWG", 0)),
+]
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+@pytest.mark.parametrize("case", FAR_FIELDS, ids=[c[0] for c in FAR_FIELDS])
+def test_far_field_fixture(case, precision, helpers, default_parameters, sphere_grid, hip_available):
+    """test/validation/operators/far_field/test_{helmholtz,maxwell}_far_field.py (k = 2.5)."""
+    import bempp_amd.api.operators.far_field as FF
+    from bempp_amd.api import GridFunction, function_space
+
+    name, op, sk = case
+    space = function_space(sphere_grid, *sk)
+    data = helpers.load_npz_data(name)
+    mod, fn = op.split(".")
+    ff = getattr(getattr(FF, mod), fn)(space, data["points"], 2.5, parameters=default_parameters,
+                                       device_interface="hip", precision=precision)
+    actual = ff.evaluate(GridFunction(space, coefficients=data["vec"]))
+    np.testing.assert_allclose(actual, data["result"], rtol=helpers.default_tolerance(precision))
+
+
+@pytest.mark.parametrize("case", FAR_FIELDS, ids=[c[0] for c in FAR_FIELDS])
+def test_far_field_complex_coefficients(case, helpers, default_parameters, sphere_grid, hip_available):
+    """Linearity in the density (test_helmholtz_far_field.py:132-168, test_maxwell_far_field.py:134-170)."""
+    import bempp_amd.api.operators.far_field as FF
+    from bempp_amd.api import GridFunction, function_space
+
+    name, op, sk = case
+    space = function_space(sphere_grid, *sk)
+    rng = np.random.default_rng(9)
+    c = rng.standard_normal(space.global_dof_count) + 1j * rng.standard_normal(space.global_dof_count)
+    pts = rng.standard_normal((3, 50))
+    pts /= np.linalg.norm(pts, axis=0)
+    mod, fn = op.split(".")
+    ff = getattr(getattr(FF, mod), fn)(space, pts, 2.5, parameters=default_parameters)
+    whole = ff.evaluate(GridFunction(space, coefficients=c))
+    parts = ff.evaluate(GridFunction(space, coefficients=c.real)) + 1j * ff.evaluate(
+        GridFunction(space, coefficients=c.imag))
+    np.testing.assert_allclose(whole, parts, rtol=1e-12, atol=1e-14 * np.abs(whole).max())
+
+
 @pytest.mark.parametrize("precision", PRECISIONS)
 def test_laplace_single_element_p0(precision, default_parameters, hip_available):
     """Closed-form value 0.0798 on the unit right triangle (test_laplace_boundary.py:196-226)."""

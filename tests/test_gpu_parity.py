@@ -75,6 +75,34 @@ def test_hypersingular_swapped_normals_vs_oracle(fam, k, ico2, default_parameter
     assert rel_fro(A32.astype(A64.dtype), A64) < 1e-6
 
 
+FAR = [("helmholtz", "single_layer", ("P", 1), 3.0), ("helmholtz", "double_layer", ("DP", 0), 3.0 + 0.5j),
+       ("maxwell", "electric_field", ("RWG", 0), 2.0 + 0.3j), ("maxwell", "magnetic_field", ("RWG", 0), 2.0)]
+
+
+@pytest.mark.parametrize("case", FAR, ids=[f"{c[0]}_{c[1]}" for c in FAR])
+def test_far_fields_vs_oracle(case, ico2, default_parameters, hip_available):
+    """200 directions: FP64 vs the oracle within 1e-12; complex64 output within 1e-6 of it."""
+    import bempp_amd.api.operators.far_field as FF
+    from bempp_amd.api import GridFunction, function_space
+    from oracle import oracle as O
+
+    fam, fn, sk, k = case
+    space = function_space(ico2, *sk)
+    rng = np.random.default_rng(21)
+    c = rng.standard_normal(space.global_dof_count) + 1j * rng.standard_normal(space.global_dof_count)
+    pts = rng.standard_normal((3, 200))
+    pts /= np.linalg.norm(pts, axis=0)
+    ref = O.potential(f"{fam}_far_field_{fn}" if fam == "helmholtz" else f"maxwell_{fn.split('_')[0]}_far_field",
+                      ico2.vertices, ico2.elements, space, c, pts, k)
+    factory = getattr(getattr(FF, fam), fn)
+    u64 = factory(space, pts, k, parameters=default_parameters, precision="double").evaluate(
+        GridFunction(space, coefficients=c))
+    assert rel_fro(u64, ref) < 1e-12
+    u32 = factory(space, pts, k, parameters=default_parameters, precision="single").evaluate(
+        GridFunction(space, coefficients=c))
+    assert u32.dtype == np.complex64 and rel_fro(u32.astype(np.complex128), ref) < 1e-6
+
+
 def test_maxwell_vs_oracle(ico2, default_parameters, hip_available):
     from bempp_amd.api import function_space
     from bempp_amd.api.operators.boundary.maxwell import electric_field

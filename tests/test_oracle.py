@@ -4,7 +4,8 @@ The oracle (oracle/bem_oracle.c) restates the reference's Numba algorithm; these
 it before it is trusted as the checker of the HIP library:
 * every dense boundary fixture on sphere.msh at the reference's tolerances
   (test/validation/operators/boundary/*.py, test/conftest.py:168-174);
-* every potential fixture (test/validation/operators/potential/*.py);
+* every potential fixture (test/validation/operators/potential/*.py) and the far-field fixtures
+  of real wavenumber (test/validation/operators/far_field/*.py);
 * the closed-form 0.0798 on the unit right triangle (test_laplace_boundary.py:196-226);
 * the Laplace single layer on the unit sphere has eigenvalue 1 for constants (V 1 = area
   up to discretisation), the Helmholtz one i k j0(k) h0(k) (spherical Bessel/Hankel),
@@ -67,7 +68,17 @@ POTENTIALS = [
 ]
 
 
-@pytest.mark.parametrize("case", POTENTIALS, ids=[c[0] for c in POTENTIALS])
+# far fields (test/validation/operators/far_field/*.py); the reference's kernels use Re k only, and
+# its tests use the fixtures of real wavenumber k = 2.5
+FAR_FIELDS = [
+    ("helmholtz_single_layer_far_field_p1", "helmholtz_far_field_single_layer", ("P", 1), 2.5),
+    ("helmholtz_double_layer_far_field_p1", "helmholtz_far_field_double_layer", ("P", 1), 2.5),
+    ("maxwell_electric_far_field", "maxwell_electric_far_field", ("RWG", 0), 2.5),
+    ("maxwell_magnetic_far_field", "maxwell_magnetic_far_field", ("RWG", 0), 2.5),
+]
+
+
+@pytest.mark.parametrize("case", POTENTIALS + FAR_FIELDS, ids=[c[0] for c in POTENTIALS + FAR_FIELDS])
 def test_oracle_potential_fixtures(case, sphere_grid, helpers):
     from bempp_amd.api import function_space
 
