@@ -59,7 +59,7 @@ class DenseAssembler(object):
 
 def _kernel_name(descriptor):
     if descriptor.assembly_type in ("maxwell_electric_field", "laplace_hypersingular", "helmholtz_hypersingular",
-                                    "maxwell_electric_far_field", "maxwell_magnetic_far_field"):
+                                    "maxwell_electric_far_field", "maxwell_magnetic_far_field", "maxwell_magnetic_field"):
         return descriptor.assembly_type
     if descriptor.assembly_type != "default_scalar":
         raise KeyError(f"Unsupported assembly type {descriptor.assembly_type}")
@@ -139,7 +139,13 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     )
     if events is not None:
         events["regular_start"].record()
-    if upper:
+    # the magnetic field operator: FP64 kernels in both modes that add to the (zeroed) output, no
+    # dof blocks and no near-field split
+    mfie = op == "maxwell_magnetic_field"
+    if mfie:
+        hb.regular_assemble(op, test, trial, grids_identical, rule, _wavenumber(operator_descriptor), mode, out,
+                            accumulate=1, near_factor=near, kahan=kahan)
+    elif upper:
         hb.regular_assemble_symmetric(op, test, rule, _wavenumber(operator_descriptor), mode, out, near_factor=near,
                                       kahan=kahan)
     elif direct:
@@ -159,7 +165,7 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
         )
     if events is not None:
         events["regular_end"].record()
-    if mode == hb.PRECISION_MIXED:
+    if mode == hb.PRECISION_MIXED and not mfie:
         from bempp_amd.core.near_field import device_near_candidates
 
         cand = device_near_candidates(dual_to_range, domain, near)

@@ -32,7 +32,8 @@ def hip_potential_assembler(space, operator_descriptor, points, parameters):
     op = _kernel_name(operator_descriptor)
     if op not in ("laplace_single_layer", "laplace_double_layer", "helmholtz_single_layer",
                   "helmholtz_double_layer", "maxwell_electric_field", "helmholtz_far_field_single_layer",
-                  "helmholtz_far_field_double_layer", "maxwell_electric_far_field", "maxwell_magnetic_far_field"):
+                  "helmholtz_far_field_double_layer", "maxwell_electric_far_field", "maxwell_magnetic_far_field",
+                  "maxwell_magnetic_field"):
         raise KeyError(f"Unsupported potential kernel {op}")
     dev = hb.device()
     dspace = hb.device_space(space)

@@ -29,6 +29,7 @@ OPERATORS = {
     "helmholtz_far_field_double_layer": 10,
     "maxwell_electric_far_field": 11,
     "maxwell_magnetic_far_field": 12,
+    "maxwell_magnetic_field": 13,
 }
 SHAPESETS = {"p0_discontinuous": 0, "p1_discontinuous": 1, "rwg0": 2, "snc0": 2}
 DTYPES = {"float32": 0, "float64": 1, "complex64": 2, "complex128": 3}

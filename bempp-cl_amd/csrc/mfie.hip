@@ -1,0 +1,279 @@
+// Maxwell magnetic field boundary operator (MFIE), RWG x SNC, for gfx950.
+//
+// V_ij = l_i l_j sum_pq w_p w_q G(r) (ik r - 1) / r^2 (x_p - y_q) . (A_i(p) x B_j(q))
+// (bempp_cl/core/numba_kernels.py:2151-2277 regular, 2065-2147 singular), with G the Helmholtz
+// single-layer kernel e^{ikr} / (4 pi r), A_i = E_t (xi_p - o_i) and B_j = E_f (eta_q - o_j) the
+// Piola-mapped RWG functions times their Jacobians (the J_t J_f of the quadrature cancels the
+// 1/J of the two Piola maps), o = ((0, 1), (1, 0), (0, 0)) and l the edge lengths.
+// Both kernels evaluate in FP64 (also for complex64 output) and add to the output with atomics:
+//  * regular: one lane per trial element, test elements staged in LDS in tiles of 16; pairs that
+//    share a vertex are left to the singular kernel (grids identical);
+//  * singular: one 64-lane wavefront per Sauter-Schwab pair, its Duffy points strided over the
+//    lanes, the 3 x 3 block reduced across the wave.
+#include "bem_device.h"
+#include "bem_internal.h"
+
+namespace bem {
+namespace mfie {
+
+constexpr int kThreads = 256;
+constexpr int kTile = 16;  // test elements per workgroup
+constexpr int kQ = 6;      // quadrature points of the regular kernel (the order-4 rule)
+constexpr double kOx[3] = {0.0, 1.0, 0.0}, kOy[3] = {1.0, 0.0, 0.0};
+
+struct Args {
+    SpaceArgs test, trial;
+    RuleArgs rule;
+    double kr, ki;
+    int identical;
+    void *out;
+    int64_t ld;
+    // singular pairs
+    int n_pairs;
+    const int32_t *pairs;
+    const double *tpts, *spts, *wts;
+};
+
+// edge lengths |v1 - v0|, |v2 - v0|, |v1 - v2| of a geometry row
+__device__ __forceinline__ void edge_lengths(const double *g, double (&l)[3]) {
+    l[0] = sqrt(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]);
+    l[1] = sqrt(g[6] * g[6] + g[7] * g[7] + g[8] * g[8]);
+    const double d0 = g[3] - g[6], d1 = g[4] - g[7], d2 = g[5] - g[8];
+    l[2] = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+}
+
+// E (xi - o_i, eta - o'_i) for the three RWG functions of an element (edge vectors e1, e2)
+__device__ __forceinline__ void piola_j(const double *e1, const double *e2, double xi, double eta, double (&A)[3][3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) A[i][d] = e1[d] * (xi - kOx[i]) + e2[d] * (eta - kOy[i]);
+}
+
+// acc[i][j] += wgt G (ik r - 1) / r^2 (x - y) . (A_i x B_j), complex wavenumber k = kr + i ki
+__device__ __forceinline__ void accumulate(const double (&diff)[3], const double (&A)[3][3], const double (&B)[3][3],
+                                           double wgt, double kr, double ki, double (&ar)[3][3], double (&ai)[3][3]) {
+    const double r2 = diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2];
+    const double r = sqrt(r2);
+    double sn, cs;
+    sincos(kr * r, &sn, &cs);
+    const double damp = ki != 0.0 ? exp(-ki * r) : 1.0;
+    const double g = wgt * kInv4Pi * damp / (r * r2);  // |G| / r^2 times the weight
+    // G (ik r - 1) / r^2 = g e^{i kr r} ((-ki r - 1) + i kr r)
+    const double fr = -ki * r - 1.0, fi = kr * r;
+    const double vr = g * (cs * fr - sn * fi), vi = g * (cs * fi + sn * fr);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        // D = diff x A_i, then D . B_j = diff . (A_i x B_j)
+        const double D0 = diff[1] * A[i][2] - diff[2] * A[i][1];
+        const double D1 = diff[2] * A[i][0] - diff[0] * A[i][2];
+        const double D2 = diff[0] * A[i][1] - diff[1] * A[i][0];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double s = D0 * B[j][0] + D1 * B[j][1] + D2 * B[j][2];
+            ar[i][j] += vr * s;
+            ai[i][j] += vi * s;
+        }
+    }
+}
+
+template <typename OutT>
+__device__ __forceinline__ void scatter(const Args &a, int e, int f, const double *gt, const double *gf,
+                                        const double (&ar)[3][3], const double (&ai)[3][3], int i, int j) {
+    double lt[3], lf[3];
+    edge_lengths(gt, lt);
+    edge_lengths(gf, lf);
+    const double m = a.test.mult[3 * (int64_t)e + i] * a.trial.mult[3 * (int64_t)f + j] * lt[i] * lf[j];
+    if (m == 0.0) return;
+    const int64_t row = a.test.l2g[3 * (int64_t)e + i], col = a.trial.l2g[3 * (int64_t)f + j];
+    OutT *o = reinterpret_cast<OutT *>(a.out) + 2 * (row * a.ld + col);
+    atomic_add(o, (OutT)(ar[i][j] * m));
+    atomic_add(o + 1, (OutT)(ai[i][j] * m));
+}
+
+struct Tile {
+    int elem[kTile];
+    int vid[kTile][3];
+    double x[kTile][kQ][3];
+    double e1[kTile][3], e2[kTile][3];
+};
+
+template <typename OutT>
+__global__ __launch_bounds__(kThreads) void mfie_regular_kernel(const Args a) {
+    __shared__ Tile T_;
+    const int tid = threadIdx.x;
+    const int jf = blockIdx.x * kThreads + tid;
+    const bool valid = jf < a.trial.n;
+    const int t0 = blockIdx.y * kTile;
+    const int ntile = min(kTile, a.test.n - t0);
+    if (tid < ntile) {
+        const int e = a.test.elements[t0 + tid];
+        const double *g = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
+        T_.elem[tid] = e;
+        for (int i = 0; i < 3; ++i) T_.vid[tid][i] = a.test.vids[4 * e + i];
+        for (int d = 0; d < 3; ++d) {
+            T_.e1[tid][d] = g[3 + d];
+            T_.e2[tid][d] = g[6 + d];
+        }
+        for (int p = 0; p < kQ; ++p)
+            for (int d = 0; d < 3; ++d) T_.x[tid][p][d] = g[d] + g[3 + d] * a.rule.x[p] + g[6 + d] * a.rule.y[p];
+    }
+    __syncthreads();
+    if (!valid) return;
+    const int f = a.trial.elements[jf];
+    const double *gf = a.trial.geom + (int64_t)f * BEM_GEOM_STRIDE;
+    const int fv0 = a.trial.vids[4 * f], fv1 = a.trial.vids[4 * f + 1], fv2 = a.trial.vids[4 * f + 2];
+    double yq[kQ][3];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) yq[q][d] = gf[d] + gf[3 + d] * a.rule.x[q] + gf[6 + d] * a.rule.y[q];
+    for (int t = 0; t < ntile; ++t) {
+        if (a.identical) {
+            bool adj = false;
+            for (int i = 0; i < 3; ++i) {
+                const int v = T_.vid[t][i];
+                adj = adj || v == fv0 || v == fv1 || v == fv2;
+            }
+            if (adj) continue;
+        }
+        double ar[3][3], ai[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) ar[i][j] = ai[i][j] = 0.0;
+#pragma unroll 1
+        for (int p = 0; p < kQ; ++p) {
+            double A[3][3];
+            piola_j(T_.e1[t], T_.e2[t], a.rule.x[p], a.rule.y[p], A);
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                double B[3][3];
+                piola_j(gf + 3, gf + 6, a.rule.x[q], a.rule.y[q], B);
+                const double diff[3] = {T_.x[t][p][0] - yq[q][0], T_.x[t][p][1] - yq[q][1], T_.x[t][p][2] - yq[q][2]};
+                accumulate(diff, A, B, a.rule.w[p] * a.rule.w[q], a.kr, a.ki, ar, ai);
+            }
+        }
+        const int e = T_.elem[t];
+        const double *gt = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) scatter<OutT>(a, e, f, gt, gf, ar, ai, i, j);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <typename OutT>
+__global__ __launch_bounds__(kThreads) void mfie_singular_kernel(const Args a) {
+    const int lane = threadIdx.x & 63;
+    const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (pair >= a.n_pairs) return;  // uniform per wavefront
+    const int32_t *pd = a.pairs + 6 * (int64_t)pair;
+    const int e = pd[0], f = pd[1], toff = pd[2], soff = pd[3], woff = pd[4], np = pd[5];
+    const double *gt = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
+    const double *gf = a.trial.geom + (int64_t)f * BEM_GEOM_STRIDE;
+    double ar[3][3], ai[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) ar[i][j] = ai[i][j] = 0.0;
+    for (int k = lane; k < np; k += 64) {
+        const double xi0 = a.tpts[2 * (int64_t)(toff + k)], xi1 = a.tpts[2 * (int64_t)(toff + k) + 1];
+        const double et0 = a.spts[2 * (int64_t)(soff + k)], et1 = a.spts[2 * (int64_t)(soff + k) + 1];
+        double A[3][3], B[3][3];
+        piola_j(gt + 3, gt + 6, xi0, xi1, A);
+        piola_j(gf + 3, gf + 6, et0, et1, B);
+        double diff[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            diff[d] = (gt[d] + gt[3 + d] * xi0 + gt[6 + d] * xi1) - (gf[d] + gf[3 + d] * et0 + gf[6 + d] * et1);
+        accumulate(diff, A, B, a.wts[woff + k], a.kr, a.ki, ar, ai);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            ar[i][j] = wave_sum(ar[i][j]);
+            ai[i][j] = wave_sum(ai[i][j]);
+        }
+    // lanes 0..8 add one entry each (compile-time indices into the register blocks)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            if (lane == 3 * i + j) scatter<OutT>(a, e, f, gt, gf, ar, ai, i, j);
+}
+
+}  // namespace mfie
+
+// regular pairs of the MFIE (FP64; out_dtype BEM_C64 or BEM_C128), added to the output
+int mfie_regular_launch(const SpaceArgs &test, const SpaceArgs &trial, const RuleArgs &rule, double kr, double ki,
+                        int identical, void *out, int64_t ld, int out_dtype, hipStream_t s) {
+    if (rule.n != mfie::kQ) {
+        set_error("the magnetic field operator uses the 6-point regular rule (order 4), got %d points", rule.n);
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (test.n == 0 || trial.n == 0) return BEM_OK;
+    const int64_t ntiles = (test.n + mfie::kTile - 1) / mfie::kTile;
+    if (ntiles > 65535) {
+        set_error("too many test tiles for one launch (%lld)", (long long)ntiles);
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    mfie::Args a{};
+    a.test = test;
+    a.trial = trial;
+    a.rule = rule;
+    a.kr = kr;
+    a.ki = ki;
+    a.identical = identical;
+    a.out = out;
+    a.ld = ld;
+    const dim3 grid((trial.n + mfie::kThreads - 1) / mfie::kThreads, (unsigned)ntiles);
+    if (out_dtype == BEM_C128)
+        hipLaunchKernelGGL(mfie::mfie_regular_kernel<double>, grid, dim3(mfie::kThreads), 0, s, a);
+    else if (out_dtype == BEM_C64)
+        hipLaunchKernelGGL(mfie::mfie_regular_kernel<float>, grid, dim3(mfie::kThreads), 0, s, a);
+    else {
+        set_error("the magnetic field operator needs complex output");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    return check_launch("mfie_regular_kernel");
+}
+
+// singular pairs of the MFIE, added to the output
+int mfie_singular_launch(const SpaceArgs &test, const SpaceArgs &trial, int n_pairs, const int32_t *pairs,
+                         const double *tpts, const double *spts, const double *wts, double kr, double ki, void *out,
+                         int64_t ld, int out_dtype, hipStream_t s) {
+    if (n_pairs == 0) return BEM_OK;
+    mfie::Args a{};
+    a.test = test;
+    a.trial = trial;
+    a.kr = kr;
+    a.ki = ki;
+    a.out = out;
+    a.ld = ld;
+    a.n_pairs = n_pairs;
+    a.pairs = pairs;
+    a.tpts = tpts;
+    a.spts = spts;
+    a.wts = wts;
+    const dim3 grid((n_pairs + 3) / 4);
+    if (out_dtype == BEM_C128)
+        hipLaunchKernelGGL(mfie::mfie_singular_kernel<double>, grid, dim3(mfie::kThreads), 0, s, a);
+    else if (out_dtype == BEM_C64)
+        hipLaunchKernelGGL(mfie::mfie_singular_kernel<float>, grid, dim3(mfie::kThreads), 0, s, a);
+    else {
+        set_error("the magnetic field operator needs complex output");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    return check_launch("mfie_singular_kernel");
+}
+
+}  // namespace bem

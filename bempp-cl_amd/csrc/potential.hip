@@ -352,11 +352,13 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed(const PotArg
 //   Helmholtz double layer  -i Re(k) (xh . n_y) K c
 //   Maxwell electric field  K (ik t1 - xh t2)
 //   Maxwell magnetic field  xh x (K ik t1), the cross product taken once per direction
-enum FarField { kFarSL = 0, kFarDL = 1, kFarE = 2, kFarH = 3 };
+// and, with the same sources, the magnetic field potential (numba_kernels.py:2363-2433) at points x:
+//   sum (x - y) x (G (ik r - 1) t1 / r^2),  G = e^{ikr} / (4 pi r), r = |x - y|
+enum FarField { kFarSL = 0, kFarDL = 1, kFarE = 2, kFarH = 3, kMagneticPotential = 4 };
 
 template <int FF>
 __global__ __launch_bounds__(kPotThreads) void far_field_eval_fp64(const PotArgs a) {
-    constexpr bool MAXWELL = FF == kFarE || FF == kFarH;
+    constexpr bool MAXWELL = FF == kFarE || FF == kFarH || FF == kMagneticPotential;
     constexpr int KD = MAXWELL ? 3 : 1;
     constexpr int NF = MAXWELL ? kSrcFields : 8;
     __shared__ double sh[NF][kPotTile];
@@ -378,13 +380,38 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_fp64(const PotArgs
         __syncthreads();
         {
             const int s = t0 + threadIdx.x;
-            // sources past the chunk: at the origin with zero weights
-            for (int k = 0; k < NF; ++k) sh[k][threadIdx.x] = s < s1 ? a.src[(int64_t)k * a.nsrc_pad + s] : 0.0;
+            // sources past the chunk: far away with zero weights (r > 0 for every target)
+            for (int k = 0; k < NF; ++k)
+                sh[k][threadIdx.x] = s < s1 ? a.src[(int64_t)k * a.nsrc_pad + s] : (k == 0 ? 1e30 : 0.0);
         }
         __syncthreads();
         const int nt = min(kPotTile, s1 - t0);
         for (int j = 0; j < nt; ++j) {
-            // padding sources of the prep sit at 1e30 with zero weights: the phase stays finite
+            if constexpr (FF == kMagneticPotential) {
+                const double d0 = x0 - sh[0][j], d1 = x1 - sh[1][j], d2 = x2 - sh[2][j];
+                const double r2 = d0 * d0 + d1 * d1 + d2 * d2, r = sqrt(r2);
+                double sn, cs;
+                sincos(kr * r, &sn, &cs);
+                const double damp = ki != 0.0 ? exp(-ki * r) : 1.0;
+                const double g = kInv4Pi * damp / (r * r2);
+                const double fr = -ki * r - 1.0, fi = kr * r;  // ik r - 1
+                const double vr = g * (cs * fr - sn * fi), vi = g * (cs * fi + sn * fr);
+                double wr[3], wi[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double t1r = sh[8 + 2 * d][j], t1i = sh[9 + 2 * d][j];
+                    wr[d] = vr * t1r - vi * t1i;
+                    wi[d] = vr * t1i + vi * t1r;
+                }
+                accr[0] += d1 * wr[2] - d2 * wr[1];
+                acci[0] += d1 * wi[2] - d2 * wi[1];
+                accr[1] += d2 * wr[0] - d0 * wr[2];
+                acci[1] += d2 * wi[0] - d0 * wi[2];
+                accr[2] += d0 * wr[1] - d1 * wr[0];
+                acci[2] += d0 * wi[1] - d1 * wi[0];
+                continue;
+            }
+            // padding sources sit at 1e30 with zero weights: the phase stays finite
             const double dot = x0 * sh[0][j] + x1 * sh[1][j] + x2 * sh[2][j];
             double sn, cs;
             sincos(-kr * dot, &sn, &cs);
@@ -469,12 +496,14 @@ static bool is_far_field_op(int op) {
 
 static bool is_potential_op(int op) {
     return op == BEM_LAPLACE_SINGLE_LAYER || op == BEM_LAPLACE_DOUBLE_LAYER || op == BEM_HELMHOLTZ_SINGLE_LAYER ||
-           op == BEM_HELMHOLTZ_DOUBLE_LAYER || op == BEM_MAXWELL_ELECTRIC_FIELD || is_far_field_op(op);
+           op == BEM_HELMHOLTZ_DOUBLE_LAYER || op == BEM_MAXWELL_ELECTRIC_FIELD || op == BEM_MAXWELL_MAGNETIC_FIELD ||
+           is_far_field_op(op);
 }
 
 // the operators whose sources are RWG fields (t1, t2) and whose values are 3-vectors
 static bool is_vector_potential_op(int op) {
-    return op == BEM_MAXWELL_ELECTRIC_FIELD || op == BEM_MAXWELL_ELECTRIC_FAR_FIELD || op == BEM_MAXWELL_MAGNETIC_FAR_FIELD;
+    return op == BEM_MAXWELL_ELECTRIC_FIELD || op == BEM_MAXWELL_ELECTRIC_FAR_FIELD ||
+           op == BEM_MAXWELL_MAGNETIC_FAR_FIELD || op == BEM_MAXWELL_MAGNETIC_FIELD;
 }
 
 // FF < 0: the potential of OP; FF >= 0: far field FF (sources prepared as for OP)
@@ -571,6 +600,7 @@ extern "C" int bem_potential_evaluate(int op, const bem_space_t *space, const vo
     case BEM_HELMHOLTZ_FAR_FIELD_DOUBLE_LAYER: return run_potential<kHelmholtzSL, kFarDL>(a, precision, out_dtype, s);
     case BEM_MAXWELL_ELECTRIC_FAR_FIELD: return run_potential<kMaxwellEFIE, kFarE>(a, precision, out_dtype, s);
     case BEM_MAXWELL_MAGNETIC_FAR_FIELD: return run_potential<kMaxwellEFIE, kFarH>(a, precision, out_dtype, s);
+    case BEM_MAXWELL_MAGNETIC_FIELD: return run_potential<kMaxwellEFIE, kMagneticPotential>(a, precision, out_dtype, s);
     default: return run_potential<kMaxwellEFIE>(a, precision, out_dtype, s);
     }
 }

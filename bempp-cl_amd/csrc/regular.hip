@@ -24,13 +24,23 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
         set_error("unknown precision mode %d", precision);
         return BEM_ERR_INVALID_ARGUMENT;
     }
-    if (op == BEM_MAXWELL_ELECTRIC_FIELD && (test->shapeset != BEM_SHAPE_RWG || trial->shapeset != BEM_SHAPE_RWG)) {
-        set_error("the electric field operator needs RWG/SNC spaces on both sides");
+    const bool maxwell = op == BEM_MAXWELL_ELECTRIC_FIELD || op == BEM_MAXWELL_MAGNETIC_FIELD;
+    if (maxwell && (test->shapeset != BEM_SHAPE_RWG || trial->shapeset != BEM_SHAPE_RWG)) {
+        set_error("the Maxwell operators need RWG/SNC spaces on both sides");
         return BEM_ERR_INVALID_ARGUMENT;
     }
-    if (op != BEM_MAXWELL_ELECTRIC_FIELD && (test->shapeset == BEM_SHAPE_RWG || trial->shapeset == BEM_SHAPE_RWG)) {
+    if (!maxwell && (test->shapeset == BEM_SHAPE_RWG || trial->shapeset == BEM_SHAPE_RWG)) {
         set_error("scalar operators need scalar (P0/P1) spaces");
         return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (op == BEM_MAXWELL_MAGNETIC_FIELD) {
+        // FP64 in both precision modes; adds to the output (dof blocks, if given, are not used)
+        if (out2 || work || upper) {
+            set_error("the magnetic field operator has no fused or symmetric pass");
+            return BEM_ERR_UNSUPPORTED;
+        }
+        return mfie_regular_launch(to_args(test), to_args(trial), to_args(rule), k_re, k_im, grids_identical, out, ld,
+                                   out_dtype, (hipStream_t)stream);
     }
     const bool tile = tblocks != nullptr;
     if ((tblocks == nullptr) != (sblocks == nullptr)) {

@@ -217,6 +217,9 @@ static int singular_entry(int op, const bem_space_t *test, const bem_space_t *tr
     case BEM_MAXWELL_ELECTRIC_FIELD: return launch_sing_op<kMaxwellEFIE>(a, out_dtype, s);
     case BEM_LAPLACE_HYPERSINGULAR: return launch_sing_op<kLaplaceHS>(a, out_dtype, s);
     case BEM_HELMHOLTZ_HYPERSINGULAR: return launch_sing_op<kHelmholtzHS>(a, out_dtype, s);
+    case BEM_MAXWELL_MAGNETIC_FIELD:
+        return mfie_singular_launch(a.test, a.trial, n_pairs, pair_data, test_points, trial_points, weights, k_re,
+                                    k_im, out, ld, out_dtype, s);
     default: set_error("unsupported operator %d", op); return BEM_ERR_UNSUPPORTED;
     }
 }

@@ -53,7 +53,11 @@ enum bem_operator {
     BEM_HELMHOLTZ_FAR_FIELD_SINGLE_LAYER = 9,
     BEM_HELMHOLTZ_FAR_FIELD_DOUBLE_LAYER = 10,
     BEM_MAXWELL_ELECTRIC_FAR_FIELD = 11,
-    BEM_MAXWELL_MAGNETIC_FAR_FIELD = 12
+    BEM_MAXWELL_MAGNETIC_FAR_FIELD = 12,
+    /* RWG domain, SNC dual; bem_regular_assemble (no dof blocks) and bem_singular_assemble add to
+       the output, FP64 in both precision modes; bem_potential_evaluate gives the magnetic field
+       potential: bempp_cl/core/numba_kernels.py:2065-2277, 2363-2433 */
+    BEM_MAXWELL_MAGNETIC_FIELD = 13
 };
 
 /* reference shapesets (bempp_cl/api/space/shapesets.py:46-158) */

@@ -15,11 +15,11 @@
  *   adjacency ordering  bempp_cl/api/grid/grid.py:1015-1098
  *   kernels             bempp_cl/core/numba_kernels.py:176-398 (regular), 453-551 (singular)
  *   regular assembly    bempp_cl/core/numba_kernels.py:980-1074 (scalar), 1853-1969 (EFIE),
- *                       1078-1333 (Laplace / Helmholtz hypersingular)
+ *                       1078-1333 (Laplace / Helmholtz hypersingular), 2151-2277 (MFIE)
  *   singular assembly   bempp_cl/core/numba_kernels.py:1471-1526 (scalar), 1973-2061 (EFIE),
- *                       1530-1697 (hypersingular),
+ *                       1530-1697 (hypersingular), 2065-2147 (MFIE),
  *                       bempp_cl/core/singular_assembler.py:69-448, dense_assembler.py:78-98
- *   potentials          bempp_cl/core/numba_kernels.py:1788-1849 (scalar), 2281-2359 (EFIE)
+ *   potentials          bempp_cl/core/numba_kernels.py:1788-1849 (scalar), 2281-2359 (EFIE), 2363-2433 (MFIE)
  *   far fields          bempp_cl/core/numba_kernels.py:402-449 (Helmholtz), 2437-2570 (Maxwell)
  *   Piola / edge length bempp_cl/core/numba_kernels.py:96-133
  *   colouring           bempp_cl/api/space/space.py:704-733 (per-colour parallel loop,
@@ -36,7 +36,8 @@
 #endif
 
 enum { OP_LAPLACE_SL = 0, OP_LAPLACE_DL, OP_LAPLACE_ADL, OP_HELMHOLTZ_SL, OP_HELMHOLTZ_DL, OP_HELMHOLTZ_ADL, OP_MAXWELL_EFIE,
-       OP_LAPLACE_HS, OP_HELMHOLTZ_HS, OP_HELMHOLTZ_FF_SL, OP_HELMHOLTZ_FF_DL, OP_MAXWELL_E_FF, OP_MAXWELL_H_FF };
+       OP_LAPLACE_HS, OP_HELMHOLTZ_HS, OP_HELMHOLTZ_FF_SL, OP_HELMHOLTZ_FF_DL, OP_MAXWELL_E_FF, OP_MAXWELL_H_FF,
+       OP_MAXWELL_MFIE };
 enum { SHAPE_P0 = 0, SHAPE_P1 = 1, SHAPE_RWG = 2 };
 
 #define M_INV_4PI_ 0.079577471545947667884
@@ -131,6 +132,22 @@ static inline cplx hs_weight(int op, double ct[3][3], double cs[3][3], const dou
     return w;
 }
 
+/* loc[i][j] += g (ik r - 1) / r^2 (x - y) . (tv_i x sv_j), the MFIE integrand at one point pair */
+static void mfie_accumulate(const double *x, const double *y, double tv[3][3], double sv[3][3], cplx g, cplx ik,
+                            cplx loc[3][3]) {
+    double diff[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]};
+    double r2 = diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2], r = sqrt(r2);
+    cplx f = {(ik.re * r - 1.0) / r2, ik.im * r / r2};
+    cplx gf = cmul(g, f);
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) {
+        double c0 = tv[i][1] * sv[j][2] - tv[i][2] * sv[j][1];
+        double c1 = tv[i][2] * sv[j][0] - tv[i][0] * sv[j][2];
+        double c2 = tv[i][0] * sv[j][1] - tv[i][1] * sv[j][0];
+        double d = diff[0] * c0 + diff[1] * c1 + diff[2] * c2;
+        loc[i][j].re += gf.re * d; loc[i][j].im += gf.im * d;
+    }
+}
+
 /* ------------------------------------------------------------------ kernels */
 /* G(x, y) for the operator. nx / ny are the (multiplied) test / trial normals. */
 static inline cplx kernel_eval(int op, const double *x, const double *y, const double *nx, const double *ny, double kr, double ki) {
@@ -145,6 +162,7 @@ static inline cplx kernel_eval(int op, const double *x, const double *y, const d
     case OP_LAPLACE_ADL: out.re = M_INV_4PI_ * (diff[0] * nx[0] + diff[1] * nx[1] + diff[2] * nx[2]) / (r * r * r); break;
     case OP_HELMHOLTZ_SL:
     case OP_HELMHOLTZ_HS:
+    case OP_MAXWELL_MFIE:
     case OP_MAXWELL_EFIE: {
         double damp = exp(-ki * r);
         out.re = cos(kr * r) * M_INV_4PI_ / r * damp;
@@ -347,13 +365,18 @@ static void regular_pair(int op, const Elem *te, const Elem *se, double tnm, dou
     for (int p = 0; p < nq; ++p) {
         double x[3]; map_point(te, qx[p], qy[p], x);
         double phi[3] = {0, 0, 0}, tv[3][3] = {{0}};
-        if (op == OP_MAXWELL_EFIE) rwg_piola(te, qx[p], qy[p], tv); else scalar_shape(tshape, qx[p], qy[p], phi);
+        if (op == OP_MAXWELL_EFIE || op == OP_MAXWELL_MFIE) rwg_piola(te, qx[p], qy[p], tv);
+        else scalar_shape(tshape, qx[p], qy[p], phi);
         for (int q = 0; q < nq; ++q) {
             double y[3]; map_point(se, qx[q], qy[q], y);
             cplx g = kernel_eval(op, x, y, tn, sn, kr, ki);
             double f = qw[p] * qw[q] * te->intel * se->intel;
             g.re *= f; g.im *= f;
-            if (op == OP_MAXWELL_EFIE) {
+            if (op == OP_MAXWELL_MFIE) {
+                /* G (ik r - 1) / r^2 (x - y) . (phi_i x psi_j) (numba_kernels.py:2226-2262) */
+                double sv[3][3]; rwg_piola(se, qx[q], qy[q], sv);
+                mfie_accumulate(x, y, tv, sv, g, ik, loc);
+            } else if (op == OP_MAXWELL_EFIE) {
                 double sv[3][3]; rwg_piola(se, qx[q], qy[q], sv);
                 double divp = 4.0 / (te->intel * se->intel);
                 /* -(ik)(phi.psi) - divp / (ik);  1/(ik) = conj(ik)/|ik|^2 */
@@ -402,7 +425,11 @@ static void singular_pair(int op, const Elem *te, const Elem *se, double tnm, do
         map_point(se, sx[k], sy[k], y);
         cplx g = kernel_eval(op, x, y, tn, sn, kr, ki);
         g.re *= w[k]; g.im *= w[k];
-        if (op == OP_MAXWELL_EFIE) {
+        if (op == OP_MAXWELL_MFIE) {
+            double tv[3][3], sv[3][3];
+            rwg_piola(te, tx[k], ty[k], tv); rwg_piola(se, sx[k], sy[k], sv);
+            mfie_accumulate(x, y, tv, sv, g, ik, loc);
+        } else if (op == OP_MAXWELL_EFIE) {
             double tv[3][3], sv[3][3];
             rwg_piola(te, tx[k], ty[k], tv); rwg_piola(se, sx[k], sy[k], sv);
             double divp = 4.0 / (te->intel * se->intel);
@@ -462,7 +489,7 @@ static int nshape_of(int shape) { return shape == SHAPE_P0 ? 1 : 3; }
 static void scatter_block(double *out, int cols, int is_complex, const Spaces *S, const Elem *te, const Elem *se, int e, int f, cplx loc[3][3]) {
     for (int i = 0; i < S->nt; ++i) for (int j = 0; j < S->ns; ++j) {
         double m = S->tmult[e * S->nt + i] * S->smult[f * S->ns + j];
-        if (S->op == OP_MAXWELL_EFIE) m *= te->len[i] * se->len[j];
+        if (S->op == OP_MAXWELL_EFIE || S->op == OP_MAXWELL_MFIE) m *= te->len[i] * se->len[j];
         cplx v = {loc[i][j].re * m, loc[i][j].im * m};
         accumulate(out, cols, is_complex, S->tl2g[e * S->nt + i], S->sl2g[f * S->ns + j], v);
     }
@@ -488,7 +515,7 @@ int bem_oracle_assemble(int op, int nv, const double *vtx, int nf, const int32_t
     Elem *G = malloc(sizeof(Elem) * nf);
     build_elements(nf, vtx, elm, G);
     Spaces S = {op, tshape, sshape, nshape_of(tshape), nshape_of(sshape), tl2g, sl2g, tmult, smult, tnm, snm, tsup, ssup};
-    if (op == OP_MAXWELL_EFIE) { S.nt = 3; S.ns = 3; }
+    if (op == OP_MAXWELL_EFIE || op == OP_MAXWELL_MFIE) { S.nt = 3; S.ns = 3; }
     if ((op == OP_LAPLACE_HS || op == OP_HELMHOLTZ_HS) && (tshape != SHAPE_P1 || sshape != SHAPE_P1)) return -2;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -568,7 +595,7 @@ int bem_oracle_potential(int op, int nv, const double *vtx, int nf, const int32_
     if (nq < 0) return -1;
     Elem *G = malloc(sizeof(Elem) * nf);
     build_elements(nf, vtx, elm, G);
-    const int maxwell = op == OP_MAXWELL_EFIE || op == OP_MAXWELL_E_FF || op == OP_MAXWELL_H_FF;
+    const int maxwell = op == OP_MAXWELL_EFIE || op == OP_MAXWELL_E_FF || op == OP_MAXWELL_H_FF || op == OP_MAXWELL_MFIE;
     const int far = op == OP_HELMHOLTZ_FF_SL || op == OP_HELMHOLTZ_FF_DL || op == OP_MAXWELL_E_FF || op == OP_MAXWELL_H_FF;
     int nsh = maxwell ? 3 : nshape_of(shape);
     int kdim = maxwell ? 3 : 1;
@@ -647,6 +674,18 @@ int bem_oracle_potential(int op, int nv, const double *vtx, int nf, const int32_
                 continue;
             }
             cplx g = kernel_eval(op, x, sp + 3 * j, zero, sn + 3 * j, kr, ki);
+            if (op == OP_MAXWELL_MFIE) {
+                /* diff x (G (ik r - 1) t1 / r^2), diff = x - y (numba_kernels.py:2420-2433) */
+                double diff[3] = {x[0] - sp[3 * j], x[1] - sp[3 * j + 1], x[2] - sp[3 * j + 2]};
+                double r2 = diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2], r = sqrt(r2);
+                cplx f = {(ik.re * r - 1.0) / r2, ik.im * r / r2};
+                cplx gf = cmul(g, f), v[3];
+                for (int d = 0; d < 3; ++d) v[d] = cmul(gf, t1[3 * j + d]);
+                acc[0].re += diff[1] * v[2].re - diff[2] * v[1].re; acc[0].im += diff[1] * v[2].im - diff[2] * v[1].im;
+                acc[1].re += diff[2] * v[0].re - diff[0] * v[2].re; acc[1].im += diff[2] * v[0].im - diff[0] * v[2].im;
+                acc[2].re += diff[0] * v[1].re - diff[1] * v[0].re; acc[2].im += diff[0] * v[1].im - diff[1] * v[0].im;
+                continue;
+            }
             if (op == OP_MAXWELL_EFIE) {
                 double diff[3] = {x[0] - sp[3 * j], x[1] - sp[3 * j + 1], x[2] - sp[3 * j + 2]};
                 double r2 = diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2], r = sqrt(r2);

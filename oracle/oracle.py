@@ -32,6 +32,7 @@ OPS = {
     "helmholtz_far_field_double_layer": 10,
     "maxwell_electric_far_field": 11,
     "maxwell_magnetic_far_field": 12,
+    "maxwell_magnetic_field": 13,
 }
 
 SHAPES = {"p0_discontinuous": 0, "p1_discontinuous": 1, "rwg0": 2, "snc0": 2}

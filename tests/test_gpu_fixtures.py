@@ -62,15 +62,17 @@ def test_boundary_fixture(case, precision, helpers, default_parameters, sphere_g
 
 
 @pytest.mark.parametrize("precision", PRECISIONS)
-@pytest.mark.parametrize("name,k", [("maxwell_electric_field_boundary", 2.5),
-                                    ("maxwell_electric_field_complex_boundary", 2.5 + 1j)])
-def test_maxwell_fixture(name, k, precision, helpers, default_parameters, sphere_grid, hip_available):
+@pytest.mark.parametrize("name,fn,k", [("maxwell_electric_field_boundary", "electric_field", 2.5),
+                                       ("maxwell_electric_field_complex_boundary", "electric_field", 2.5 + 1j),
+                                       ("maxwell_magnetic_field_boundary", "magnetic_field", 2.5),
+                                       ("maxwell_magnetic_field_complex_boundary", "magnetic_field", 2.5 + 1j)])
+def test_maxwell_fixture(name, fn, k, precision, helpers, default_parameters, sphere_grid, hip_available):
+    import bempp_amd.api.operators.boundary.maxwell as M
     from bempp_amd.api import function_space
-    from bempp_amd.api.operators.boundary.maxwell import electric_field
 
     rwg = function_space(sphere_grid, "RWG", 0)
     snc = function_space(sphere_grid, "SNC", 0)
-    mat = electric_field(rwg, rwg, snc, k, assembler="dense", device_interface="hip", precision=precision,
+    mat = getattr(M, fn)(rwg, rwg, snc, k, assembler="dense", device_interface="hip", precision=precision,
                          parameters=default_parameters).weak_form().to_dense()
     rtol, atol = (1e-5, 1e-7) if precision == "single" else (1e-10, 1e-14)
     np.testing.assert_allclose(mat, helpers.load_npy_data(name), rtol=rtol, atol=atol)
@@ -86,6 +88,8 @@ POTENTIALS = [
     ("helmholtz_double_layer_potential_complex_p1", "helmholtz.double_layer", ("P", 1), 2.5 + 1j),
     ("maxwell_electric_field_potential", "maxwell.electric_field", ("RWG", 0), 2.5),
     ("maxwell_electric_field_potential_complex", "maxwell.electric_field", ("RWG", 0), 2.5 + 1j),
+    ("maxwell_magnetic_field_potential", "maxwell.magnetic_field", ("RWG", 0), 2.5),
+    ("maxwell_magnetic_field_potential_complex", "maxwell.magnetic_field", ("RWG", 0), 2.5 + 1j),
 ]
 
 

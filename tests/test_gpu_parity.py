@@ -103,6 +103,31 @@ def test_far_fields_vs_oracle(case, ico2, default_parameters, hip_available):
     assert u32.dtype == np.complex64 and rel_fro(u32.astype(np.complex128), ref) < 1e-6
 
 
+@pytest.mark.parametrize("k", [2.0, 1.5 + 0.4j])
+def test_magnetic_field_vs_oracle(k, ico2, default_parameters, hip_available):
+    """MFIE RWG x SNC and the magnetic field potential: FP64 vs the oracle; single (FP64 computed,
+    complex64 stored) within 1e-6."""
+    from bempp_amd.api import GridFunction, function_space
+    from bempp_amd.api.operators.boundary.maxwell import magnetic_field
+    from bempp_amd.api.operators.potential import maxwell as mpot
+    from oracle import oracle as O
+
+    rwg = function_space(ico2, "RWG", 0)
+    snc = function_space(ico2, "SNC", 0)
+    A64 = magnetic_field(rwg, rwg, snc, k, parameters=default_parameters).weak_form().to_dense()
+    ref = O.assemble("maxwell_magnetic_field", ico2.vertices, ico2.elements, rwg, snc, k)
+    assert rel_fro(A64, ref) < 1e-12
+    A32 = magnetic_field(rwg, rwg, snc, k, parameters=default_parameters, precision="single").weak_form().to_dense()
+    assert rel_fro(A32.astype(A64.dtype), A64) < 1e-6
+    rng = np.random.default_rng(5)
+    c = rng.standard_normal(rwg.global_dof_count) + 1j * rng.standard_normal(rwg.global_dof_count)
+    pts = rng.standard_normal((3, 100))
+    pts = 2.5 * pts / np.linalg.norm(pts, axis=0)
+    u = mpot.magnetic_field(rwg, pts, k, parameters=default_parameters).evaluate(GridFunction(rwg, coefficients=c))
+    uref = O.potential("maxwell_magnetic_field", ico2.vertices, ico2.elements, rwg, c, pts, k)
+    assert rel_fro(u, uref) < 1e-12
+
+
 def test_maxwell_vs_oracle(ico2, default_parameters, hip_available):
     from bempp_amd.api import function_space
     from bempp_amd.api.operators.boundary.maxwell import electric_field

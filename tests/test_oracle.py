@@ -40,6 +40,8 @@ BOUNDARY = [
     ("laplace_hypersingular_boundary", "laplace_hypersingular", ("P", 1), ("P", 1), 0.0),
     ("helmholtz_hypersingular_boundary", "helmholtz_hypersingular", ("P", 1), ("P", 1), 2.5),
     ("helmholtz_complex_hypersingular_boundary", "helmholtz_hypersingular", ("P", 1), ("P", 1), 2.5 + 1j),
+    ("maxwell_magnetic_field_boundary", "maxwell_magnetic_field", ("RWG", 0), ("SNC", 0), 2.5),
+    ("maxwell_magnetic_field_complex_boundary", "maxwell_magnetic_field", ("RWG", 0), ("SNC", 0), 2.5 + 1j),
 ]
 
 
@@ -65,6 +67,8 @@ POTENTIALS = [
     ("helmholtz_double_layer_potential_complex_p1", "helmholtz_double_layer", ("P", 1), 2.5 + 1j),
     ("maxwell_electric_field_potential", "maxwell_electric_field", ("RWG", 0), 2.5),
     ("maxwell_electric_field_potential_complex", "maxwell_electric_field", ("RWG", 0), 2.5 + 1j),
+    ("maxwell_magnetic_field_potential", "maxwell_magnetic_field", ("RWG", 0), 2.5),
+    ("maxwell_magnetic_field_potential_complex", "maxwell_magnetic_field", ("RWG", 0), 2.5 + 1j),
 ]
 
 
