@@ -64,6 +64,8 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the loss/gradient measurement")
     ap.add_argument("--no-solve", action="store_true", help="skip the GMRES solve on the assembled operator")
     ap.add_argument("--no-hypersingular", action="store_true", help="skip the hypersingular assembly")
+    ap.add_argument("--no-maxwell-extras", action="store_true",
+                    help="skip the MFIE assembly, the magnetic potential and the far fields")
     ap.add_argument("--adjoint-level", type=int, default=6)
     ap.add_argument("--adjoint-k", type=float, default=8.0)
     ap.add_argument("--fd-vertices", type=int, default=32)
@@ -485,6 +487,53 @@ def hypersingular_bench(space, k, reps=3):
     }
 
 
+def maxwell_extras_bench(level, k, reps=3):
+    """C4's grid and wavenumber: the MFIE (RWG x SNC, FP64 kernels), the magnetic field potential
+    at 65,536 points on radius 10 and the electric / magnetic far fields at 65,536 directions of
+    the same seeded RWG density."""
+    import numpy as np
+    import torch
+
+    import bempp_amd.api as bempp
+    from bempp_amd.api.operators.boundary.maxwell import magnetic_field
+    from bempp_amd.api.operators.far_field import maxwell as ff
+    from bempp_amd.api.operators.potential import maxwell as mpot
+    from bempp_amd.core.dense_assembler import hip_dense_assemble
+
+    grid = bempp.shapes.icosphere(level)
+    rwg = bempp.function_space(grid, "RWG", 0)
+    snc = bempp.function_space(grid, "SNC", 0)
+    n = rwg.global_dof_count
+    op = magnetic_field(rwg, rwg, snc, k, precision="single")
+    out = hip_dense_assemble(rwg, snc, op.assembler.parameters, op.descriptor)  # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        out = hip_dense_assemble(rwg, snc, op.assembler.parameters, op.descriptor, out=out, accumulate=False)
+    torch.cuda.synchronize()
+    t_mfie = (time.perf_counter() - t) / reps
+    rng = np.random.default_rng(2)
+    c = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    fun = bempp.GridFunction(rwg, coefficients=c)
+    g = np.random.default_rng(3).standard_normal((3, 65536))
+    dirs = g / np.linalg.norm(g, axis=0)
+    res = {"workload": f"icosphere level {level} ({grid.number_of_elements} triangles, {n} RWG dofs), k={k}",
+           "mfie_ms": 1e3 * t_mfie, "mfie_entries_per_s": n * n / t_mfie,
+           "mfie_note": "FP64 kernels (also for complex64 output)"}
+    for name, opr, pts in (("magnetic_potential", mpot.magnetic_field(rwg, 10.0 * dirs, k, precision="single"), 10.0 * dirs),
+                           ("electric_far_field", ff.electric_field(rwg, dirs, k, precision="single"), dirs),
+                           ("magnetic_far_field", ff.magnetic_field(rwg, dirs, k, precision="single"), dirs)):
+        opr.evaluate(fun)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            opr.evaluate(fun)
+        dt = (time.perf_counter() - t) / reps
+        res[f"{name}_ms"] = 1e3 * dt
+        res[f"{name}_evals_per_s"] = pts.shape[1] * grid.number_of_elements * 6 / dt
+    return res
+
+
 def traffic_from_profiles(kernel_key):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if recorded."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -795,6 +844,10 @@ def main():
     if not args.no_hypersingular:
         result["hypersingular"] = hypersingular_bench(space, k)
         log(f"hypersingular {result['hypersingular']}")
+
+    if not args.no_maxwell_extras:
+        result["maxwell_extras"] = maxwell_extras_bench(4, 2.0)
+        log(f"maxwell extras {result['maxwell_extras']}")
 
     if not args.no_efie:
         result["efie"] = efie_bench(4, 2.0, 3)
