@@ -1,12 +1,14 @@
 // Singular (Sauter-Schwab) pairs, FP64, for gfx950.
 //
-// One 64-lane wavefront per (test element, trial element) pair that shares a vertex,
+// A group of kSingGroup lanes per (test element, trial element) pair that shares a vertex,
 // an edge or the whole triangle (bempp_cl/core/numba_kernels.py:1471-1526 scalar,
-// 1973-2061 EFIE). The pair's 512..1536 Duffy points are strided over the lanes; each
-// lane accumulates the 3x3 moment block S[a][b] = sum_k G_k w_k u_a(xi_k) u_b(eta_k)
-// with u = (1, xi_0, xi_1), the block is reduced across the wave with cross-lane
-// shuffles, scaled by J_test J_trial and turned into the local matrix block exactly as
-// for regular pairs. Lanes 0..8 add the multiplied values to the dense output with
+// 1973-2061 EFIE); a wavefront takes 8 consecutive pairs, which are mostly of one kind (the
+// pair table lists coincident, then edge, then vertex pairs). The pair's 512..1536 Duffy
+// points are strided over the group's lanes; each lane accumulates the 3x3 moment block
+// S[a][b] = sum_k G_k w_k u_a(xi_k) u_b(eta_k) with u = (1, xi_0, xi_1), the block is summed
+// over the group with 3 shuffle steps (a reduction over the whole wavefront would cost 6 steps
+// for one pair), scaled by J_test J_trial and turned into the local matrix block exactly as
+// for regular pairs. The group's lanes add the multiplied entries to the dense output with
 // hardware atomics, which replaces the reference's host-side np.add.at scatter
 // (bempp_cl/core/dense_assembler.py:78-98).
 #include "bem_device.h"
@@ -25,10 +27,12 @@ struct SingularArgs {
     int64_t ld;
 };
 
+constexpr int kSingGroup = 8;  // lanes per pair
+
 template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
+__device__ __forceinline__ T group_sum(T v) {
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    for (int off = kSingGroup / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kSingGroup);
     return v;
 }
 
@@ -37,9 +41,11 @@ __device__ __forceinline__ T wave_sum(T v) {
 template <int OP, int NTS, int NSS, typename OutT, bool DUAL = false>
 __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
     constexpr bool CPLX = OpTraits<OP>::is_complex;
-    const int lane = threadIdx.x & 63;
-    const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (pair >= a.n_pairs) return;  // uniform per wave
+    const int lane = threadIdx.x & (kSingGroup - 1);  // lane within the pair's group
+    const int pair = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kSingGroup);
+    // the group's lanes share the pair, so whole groups leave together and the group shuffles
+    // below only read lanes of groups that stay
+    if (pair >= a.n_pairs) return;
     const int32_t *pd = a.pairs + 6 * (int64_t)pair;
     const int e = pd[0], f = pd[1], toff = pd[2], soff = pd[3], woff = pd[4], np = pd[5];
     const double *gt = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
@@ -62,7 +68,7 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
     // wave-uniform bound selects the series sin/cos without range reduction when k r <= 1
     auto points = [&](auto small_tag) {
         constexpr bool SMALL = decltype(small_tag)::value;
-        for (int k = lane; k < np; k += 64) {
+        for (int k = lane; k < np; k += kSingGroup) {
             double xi0 = a.tpts[2 * (int64_t)(toff + k)], xi1 = a.tpts[2 * (int64_t)(toff + k) + 1];
             double et0 = a.spts[2 * (int64_t)(soff + k)], et1 = a.spts[2 * (int64_t)(soff + k) + 1];
             double w = a.wts[woff + k];
@@ -99,18 +105,19 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
     for (int i = 0; i < NTS; ++i)
 #pragma unroll
         for (int j = 0; j < NSS; ++j) {
-            S[i][j].re = wave_sum(Sr[i][j]) * jj;
-            S[i][j].im = CPLX ? wave_sum(Si[i][j]) * jj : 0.0;
+            S[i][j].re = group_sum(Sr[i][j]) * jj;
+            S[i][j].im = CPLX ? group_sum(Si[i][j]) * jj : 0.0;
             if constexpr (DUAL) {
-                S2[i][j].re = wave_sum(Sr2[i][j]) * jj;
-                S2[i][j].im = CPLX ? wave_sum(Si2[i][j]) * jj : 0.0;
+                S2[i][j].re = group_sum(Sr2[i][j]) * jj;
+                S2[i][j].im = CPLX ? group_sum(Si2[i][j]) * jj : 0.0;
             }
         }
-    // lanes 0 .. NTS*NSS-1 each add one entry (DUAL: lanes NTS*NSS .. 2*NTS*NSS-1 the double layer)
+    // the group's lanes add entries l = lane, lane + kSingGroup, ... (DUAL: l >= NTS*NSS are the
+    // double layer's)
     const int nb = NTS * NSS;
-    if (lane < (DUAL ? 2 * nb : nb)) {
-        const bool second = DUAL && lane >= nb;
-        const int l = second ? lane - nb : lane;
+    for (int ll = lane; ll < (DUAL ? 2 * nb : nb); ll += kSingGroup) {
+        const bool second = DUAL && ll >= nb;
+        const int l = second ? ll - nb : ll;
         int i = l / NSS, j = l % NSS;
         double m = a.test.mult[(int64_t)e * NTS + i] * a.trial.mult[(int64_t)f * NSS + j];
         if (m != 0.0) {
@@ -137,7 +144,7 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
 
 template <int OP, typename OutT, bool DUAL>
 static int launch_sing(const SingularArgs &a, hipStream_t s) {
-    dim3 grid((a.n_pairs + 3) / 4);
+    dim3 grid((unsigned)(((int64_t)a.n_pairs * kSingGroup + 255) / 256));
     if (a.n_pairs == 0) return BEM_OK;
     if constexpr (OP == kMaxwellEFIE) {
         hipLaunchKernelGGL((singular_kernel<OP, 3, 3, OutT>), grid, dim3(256), 0, s, a);
