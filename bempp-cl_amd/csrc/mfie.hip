@@ -5,13 +5,18 @@
 // single-layer kernel e^{ikr} / (4 pi r), A_i = E_t (xi_p - o_i) and B_j = E_f (eta_q - o_j) the
 // Piola-mapped RWG functions times their Jacobians (the J_t J_f of the quadrature cancels the
 // 1/J of the two Piola maps), o = ((0, 1), (1, 0), (0, 0)) and l the edge lengths.
-// Both kernels evaluate in FP64 (also for complex64 output) and add to the output with atomics:
-//  * regular: one lane per trial element, test elements staged in LDS in tiles of 16; pairs that
-//    share a vertex are left to the singular kernel (grids identical);
-//  * singular: one 64-lane wavefront per Sauter-Schwab pair, its Duffy points strided over the
-//    lanes, the 3 x 3 block reduced across the wave.
-#include "bem_device.h"
+// The kernels add to the output with atomics:
+//  * regular, FP64: one lane per trial element, test elements staged in LDS in tiles of 16;
+//    pairs that share a vertex are left to the singular kernel (grids identical); NEAR_ONLY
+//    takes only the near pairs of the mixed mode;
+//  * regular, mixed: far pairs in FP32 with the pair's reference distance r0 = |c_f - c_e| and
+//    e^{i k r0} formed in FP64 and only r - r0 (from the small offsets of the points from the
+//    centroids) in FP32, as the other mixed kernels (far_split.h); near pairs (pair_is_near) go
+//    to the FP64 kernel;
+//  * singular, FP64: one 64-lane wavefront per Sauter-Schwab pair, its Duffy points strided over
+//    the lanes, the 3 x 3 block reduced across the wave.
 #include "bem_internal.h"
+#include "far_split.h"
 
 namespace bem {
 namespace mfie {
@@ -20,11 +25,13 @@ constexpr int kThreads = 256;
 constexpr int kTile = 16;  // test elements per workgroup
 constexpr int kQ = 6;      // quadrature points of the regular kernel (the order-4 rule)
 constexpr double kOx[3] = {0.0, 1.0, 0.0}, kOy[3] = {1.0, 0.0, 0.0};
+constexpr double kInv2Pi = 0.15915494309189533577;  // phases in revolutions
 
 struct Args {
     SpaceArgs test, trial;
     RuleArgs rule;
     double kr, ki;
+    double near_factor;  // mixed mode: the pair_is_near split
     int identical;
     void *out;
     int64_t ld;
@@ -77,6 +84,33 @@ __device__ __forceinline__ void accumulate(const double (&diff)[3], const double
     }
 }
 
+// dofs and multiplier x edge length of the three RWG functions of element e of a space
+__device__ __forceinline__ void dof_factors(const SpaceArgs &sp, int e, int (&dof)[3], double (&ml)[3]) {
+    double l[3];
+    edge_lengths(sp.geom + (int64_t)e * BEM_GEOM_STRIDE, l);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        dof[i] = sp.l2g[3 * (int64_t)e + i];
+        ml[i] = sp.mult[3 * (int64_t)e + i] * l[i];
+    }
+}
+
+// out[dof_t[i], dof_f[j]] += (ar + i ai)[i][j] ml_t[i] ml_f[j] for the 9 entries of a pair
+template <typename OutT, typename AccT>
+__device__ __forceinline__ void scatter_block(const Args &a, const int *dt, const double *mt, const int (&df)[3],
+                                              const double (&mf)[3], const AccT (&ar)[3][3], const AccT (&ai)[3][3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double m = mt[i] * mf[j];
+            if (m == 0.0) continue;
+            OutT *o = reinterpret_cast<OutT *>(a.out) + 2 * ((int64_t)dt[i] * a.ld + df[j]);
+            atomic_add(o, (OutT)(ar[i][j] * m));
+            atomic_add(o + 1, (OutT)(ai[i][j] * m));
+        }
+}
+
 template <typename OutT>
 __device__ __forceinline__ void scatter(const Args &a, int e, int f, const double *gt, const double *gf,
                                         const double (&ar)[3][3], const double (&ai)[3][3], int i, int j) {
@@ -96,9 +130,13 @@ struct Tile {
     int vid[kTile][3];
     double x[kTile][kQ][3];
     double e1[kTile][3], e2[kTile][3];
+    double c[kTile][3];  // centroids (NEAR_ONLY)
+    double diam[kTile];
+    int dof[kTile][3];
+    double ml[kTile][3];  // multiplier x edge length
 };
 
-template <typename OutT>
+template <typename OutT, bool NEAR_ONLY = false>
 __global__ __launch_bounds__(kThreads) void mfie_regular_kernel(const Args a) {
     __shared__ Tile T_;
     const int tid = threadIdx.x;
@@ -117,12 +155,20 @@ __global__ __launch_bounds__(kThreads) void mfie_regular_kernel(const Args a) {
         }
         for (int p = 0; p < kQ; ++p)
             for (int d = 0; d < 3; ++d) T_.x[tid][p][d] = g[d] + g[3 + d] * a.rule.x[p] + g[6 + d] * a.rule.y[p];
+        centroid_rn(g, T_.c[tid]);
+        T_.diam[tid] = g[13];
+        dof_factors(a.test, e, T_.dof[tid], T_.ml[tid]);
     }
     __syncthreads();
     if (!valid) return;
     const int f = a.trial.elements[jf];
     const double *gf = a.trial.geom + (int64_t)f * BEM_GEOM_STRIDE;
     const int fv0 = a.trial.vids[4 * f], fv1 = a.trial.vids[4 * f + 1], fv2 = a.trial.vids[4 * f + 2];
+    double cf[3];
+    centroid_rn(gf, cf);
+    int fdof[3];
+    double fml[3];
+    dof_factors(a.trial, f, fdof, fml);
     double yq[kQ][3];
 #pragma unroll
     for (int q = 0; q < kQ; ++q)
@@ -136,6 +182,10 @@ __global__ __launch_bounds__(kThreads) void mfie_regular_kernel(const Args a) {
                 adj = adj || v == fv0 || v == fv1 || v == fv2;
             }
             if (adj) continue;
+        }
+        if constexpr (NEAR_ONLY) {
+            double dist2;
+            if (!pair_is_near(T_.c[t], cf, T_.diam[t], gf[13], a.near_factor, &dist2)) continue;
         }
         double ar[3][3], ai[3][3];
 #pragma unroll
@@ -154,12 +204,138 @@ __global__ __launch_bounds__(kThreads) void mfie_regular_kernel(const Args a) {
                 accumulate(diff, A, B, a.rule.w[p] * a.rule.w[q], a.kr, a.ki, ar, ai);
             }
         }
-        const int e = T_.elem[t];
-        const double *gt = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
+        scatter_block<OutT>(a, T_.dof[t], T_.ml[t], fdof, fml, ar, ai);
+    }
+}
+
+// ------------------------------------------------------------------ mixed: far pairs in FP32
+struct TileF32 {
+    int elem[kTile];
+    int vid[kTile][3];
+    double c[kTile][3];
+    double diam[kTile];
+    float a[kTile][kQ][3];  // x_p - c_e
+    float e1[kTile][3], e2[kTile][3];
+    int dof[kTile][3];
+    double ml[kTile][3];  // multiplier x edge length
+};
+
+template <typename OutT>
+__global__ __launch_bounds__(kThreads) void mfie_regular_mixed_kernel(const Args a) {
+    __shared__ TileF32 T_;
+    const int tid = threadIdx.x;
+    const int jf = blockIdx.x * kThreads + tid;
+    const bool valid = jf < a.trial.n;
+    const int t0 = blockIdx.y * kTile;
+    const int ntile = min(kTile, a.test.n - t0);
+    if (tid < ntile) {
+        const int e = a.test.elements[t0 + tid];
+        const double *g = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
+        T_.elem[tid] = e;
+        for (int i = 0; i < 3; ++i) T_.vid[tid][i] = a.test.vids[4 * e + i];
+        centroid_rn(g, T_.c[tid]);
+        T_.diam[tid] = g[13];
+        for (int d = 0; d < 3; ++d) {
+            T_.e1[tid][d] = (float)g[3 + d];
+            T_.e2[tid][d] = (float)g[6 + d];
+        }
+        for (int p = 0; p < kQ; ++p)
+            for (int d = 0; d < 3; ++d)
+                T_.a[tid][p][d] = (float)(g[3 + d] * (a.rule.x[p] - 1.0 / 3.0) + g[6 + d] * (a.rule.y[p] - 1.0 / 3.0));
+        dof_factors(a.test, e, T_.dof[tid], T_.ml[tid]);
+    }
+    __syncthreads();
+    if (!valid) return;
+    const int f = a.trial.elements[jf];
+    const double *gf = a.trial.geom + (int64_t)f * BEM_GEOM_STRIDE;
+    const int fv0 = a.trial.vids[4 * f], fv1 = a.trial.vids[4 * f + 1], fv2 = a.trial.vids[4 * f + 2];
+    double cf[3];
+    centroid_rn(gf, cf);
+    int fdof[3];
+    double fml[3];
+    dof_factors(a.trial, f, fdof, fml);
+    // trial points as offsets from the centroid and the Piola vectors B_j(q) (times J), FP32
+    float b[kQ][3], B[kQ][3][3], wq[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        const double s0 = a.rule.x[q], s1 = a.rule.y[q];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            b[q][d] = (float)(gf[3 + d] * (s0 - 1.0 / 3.0) + gf[6 + d] * (s1 - 1.0 / 3.0));
+#pragma unroll
+            for (int j = 0; j < 3; ++j) B[q][j][d] = (float)(gf[3 + d] * (s0 - kOx[j]) + gf[6 + d] * (s1 - kOy[j]));
+        }
+        wq[q] = (float)(a.rule.w[q] * kInv4Pi);
+    }
+    const float kr = (float)a.kr, ki = (float)a.ki, krev = (float)(a.kr * kInv2Pi);
+    for (int t = 0; t < ntile; ++t) {
+        if (a.identical) {
+            bool adj = false;
+            for (int i = 0; i < 3; ++i) {
+                const int v = T_.vid[t][i];
+                adj = adj || v == fv0 || v == fv1 || v == fv2;
+            }
+            if (adj) continue;
+        }
+        double dist2;
+        if (pair_is_near(T_.c[t], cf, T_.diam[t], gf[13], a.near_factor, &dist2)) continue;  // FP64 kernel
+        const FarPair P = far_pair_setup(T_.c[t], cf);  // fd = c_f - c_e
+        float cs0, sn0;
+        phase_rev_f32(a.kr, (double)P.r0 * kInv2Pi, cs0, sn0);
+        const float damp0 = a.ki != 0.0 ? (float)exp(-a.ki * (double)P.r0) : 1.f;
+        cs0 *= damp0;
+        sn0 *= damp0;
+        float ar[3][3], ai[3][3];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) scatter<OutT>(a, e, f, gt, gf, ar, ai, i, j);
+            for (int j = 0; j < 3; ++j) ar[i][j] = ai[i][j] = 0.f;
+#pragma unroll 1
+        for (int p = 0; p < kQ; ++p) {
+            const float xi = (float)a.rule.x[p], eta = (float)a.rule.y[p], wp = (float)a.rule.w[p];
+            float A[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+                    A[i][d] = T_.e1[t][d] * (xi - (float)kOx[i]) + T_.e2[t][d] * (eta - (float)kOy[i]);
+            const float ap0 = T_.a[t][p][0], ap1 = T_.a[t][p][1], ap2 = T_.a[t][p][2];
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) {
+                // x - y = (c_e + a_p) - (c_f + b_q) = u - fd with u = a_p - b_q small
+                const float u0 = ap0 - b[q][0], u1 = ap1 - b[q][1], u2 = ap2 - b[q][2];
+                const float tt = P.resid - 2.f * (P.fd0 * u0 + P.fd1 * u1 + P.fd2 * u2) + (u0 * u0 + u1 * u1 + u2 * u2);
+                const float r2 = P.r0sq + tt;
+                const float rinv = __builtin_amdgcn_rsqf(r2);
+                const float r = r2 * rinv;
+                const float delta = tt * __builtin_amdgcn_rcpf(r + P.r0);  // r - r0
+                const float c = __builtin_amdgcn_cosf(krev * delta), s = __builtin_amdgcn_sinf(krev * delta);
+                float er = cs0 * c - sn0 * s, ei = cs0 * s + sn0 * c;  // e^{ik r} (with e^{-Im k r0})
+                if (ki != 0.f) {
+                    const float dm = __expf(-ki * delta);
+                    er *= dm;
+                    ei *= dm;
+                }
+                // G (ik r - 1) / r^2 times the weights: e^{ikr} ((-Im k r - 1) + i Re k r) w_p w_q / (4 pi r^3)
+                const float mag = wp * wq[q] * rinv * rinv * rinv;
+                const float fr = -ki * r - 1.f, fi = kr * r;
+                const float vr = mag * (er * fr - ei * fi), vi = mag * (er * fi + ei * fr);
+                const float w0 = u0 - P.fd0, w1 = u1 - P.fd1, w2 = u2 - P.fd2;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    const float D0 = w1 * A[i][2] - w2 * A[i][1];
+                    const float D1 = w2 * A[i][0] - w0 * A[i][2];
+                    const float D2 = w0 * A[i][1] - w1 * A[i][0];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        const float sij = D0 * B[q][j][0] + D1 * B[q][j][1] + D2 * B[q][j][2];
+                        ar[i][j] += vr * sij;
+                        ai[i][j] += vi * sij;
+                    }
+                }
+            }
+        }
+        scatter_block<OutT>(a, T_.dof[t], T_.ml[t], fdof, fml, ar, ai);
     }
 }
 
@@ -213,9 +389,11 @@ __global__ __launch_bounds__(kThreads) void mfie_singular_kernel(const Args a) {
 
 }  // namespace mfie
 
-// regular pairs of the MFIE (FP64; out_dtype BEM_C64 or BEM_C128), added to the output
+// regular pairs of the MFIE (out_dtype BEM_C64 or BEM_C128), added to the output: FP64, or
+// MIXED (far pairs FP32, near pairs FP64)
 int mfie_regular_launch(const SpaceArgs &test, const SpaceArgs &trial, const RuleArgs &rule, double kr, double ki,
-                        int identical, void *out, int64_t ld, int out_dtype, hipStream_t s) {
+                        int identical, int precision, double near_factor, void *out, int64_t ld, int out_dtype,
+                        hipStream_t s) {
     if (rule.n != mfie::kQ) {
         set_error("the magnetic field operator uses the 6-point regular rule (order 4), got %d points", rule.n);
         return BEM_ERR_INVALID_ARGUMENT;
@@ -232,18 +410,28 @@ int mfie_regular_launch(const SpaceArgs &test, const SpaceArgs &trial, const Rul
     a.rule = rule;
     a.kr = kr;
     a.ki = ki;
+    a.near_factor = near_factor;
     a.identical = identical;
     a.out = out;
     a.ld = ld;
-    const dim3 grid((trial.n + mfie::kThreads - 1) / mfie::kThreads, (unsigned)ntiles);
-    if (out_dtype == BEM_C128)
-        hipLaunchKernelGGL(mfie::mfie_regular_kernel<double>, grid, dim3(mfie::kThreads), 0, s, a);
-    else if (out_dtype == BEM_C64)
-        hipLaunchKernelGGL(mfie::mfie_regular_kernel<float>, grid, dim3(mfie::kThreads), 0, s, a);
-    else {
+    if (out_dtype != BEM_C128 && out_dtype != BEM_C64) {
         set_error("the magnetic field operator needs complex output");
         return BEM_ERR_INVALID_ARGUMENT;
     }
+    const dim3 grid((trial.n + mfie::kThreads - 1) / mfie::kThreads, (unsigned)ntiles);
+    const bool c128 = out_dtype == BEM_C128;
+    if (precision == BEM_PRECISION_MIXED) {
+        if (c128) {
+            hipLaunchKernelGGL(mfie::mfie_regular_mixed_kernel<double>, grid, dim3(mfie::kThreads), 0, s, a);
+            hipLaunchKernelGGL((mfie::mfie_regular_kernel<double, true>), grid, dim3(mfie::kThreads), 0, s, a);
+        } else {
+            hipLaunchKernelGGL(mfie::mfie_regular_mixed_kernel<float>, grid, dim3(mfie::kThreads), 0, s, a);
+            hipLaunchKernelGGL((mfie::mfie_regular_kernel<float, true>), grid, dim3(mfie::kThreads), 0, s, a);
+        }
+        return check_launch("mfie_regular_mixed_kernel");
+    }
+    if (c128) hipLaunchKernelGGL(mfie::mfie_regular_kernel<double>, grid, dim3(mfie::kThreads), 0, s, a);
+    else hipLaunchKernelGGL(mfie::mfie_regular_kernel<float>, grid, dim3(mfie::kThreads), 0, s, a);
     return check_launch("mfie_regular_kernel");
 }
 

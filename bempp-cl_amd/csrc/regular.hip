@@ -34,13 +34,13 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
         return BEM_ERR_INVALID_ARGUMENT;
     }
     if (op == BEM_MAXWELL_MAGNETIC_FIELD) {
-        // FP64 in both precision modes; adds to the output (dof blocks, if given, are not used)
+        // adds to the output (dof blocks, if given, are not used); MIXED includes the FP64 near pairs
         if (out2 || work || upper) {
             set_error("the magnetic field operator has no fused or symmetric pass");
             return BEM_ERR_UNSUPPORTED;
         }
-        return mfie_regular_launch(to_args(test), to_args(trial), to_args(rule), k_re, k_im, grids_identical, out, ld,
-                                   out_dtype, (hipStream_t)stream);
+        return mfie_regular_launch(to_args(test), to_args(trial), to_args(rule), k_re, k_im, grids_identical,
+                                   precision, mixed_near_factor, out, ld, out_dtype, (hipStream_t)stream);
     }
     const bool tile = tblocks != nullptr;
     if ((tblocks == nullptr) != (sblocks == nullptr)) {

@@ -488,7 +488,7 @@ def hypersingular_bench(space, k, reps=3):
 
 
 def maxwell_extras_bench(level, k, reps=3):
-    """C4's grid and wavenumber: the MFIE (RWG x SNC, FP64 kernels), the magnetic field potential
+    """C4's grid and wavenumber: the MFIE (RWG x SNC, mixed and FP64), the magnetic field potential
     at 65,536 points on radius 10 and the electric / magnetic far fields at 65,536 directions of
     the same seeded RWG density."""
     import numpy as np
@@ -517,9 +517,17 @@ def maxwell_extras_bench(level, k, reps=3):
     fun = bempp.GridFunction(rwg, coefficients=c)
     g = np.random.default_rng(3).standard_normal((3, 65536))
     dirs = g / np.linalg.norm(g, axis=0)
+    op64 = magnetic_field(rwg, rwg, snc, k, precision="double")
+    ref = hip_dense_assemble(rwg, snc, op64.assembler.parameters, op64.descriptor)  # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    ref = hip_dense_assemble(rwg, snc, op64.assembler.parameters, op64.descriptor, out=ref, accumulate=False)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t
     res = {"workload": f"icosphere level {level} ({grid.number_of_elements} triangles, {n} RWG dofs), k={k}",
-           "mfie_ms": 1e3 * t_mfie, "mfie_entries_per_s": n * n / t_mfie,
-           "mfie_note": "FP64 kernels (also for complex64 output)"}
+           "mfie_ms": 1e3 * t_mfie, "mfie_entries_per_s": n * n / t_mfie, "mfie_fp64_ms": 1e3 * t64,
+           "mfie_speedup_vs_fp64": t64 / t_mfie, "mfie_rel_err_vs_fp64": rel_fro(out, ref)}
+    del ref
     for name, opr, pts in (("magnetic_potential", mpot.magnetic_field(rwg, 10.0 * dirs, k, precision="single"), 10.0 * dirs),
                            ("electric_far_field", ff.electric_field(rwg, dirs, k, precision="single"), dirs),
                            ("magnetic_far_field", ff.magnetic_field(rwg, dirs, k, precision="single"), dirs)):
