@@ -139,10 +139,10 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     )
     if events is not None:
         events["regular_start"].record()
-    # the magnetic field operator: FP64 kernels in both modes that add to the (zeroed) output, no
-    # dof blocks and no near-field split
+    # the magnetic field operator adds to the (zeroed) output: FP64 without dof blocks; the mixed mode
+    # takes the RWG dof blocks (far pairs in FP32) and the near pass below
     mfie = op == "maxwell_magnetic_field"
-    if mfie:
+    if mfie and mode != hb.PRECISION_MIXED:
         hb.regular_assemble(op, test, trial, grids_identical, rule, _wavenumber(operator_descriptor), mode, out,
                             accumulate=1, near_factor=near, kahan=kahan)
     elif upper:
@@ -156,8 +156,8 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     else:
         from bempp_amd.core.blocking import device_blocks
 
-        # the mixed EFIE kernel takes 256-element trial blocks of up to ~400 edges
-        rwg_mixed = op == "maxwell_electric_field" and mode == hb.PRECISION_MIXED
+        # the mixed EFIE / MFIE kernels take 256-element trial blocks of up to ~400 edges
+        rwg_mixed = op in ("maxwell_electric_field", "maxwell_magnetic_field") and mode == hb.PRECISION_MIXED
         hb.regular_assemble_blocked(
             op, device_blocks(dual_to_range, "test_rwg_mixed" if rwg_mixed else "test"),
             device_blocks(domain, "trial_rwg_mixed" if rwg_mixed else "trial"),
@@ -165,7 +165,7 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
         )
     if events is not None:
         events["regular_end"].record()
-    if mode == hb.PRECISION_MIXED and not mfie:
+    if mode == hb.PRECISION_MIXED:
         from bempp_amd.core.near_field import device_near_candidates
 
         cand = device_near_candidates(dual_to_range, domain, near)

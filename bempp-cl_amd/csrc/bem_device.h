@@ -29,7 +29,14 @@ enum Op {
     kMaxwellEFIE = BEM_MAXWELL_ELECTRIC_FIELD,
     kLaplaceHS = BEM_LAPLACE_HYPERSINGULAR,
     kHelmholtzHS = BEM_HELMHOLTZ_HYPERSINGULAR,
+    kMaxwellMFIE = BEM_MAXWELL_MAGNETIC_FIELD,
 };
+
+// operators on RWG/SNC spaces (edge dofs: at most two elements per dof)
+template <int OP>
+constexpr bool is_rwg_op() {
+    return OP == kMaxwellEFIE || OP == kMaxwellMFIE;
+}
 
 // hypersingular operators: moments of their single layer, combined with surface curls in the epilogue
 template <int OP>

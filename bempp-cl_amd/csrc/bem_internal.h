@@ -68,11 +68,14 @@ int check_launch(const char *what);
 // into the lower one and zeroes the diagonal (matvec.hip).
 int square_matrix_op(int what, void *A, int dtype, int64_t n, int64_t ld, hipStream_t s);
 
-// Maxwell magnetic field operator (mfie.hip): regular pairs (FP64 or mixed) and singular pairs (FP64),
-// added to the output.
+// Maxwell magnetic field operator (mfie.hip): regular pairs (FP64, or the far pairs in mixed
+// precision), near pairs of the mixed mode and singular pairs (FP64), added to the output.
 int mfie_regular_launch(const SpaceArgs &test, const SpaceArgs &trial, const RuleArgs &rule, double kr, double ki,
                         int identical, int precision, double near_factor, void *out, int64_t ld, int out_dtype,
                         hipStream_t s);
+int mfie_near_launch(const SpaceArgs &test, const SpaceArgs &trial, const RuleArgs &rule, double kr, double ki,
+                     int identical, double near_factor, int n_pairs, const int32_t *pairs, void *out, int64_t ld,
+                     int out_dtype, hipStream_t s);
 int mfie_singular_launch(const SpaceArgs &test, const SpaceArgs &trial, int n_pairs, const int32_t *pairs,
                          const double *tpts, const double *spts, const double *wts, double kr, double ki, void *out,
                          int64_t ld, int out_dtype, hipStream_t s);

@@ -7,16 +7,18 @@
 // 1/J of the two Piola maps), o = ((0, 1), (1, 0), (0, 0)) and l the edge lengths.
 // The kernels add to the output with atomics:
 //  * regular, FP64: one lane per trial element, test elements staged in LDS in tiles of 16;
-//    pairs that share a vertex are left to the singular kernel (grids identical); NEAR_ONLY
-//    takes only the near pairs of the mixed mode;
-//  * regular, mixed: far pairs in FP32 with the pair's reference distance r0 = |c_f - c_e| and
-//    e^{i k r0} formed in FP64 and only r - r0 (from the small offsets of the points from the
-//    centroids) in FP32, as the other mixed kernels (far_split.h); near pairs (pair_is_near) go
-//    to the FP64 kernel;
+//    pairs that share a vertex are left to the singular kernel (grids identical);
+//  * regular, mixed, without dof blocks: far pairs in FP32 with the pair's reference distance
+//    r0 = |c_f - c_e| and e^{i k r0} formed in FP64 and only r - r0 (from the small offsets of the
+//    points from the centroids) in FP32, as the other mixed kernels (far_split.h); with dof blocks
+//    the far pairs go through the tiled regular kernel (regular_mfie.hip) instead;
+//  * near pairs of the mixed mode (pair_is_near), FP64: one lane per candidate pair
+//    (bem_near_assemble);
 //  * singular, FP64: one 64-lane wavefront per Sauter-Schwab pair, its Duffy points strided over
 //    the lanes, the 3 x 3 block reduced across the wave.
 #include "bem_internal.h"
 #include "far_split.h"
+#include "mfie_block.h"
 
 namespace bem {
 namespace mfie {
@@ -24,7 +26,6 @@ namespace mfie {
 constexpr int kThreads = 256;
 constexpr int kTile = 16;  // test elements per workgroup
 constexpr int kQ = 6;      // quadrature points of the regular kernel (the order-4 rule)
-constexpr double kOx[3] = {0.0, 1.0, 0.0}, kOy[3] = {1.0, 0.0, 0.0};
 constexpr double kInv2Pi = 0.15915494309189533577;  // phases in revolutions
 
 struct Args {
@@ -56,6 +57,7 @@ __device__ __forceinline__ void piola_j(const double *e1, const double *e2, doub
 #pragma unroll
         for (int d = 0; d < 3; ++d) A[i][d] = e1[d] * (xi - kOx[i]) + e2[d] * (eta - kOy[i]);
 }
+
 
 // acc[i][j] += wgt G (ik r - 1) / r^2 (x - y) . (A_i x B_j), complex wavenumber k = kr + i ki
 __device__ __forceinline__ void accumulate(const double (&diff)[3], const double (&A)[3][3], const double (&B)[3][3],
@@ -125,18 +127,67 @@ __device__ __forceinline__ void scatter(const Args &a, int e, int f, const doubl
     atomic_add(o + 1, (OutT)(ai[i][j] * m));
 }
 
+// The 3 x 3 block of a regular pair in FP64 (moments form of mfie_block.h): ap(p) the offsets of
+// the test points from the test centroid, e1 / e2 the test edge vectors, b the trial offsets, kap
+// c_f - Q_j of the trial element, fd = c_f - c_e.
+template <typename AP>
+__device__ __forceinline__ void pair_block_fp64(AP ap_of, const double *e1, const double *e2, const double (&b)[kQ][3],
+                                                const double (&kap)[3][3], const double (&fd)[3], const RuleArgs &rule,
+                                                double kr, double ki, double (&ar)[3][3], double (&ai)[3][3]) {
+    PairMoments<double> M;
+    M.clear();
+#pragma unroll 1
+    for (int p = 0; p < kQ; ++p) {
+        double ap[3];
+        ap_of(p, ap);
+        const double d0 = ap[0] - fd[0], d1 = ap[1] - fd[1], d2 = ap[2] - fd[2];
+        double Rp[2] = {0.0, 0.0}, Yp[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) {
+            const double u0 = d0 - b[q][0], u1 = d1 - b[q][1], u2 = d2 - b[q][2];  // x_p - y_q
+            const double r2 = u0 * u0 + u1 * u1 + u2 * u2;
+            const double r = sqrt(r2);
+            double sn, cs;
+            sincos(kr * r, &sn, &cs);
+            const double damp = ki != 0.0 ? exp(-ki * r) : 1.0;
+            const double g = rule.w[p] * rule.w[q] * kInv4Pi * damp / (r * r2);
+            const double fr = -ki * r - 1.0, fi = kr * r;
+            const double vr = g * (cs * fr - sn * fi), vi = g * (cs * fi + sn * fr);
+            Rp[0] += vr;
+            Rp[1] += vi;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                Yp[0][d] += vr * b[q][d];
+                Yp[1][d] += vi * b[q][d];
+            }
+        }
+        M.add_point(Rp, Yp, ap, fd);
+    }
+    double pi[3][3];
+    rwg_offsets(e1, e2, pi);
+    M.block(pi, kap, fd, ar, ai);
+}
+
+// offsets of the quadrature points from the centroid, b_q = e1 (x_q - 1/3) + e2 (y_q - 1/3)
+__device__ __forceinline__ void centroid_offsets(const double *g, const RuleArgs &rule, double (&b)[kQ][3]) {
+#pragma unroll
+    for (int q = 0; q < kQ; ++q)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) b[q][d] = g[3 + d] * (rule.x[q] - 1.0 / 3.0) + g[6 + d] * (rule.y[q] - 1.0 / 3.0);
+}
+
 struct Tile {
     int elem[kTile];
     int vid[kTile][3];
-    double x[kTile][kQ][3];
+    double a[kTile][kQ][3];  // x_p - c_e
     double e1[kTile][3], e2[kTile][3];
-    double c[kTile][3];  // centroids (NEAR_ONLY)
+    double c[kTile][3];  // centroids
     double diam[kTile];
     int dof[kTile][3];
     double ml[kTile][3];  // multiplier x edge length
 };
 
-template <typename OutT, bool NEAR_ONLY = false>
+template <typename OutT>
 __global__ __launch_bounds__(kThreads) void mfie_regular_kernel(const Args a) {
     __shared__ Tile T_;
     const int tid = threadIdx.x;
@@ -154,7 +205,8 @@ __global__ __launch_bounds__(kThreads) void mfie_regular_kernel(const Args a) {
             T_.e2[tid][d] = g[6 + d];
         }
         for (int p = 0; p < kQ; ++p)
-            for (int d = 0; d < 3; ++d) T_.x[tid][p][d] = g[d] + g[3 + d] * a.rule.x[p] + g[6 + d] * a.rule.y[p];
+            for (int d = 0; d < 3; ++d)
+                T_.a[tid][p][d] = g[3 + d] * (a.rule.x[p] - 1.0 / 3.0) + g[6 + d] * (a.rule.y[p] - 1.0 / 3.0);
         centroid_rn(g, T_.c[tid]);
         T_.diam[tid] = g[13];
         dof_factors(a.test, e, T_.dof[tid], T_.ml[tid]);
@@ -169,11 +221,10 @@ __global__ __launch_bounds__(kThreads) void mfie_regular_kernel(const Args a) {
     int fdof[3];
     double fml[3];
     dof_factors(a.trial, f, fdof, fml);
-    double yq[kQ][3];
-#pragma unroll
-    for (int q = 0; q < kQ; ++q)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) yq[q][d] = gf[d] + gf[3 + d] * a.rule.x[q] + gf[6 + d] * a.rule.y[q];
+    // trial points as offsets b_q from the centroid
+    double b[kQ][3], kap[3][3];
+    centroid_offsets(gf, a.rule, b);
+    rwg_offsets(gf + 3, gf + 6, kap);
     for (int t = 0; t < ntile; ++t) {
         if (a.identical) {
             bool adj = false;
@@ -183,29 +234,48 @@ __global__ __launch_bounds__(kThreads) void mfie_regular_kernel(const Args a) {
             }
             if (adj) continue;
         }
-        if constexpr (NEAR_ONLY) {
-            double dist2;
-            if (!pair_is_near(T_.c[t], cf, T_.diam[t], gf[13], a.near_factor, &dist2)) continue;
-        }
+        const double fd[3] = {cf[0] - T_.c[t][0], cf[1] - T_.c[t][1], cf[2] - T_.c[t][2]};
         double ar[3][3], ai[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) ar[i][j] = ai[i][j] = 0.0;
-#pragma unroll 1
-        for (int p = 0; p < kQ; ++p) {
-            double A[3][3];
-            piola_j(T_.e1[t], T_.e2[t], a.rule.x[p], a.rule.y[p], A);
-#pragma unroll
-            for (int q = 0; q < kQ; ++q) {
-                double B[3][3];
-                piola_j(gf + 3, gf + 6, a.rule.x[q], a.rule.y[q], B);
-                const double diff[3] = {T_.x[t][p][0] - yq[q][0], T_.x[t][p][1] - yq[q][1], T_.x[t][p][2] - yq[q][2]};
-                accumulate(diff, A, B, a.rule.w[p] * a.rule.w[q], a.kr, a.ki, ar, ai);
-            }
-        }
+        pair_block_fp64([&](int p, double (&ap)[3]) {
+            for (int d = 0; d < 3; ++d) ap[d] = T_.a[t][p][d];
+        }, T_.e1[t], T_.e2[t], b, kap, fd, a.rule, a.kr, a.ki, ar, ai);
         scatter_block<OutT>(a, T_.dof[t], T_.ml[t], fdof, fml, ar, ai);
     }
+}
+
+// ------------------------------------------------------------------ near pairs (mixed mode), FP64
+// one lane per candidate pair (e, f) of a.pairs ((n, 2) rows); pairs that share a vertex (grids
+// identical) or lie outside the near factor are skipped
+template <typename OutT>
+__global__ __launch_bounds__(kThreads) void mfie_near_pairs_kernel(const Args a) {
+    const int p = blockIdx.x * kThreads + threadIdx.x;
+    if (p >= a.n_pairs) return;
+    const int e = a.pairs[2 * (int64_t)p], f = a.pairs[2 * (int64_t)p + 1];
+    const double *gt = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
+    const double *gf = a.trial.geom + (int64_t)f * BEM_GEOM_STRIDE;
+    if (a.identical) {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                if (a.test.vids[4 * e + i] == a.trial.vids[4 * f + j]) return;
+    }
+    double ct[3], cf[3], dist2;
+    centroid_rn(gt, ct);
+    centroid_rn(gf, cf);
+    if (!pair_is_near(ct, cf, gt[13], gf[13], a.near_factor, &dist2)) return;
+    double at[kQ][3], b[kQ][3], kap[3][3];
+    centroid_offsets(gt, a.rule, at);
+    centroid_offsets(gf, a.rule, b);
+    rwg_offsets(gf + 3, gf + 6, kap);
+    const double fd[3] = {cf[0] - ct[0], cf[1] - ct[1], cf[2] - ct[2]};
+    double ar[3][3], ai[3][3];
+    pair_block_fp64([&](int q, double (&ap)[3]) {
+        for (int d = 0; d < 3; ++d) ap[d] = at[q][d];
+    }, gt + 3, gt + 6, b, kap, fd, a.rule, a.kr, a.ki, ar, ai);
+    int tdof[3], fdof[3];
+    double tml[3], fml[3];
+    dof_factors(a.test, e, tdof, tml);
+    dof_factors(a.trial, f, fdof, fml);
+    scatter_block<OutT>(a, tdof, tml, fdof, fml, ar, ai);
 }
 
 // ------------------------------------------------------------------ mixed: far pairs in FP32
@@ -254,19 +324,16 @@ __global__ __launch_bounds__(kThreads) void mfie_regular_mixed_kernel(const Args
     int fdof[3];
     double fml[3];
     dof_factors(a.trial, f, fdof, fml);
-    // trial points as offsets from the centroid and the Piola vectors B_j(q) (times J), FP32
-    float b[kQ][3], B[kQ][3][3], wq[kQ];
+    // trial points as offsets b_q from the centroid, FP32
+    float b[kQ][3], kap[3][3], wq[kQ];
 #pragma unroll
     for (int q = 0; q < kQ; ++q) {
         const double s0 = a.rule.x[q], s1 = a.rule.y[q];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            b[q][d] = (float)(gf[3 + d] * (s0 - 1.0 / 3.0) + gf[6 + d] * (s1 - 1.0 / 3.0));
-#pragma unroll
-            for (int j = 0; j < 3; ++j) B[q][j][d] = (float)(gf[3 + d] * (s0 - kOx[j]) + gf[6 + d] * (s1 - kOy[j]));
-        }
+        for (int d = 0; d < 3; ++d) b[q][d] = (float)(gf[3 + d] * (s0 - 1.0 / 3.0) + gf[6 + d] * (s1 - 1.0 / 3.0));
         wq[q] = (float)(a.rule.w[q] * kInv4Pi);
     }
+    rwg_offsets(gf + 3, gf + 6, kap);
     const float kr = (float)a.kr, ki = (float)a.ki, krev = (float)(a.kr * kInv2Pi);
     for (int t = 0; t < ntile; ++t) {
         if (a.identical) {
@@ -278,28 +345,21 @@ __global__ __launch_bounds__(kThreads) void mfie_regular_mixed_kernel(const Args
             if (adj) continue;
         }
         double dist2;
-        if (pair_is_near(T_.c[t], cf, T_.diam[t], gf[13], a.near_factor, &dist2)) continue;  // FP64 kernel
+        if (pair_is_near(T_.c[t], cf, T_.diam[t], gf[13], a.near_factor, &dist2)) continue;  // near pass
         const FarPair P = far_pair_setup(T_.c[t], cf);  // fd = c_f - c_e
         float cs0, sn0;
         phase_rev_f32(a.kr, (double)P.r0 * kInv2Pi, cs0, sn0);
         const float damp0 = a.ki != 0.0 ? (float)exp(-a.ki * (double)P.r0) : 1.f;
         cs0 *= damp0;
         sn0 *= damp0;
-        float ar[3][3], ai[3][3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) ar[i][j] = ai[i][j] = 0.f;
+        const float fd[3] = {P.fd0, P.fd1, P.fd2};
+        PairMoments<float> M;
+        M.clear();
 #pragma unroll 1
         for (int p = 0; p < kQ; ++p) {
-            const float xi = (float)a.rule.x[p], eta = (float)a.rule.y[p], wp = (float)a.rule.w[p];
-            float A[3][3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int d = 0; d < 3; ++d)
-                    A[i][d] = T_.e1[t][d] * (xi - (float)kOx[i]) + T_.e2[t][d] * (eta - (float)kOy[i]);
+            const float wp = (float)a.rule.w[p];
             const float ap0 = T_.a[t][p][0], ap1 = T_.a[t][p][1], ap2 = T_.a[t][p][2];
+            float Rp[2] = {0.f, 0.f}, Yp[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
 #pragma unroll
             for (int q = 0; q < kQ; ++q) {
                 // x - y = (c_e + a_p) - (c_f + b_q) = u - fd with u = a_p - b_q small
@@ -320,21 +380,20 @@ __global__ __launch_bounds__(kThreads) void mfie_regular_mixed_kernel(const Args
                 const float mag = wp * wq[q] * rinv * rinv * rinv;
                 const float fr = -ki * r - 1.f, fi = kr * r;
                 const float vr = mag * (er * fr - ei * fi), vi = mag * (er * fi + ei * fr);
-                const float w0 = u0 - P.fd0, w1 = u1 - P.fd1, w2 = u2 - P.fd2;
+                Rp[0] += vr;
+                Rp[1] += vi;
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    const float D0 = w1 * A[i][2] - w2 * A[i][1];
-                    const float D1 = w2 * A[i][0] - w0 * A[i][2];
-                    const float D2 = w0 * A[i][1] - w1 * A[i][0];
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) {
-                        const float sij = D0 * B[q][j][0] + D1 * B[q][j][1] + D2 * B[q][j][2];
-                        ar[i][j] += vr * sij;
-                        ai[i][j] += vi * sij;
-                    }
+                for (int d = 0; d < 3; ++d) {
+                    Yp[0][d] += vr * b[q][d];
+                    Yp[1][d] += vi * b[q][d];
                 }
             }
+            const float ap[3] = {ap0, ap1, ap2};
+            M.add_point(Rp, Yp, ap, fd);
         }
+        float pi[3][3], ar[3][3], ai[3][3];
+        rwg_offsets(T_.e1[t], T_.e2[t], pi);
+        M.block(pi, kap, fd, ar, ai);
         scatter_block<OutT>(a, T_.dof[t], T_.ml[t], fdof, fml, ar, ai);
     }
 }
@@ -390,12 +449,16 @@ __global__ __launch_bounds__(kThreads) void mfie_singular_kernel(const Args a) {
 }  // namespace mfie
 
 // regular pairs of the MFIE (out_dtype BEM_C64 or BEM_C128), added to the output: FP64, or
-// MIXED (far pairs FP32, near pairs FP64)
+// MIXED (far pairs only, FP32; the near pairs are left to mfie_near_launch)
 int mfie_regular_launch(const SpaceArgs &test, const SpaceArgs &trial, const RuleArgs &rule, double kr, double ki,
                         int identical, int precision, double near_factor, void *out, int64_t ld, int out_dtype,
                         hipStream_t s) {
     if (rule.n != mfie::kQ) {
         set_error("the magnetic field operator uses the 6-point regular rule (order 4), got %d points", rule.n);
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (out_dtype != BEM_C128 && out_dtype != BEM_C64) {
+        set_error("the magnetic field operator needs complex output");
         return BEM_ERR_INVALID_ARGUMENT;
     }
     if (test.n == 0 || trial.n == 0) return BEM_OK;
@@ -414,25 +477,47 @@ int mfie_regular_launch(const SpaceArgs &test, const SpaceArgs &trial, const Rul
     a.identical = identical;
     a.out = out;
     a.ld = ld;
-    if (out_dtype != BEM_C128 && out_dtype != BEM_C64) {
-        set_error("the magnetic field operator needs complex output");
-        return BEM_ERR_INVALID_ARGUMENT;
-    }
     const dim3 grid((trial.n + mfie::kThreads - 1) / mfie::kThreads, (unsigned)ntiles);
     const bool c128 = out_dtype == BEM_C128;
     if (precision == BEM_PRECISION_MIXED) {
-        if (c128) {
-            hipLaunchKernelGGL(mfie::mfie_regular_mixed_kernel<double>, grid, dim3(mfie::kThreads), 0, s, a);
-            hipLaunchKernelGGL((mfie::mfie_regular_kernel<double, true>), grid, dim3(mfie::kThreads), 0, s, a);
-        } else {
-            hipLaunchKernelGGL(mfie::mfie_regular_mixed_kernel<float>, grid, dim3(mfie::kThreads), 0, s, a);
-            hipLaunchKernelGGL((mfie::mfie_regular_kernel<float, true>), grid, dim3(mfie::kThreads), 0, s, a);
-        }
+        if (c128) hipLaunchKernelGGL(mfie::mfie_regular_mixed_kernel<double>, grid, dim3(mfie::kThreads), 0, s, a);
+        else hipLaunchKernelGGL(mfie::mfie_regular_mixed_kernel<float>, grid, dim3(mfie::kThreads), 0, s, a);
         return check_launch("mfie_regular_mixed_kernel");
     }
     if (c128) hipLaunchKernelGGL(mfie::mfie_regular_kernel<double>, grid, dim3(mfie::kThreads), 0, s, a);
     else hipLaunchKernelGGL(mfie::mfie_regular_kernel<float>, grid, dim3(mfie::kThreads), 0, s, a);
     return check_launch("mfie_regular_kernel");
+}
+
+// near pairs of the mixed mode among n_pairs candidates ((n, 2) int32 rows), FP64, added to the output
+int mfie_near_launch(const SpaceArgs &test, const SpaceArgs &trial, const RuleArgs &rule, double kr, double ki,
+                     int identical, double near_factor, int n_pairs, const int32_t *pairs, void *out, int64_t ld,
+                     int out_dtype, hipStream_t s) {
+    if (rule.n != mfie::kQ) {
+        set_error("the magnetic field operator uses the 6-point regular rule (order 4), got %d points", rule.n);
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (out_dtype != BEM_C128 && out_dtype != BEM_C64) {
+        set_error("the magnetic field operator needs complex output");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (n_pairs == 0) return BEM_OK;
+    mfie::Args a{};
+    a.test = test;
+    a.trial = trial;
+    a.rule = rule;
+    a.kr = kr;
+    a.ki = ki;
+    a.near_factor = near_factor;
+    a.identical = identical;
+    a.out = out;
+    a.ld = ld;
+    a.n_pairs = n_pairs;
+    a.pairs = pairs;
+    const dim3 grid((unsigned)((n_pairs + mfie::kThreads - 1) / mfie::kThreads));
+    if (out_dtype == BEM_C128) hipLaunchKernelGGL(mfie::mfie_near_pairs_kernel<double>, grid, dim3(mfie::kThreads), 0, s, a);
+    else hipLaunchKernelGGL(mfie::mfie_near_pairs_kernel<float>, grid, dim3(mfie::kThreads), 0, s, a);
+    return check_launch("mfie_near_pairs_kernel");
 }
 
 // singular pairs of the MFIE, added to the output

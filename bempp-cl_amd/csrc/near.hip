@@ -301,6 +301,9 @@ static int near_entry(int op, const bem_space_t *test, const bem_space_t *trial,
     case BEM_MAXWELL_ELECTRIC_FIELD: return near_launch_op<kMaxwellEFIE>(a, out_dtype, s);
     case BEM_LAPLACE_HYPERSINGULAR: return near_launch_op<kLaplaceHS>(a, out_dtype, s);
     case BEM_HELMHOLTZ_HYPERSINGULAR: return near_launch_op<kHelmholtzHS>(a, out_dtype, s);
+    case BEM_MAXWELL_MAGNETIC_FIELD:
+        return mfie_near_launch(a.test, a.trial, a.rule, k_re, k_im, grids_identical, near_factor, n_pairs, pairs, out,
+                                ld, out_dtype, s);
     default: set_error("unsupported operator %d", op); return BEM_ERR_UNSUPPORTED;
     }
 }

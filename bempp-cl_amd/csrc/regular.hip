@@ -33,16 +33,16 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
         set_error("scalar operators need scalar (P0/P1) spaces");
         return BEM_ERR_INVALID_ARGUMENT;
     }
-    if (op == BEM_MAXWELL_MAGNETIC_FIELD) {
-        // adds to the output (dof blocks, if given, are not used); MIXED includes the FP64 near pairs
-        if (out2 || work || upper) {
-            set_error("the magnetic field operator has no fused or symmetric pass");
-            return BEM_ERR_UNSUPPORTED;
-        }
+    if (op == BEM_MAXWELL_MAGNETIC_FIELD && (out2 || work || upper)) {
+        set_error("the magnetic field operator has no fused or symmetric pass");
+        return BEM_ERR_UNSUPPORTED;
+    }
+    const bool tile = tblocks != nullptr;
+    if (op == BEM_MAXWELL_MAGNETIC_FIELD && !(tile && precision == BEM_PRECISION_MIXED)) {
+        // adds to the output: FP64, or without dof blocks the untiled mixed kernel (far pairs)
         return mfie_regular_launch(to_args(test), to_args(trial), to_args(rule), k_re, k_im, grids_identical,
                                    precision, mixed_near_factor, out, ld, out_dtype, (hipStream_t)stream);
     }
-    const bool tile = tblocks != nullptr;
     if ((tblocks == nullptr) != (sblocks == nullptr)) {
         set_error("dof blocks must be given for both sides or for none");
         return BEM_ERR_INVALID_ARGUMENT;
@@ -75,6 +75,7 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
                          sblocks->n_blocks, sblocks->max_dofs, sblocks->max_incidence};
     }
     hipStream_t s = (hipStream_t)stream;
+    if (op == BEM_MAXWELL_MAGNETIC_FIELD) return regular_launch_mfie(a, out_dtype, s);  // mixed, tiled: far pairs
     if (upper) {
         if (op == BEM_LAPLACE_SINGLE_LAYER) return regular_launch_upper_0(a, precision, kahan, out_dtype, s);
         if (op == BEM_HELMHOLTZ_SINGLE_LAYER) return regular_launch_upper_3(a, precision, kahan, out_dtype, s);

@@ -39,6 +39,7 @@
 
 #include "bem_device.h"
 #include "bem_internal.h"
+#include "mfie_block.h"
 
 namespace bem {
 
@@ -572,6 +573,98 @@ __device__ __forceinline__ void hs_block_f32(const C2f (&S)[3][3], const float (
         }
 }
 
+// Maxwell magnetic field (MFIE) block of a far pair in FP32 (mixed mode), from the moments of
+// mfie_block.h: S[i][j] = sum_pq w_p J_t w_q J_f G (ikr - 1) / r^2 (x_p - y_q) . ((x_p - P_i) x (y_q - Q_j))
+// with G = e^{ikr} / (4 pi r) (bempp_cl/core/numba_kernels.py:2151-2277); mfie_block_f32 scales it
+// by l_i l_j / (J_t J_f). The distance uses the reference-distance split of pair_fp32: the phase
+// e^{i k r0} is formed once per pair in FP64 and only r - r0 comes from the small FP32 offsets.
+// kap: c_f - Q_j of the trial element.
+template <int QT, typename TT>
+__device__ __forceinline__ void mfie_pair_fp32(const TT &T_, int t, const TrialF32<QT> &F, const float (&kap)[3][3],
+                                               double d0, double d1, double d2, double dist2, const RuleArgs &rule,
+                                               double kr, double ki, C2f (&S)[3][3]) {
+    const int nq = QT > 0 ? QT : rule.n;
+    const float fd[3] = {(float)d0, (float)d1, (float)d2};
+    const float r0 = sqrtf((float)dist2);
+    const float r0sq = r0 * r0;
+    const float resid = (float)(dist2 - (double)r0 * (double)r0);
+    const float kr_rev = (float)(kr * kInv2Pi), kr_f = (float)kr, ki_f = (float)ki;
+    const float ki_log2e = (float)(ki * 1.4426950408889634);
+    // e^{i k r0} (with e^{-Im k r0}), the phase reduced to one revolution in FP64
+    double rev = kr * (double)r0 * kInv2Pi;
+    rev -= rint(rev);
+    float c0 = cos_rev((float)rev), s0 = sin_rev((float)rev);
+    if (ki != 0.0) {
+        const float dm = (float)exp(-ki * (double)r0);
+        c0 *= dm;
+        s0 *= dm;
+    }
+    mfie::PairMoments<float> M;
+    M.clear();
+#pragma unroll 1
+    for (int p = 0; p < nq; ++p) {
+        const float ap[3] = {T_.a[t][p][0], T_.a[t][p][1], T_.a[t][p][2]};
+        // x_p - y_q = w - b_q with w = a_p - d; |x - y|^2 - r0^2 = Ap - 2 w.b_q + |b_q|^2
+        const float w0 = ap[0] - fd[0], w1 = ap[1] - fd[1], w2 = ap[2] - fd[2];
+        const float Ap = resid - 2.f * (fd[0] * ap[0] + fd[1] * ap[1] + fd[2] * ap[2]) + T_.aa[t][p];
+        const float tw0 = 2.f * w0, tw1 = 2.f * w1, tw2 = 2.f * w2;
+        const float wp = T_.wp[t][p];
+        float Rp[2] = {0.f, 0.f}, Yp[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+#pragma unroll(QT > 0 ? QT : 1)
+        for (int q = 0; q < nq; ++q) {
+            const float tt = Ap + F.bb[q] - (tw0 * F.b[q][0] + tw1 * F.b[q][1] + tw2 * F.b[q][2]);
+            const float r2 = r0sq + tt;
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const float r = r2 * rinv;
+            const float delta = tt * __builtin_amdgcn_rcpf(r + r0);  // r - r0
+            const float ph = kr_rev * delta;
+            const float c = cos_rev(ph), sn = sin_rev(ph);
+            float er = c0 * c - s0 * sn, ei = c0 * sn + s0 * c;
+            if (ki_f != 0.f) {
+                const float dm = __builtin_amdgcn_exp2f(-ki_log2e * delta);
+                er *= dm;
+                ei *= dm;
+            }
+            // e^{ikr} ((-Im k r - 1) + i Re k r) w_p J_t w_q J_f / (4 pi r^3)
+            const float mag = wp * F.wq[q] * rinv * rinv * rinv;
+            const float fr = -ki_f * r - 1.f, fi = kr_f * r;
+            const float vr = mag * (er * fr - ei * fi), vi = mag * (er * fi + ei * fr);
+            Rp[0] += vr;
+            Rp[1] += vi;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                Yp[0][d] += vr * F.b[q][d];
+                Yp[1][d] += vi * F.b[q][d];
+            }
+        }
+        M.add_point(Rp, Yp, ap, fd);
+    }
+    float pi[3][3], ar[3][3], ai[3][3];
+    mfie::rwg_offsets(T_.ee[t][0], T_.ee[t][1], pi);
+    M.block(pi, kap, fd, ar, ai);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            S[i][j].re = ar[i][j];
+            S[i][j].im = ai[i][j];
+        }
+}
+
+// V_ij = S_ij l_i l_j / (J_t J_f): lt, lf edge lengths, jti, jfi the inverse Jacobians
+__device__ __forceinline__ void mfie_block_f32(const C2f (&S)[3][3], const float (&lt)[3], float jti,
+                                               const float (&lf)[3], float jfi, C2f (&V)[3][3]) {
+    const float jj = jti * jfi;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const float m = lt[i] * lf[j] * jj;
+            V[i][j].re = S[i][j].re * m;
+            V[i][j].im = S[i][j].im * m;
+        }
+}
+
 // Edge vectors, edge lengths (|v1 - v0|, |v2 - v0|, |v1 - v2|) and 1 / J of a geometry row, in FP32.
 __device__ __forceinline__ void efie_geom_f32(const double *g, float (&e)[2][3], float (&l)[3], float &jinv) {
 #pragma unroll
@@ -625,7 +718,7 @@ __device__ __forceinline__ void store_direct(void *out, int64_t ld, int row, int
 // the symmetric pass and the mixed EFIE pass keep more state (2); the others leave it to the compiler
 template <int OP, bool MIXED, bool DUAL, bool SYM>
 constexpr int regular_waves() {
-    return !MIXED ? 1 : SYM ? 2 : DUAL ? 3 : OP == kMaxwellEFIE ? 2 : 1;
+    return !MIXED ? 1 : SYM ? 2 : DUAL ? 3 : is_rwg_op<OP>() ? 2 : 1;
 }
 
 // UPPER (direct DP0 stores, a symmetric operator on one space): only the pairs whose trial element
@@ -657,7 +750,10 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
     constexpr bool EFX = OP == kMaxwellEFIE && MIXED;
     // HSX: the MIXED hypersingular block is formed in FP32 from staged edge vectors and nm / J
     constexpr bool HSX = is_hypersingular<OP>() && MIXED;
-    __shared__ TestTile<TMC, MIXED ? 1 : TMC, QC, EFX || HSX> T_;
+    // MFX: the MIXED magnetic field block (mfie_pair_fp32 / mfie_block_f32); there is no FP64 or untiled mode
+    constexpr bool MFX = OP == kMaxwellMFIE;
+    static_assert(!MFX || (MIXED && TILE && !DUAL && NTS == 3 && NSS == 3), "the tiled MFIE pass is mixed precision");
+    __shared__ TestTile<TMC, MIXED ? 1 : TMC, QC, EFX || HSX || MFX> T_;
     __shared__ AccT acc[TILE ? MAXU_T * MAXU_S * NC : 1];
     __shared__ AccT acc2[TILE && DUAL ? MAXU_T * MAXU_S * NC : 1];
     __shared__ AccT acc3[TILE && SYM ? MAXU_T * MAXU_S * NC : 1];
@@ -716,7 +812,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                 T_.nx[el][0] = (float)(g[9] * nm);
                 T_.nx[el][1] = (float)(g[10] * nm);
                 T_.nx[el][2] = (float)(g[11] * nm);
-                if constexpr (EFX || HSX) efie_geom_f32(g, T_.ee[el], T_.el[el], T_.ej[el]);
+                if constexpr (EFX || HSX || MFX) efie_geom_f32(g, T_.ee[el], T_.el[el], T_.ej[el]);
                 if constexpr (HSX) T_.ej[el] *= (float)nm;
             } else {
                 const int p = (task - n) - el * nq;
@@ -751,6 +847,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
     double yq[QT > 0 ? QT : 1][3], wq64[QT > 0 ? QT : 1], ny64[3] = {0, 0, 0};
     float ef[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}}, lf[3] = {0.f, 0.f, 0.f}, jfi = 0.f;  // EFX
     float ikr = 0.f, iki = 0.f, invr = 0.f, invi = 0.f;  // EFX: ik = -Im k + i Re k and 1 / (ik)
+    float kapf[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};  // MFX: c_f - Q_j
     const float k2r = (float)(a.kr * a.kr - a.ki * a.ki), k2i = (float)(2.0 * a.kr * a.ki);  // HSX: k^2
     if constexpr (EFX) {
         const double den = a.ki * a.ki + a.kr * a.kr;
@@ -785,8 +882,9 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
         ny64[0] = gf[9] * snm;
         ny64[1] = gf[10] * snm;
         ny64[2] = gf[11] * snm;
-        if constexpr (EFX || HSX) efie_geom_f32(gf, ef, lf, jfi);
+        if constexpr (EFX || HSX || MFX) efie_geom_f32(gf, ef, lf, jfi);
         if constexpr (HSX) jfi *= (float)snm;
+        if constexpr (MFX) mfie::rwg_offsets(ef[0], ef[1], kapf);
         if constexpr (MIXED) {
             F32.ny[0] = (float)ny64[0];
             F32.ny[1] = (float)ny64[1];
@@ -818,7 +916,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
     // tid + 256 when a block has more slots than lanes) in this block, as stage offsets of row
     // (j, i = 0) and column p; missing ones point at the zero column
     constexpr int OWNS = OWNER && MAXU_S > kRegThreads ? 2 : 1;
-    constexpr int KINC = OP == kMaxwellEFIE ? kRwgMaxInc : kMaxInc;
+    constexpr int KINC = is_rwg_op<OP>() ? kRwgMaxInc : kMaxInc;
     int own_off[OWNER ? OWNS : 1][OWNER ? KINC : 1];
     int own_nds = 0;
     if constexpr (OWNER) {
@@ -906,7 +1004,9 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                     skip = pair_is_near(T_.c[te], cfv, T_.diam[te], diam_f, a.near_factor, &dist2);
                     if (!skip) {
                         double d0 = cf0 - T_.c[te][0], d1 = cf1 - T_.c[te][1], d2 = cf2 - T_.c[te][2];
-                        if (OpTraits<OP>::is_complex && a.ki != 0.0)
+                        if constexpr (MFX)
+                            mfie_pair_fp32<QT>(T_, te, F32, kapf, d0, d1, d2, dist2, a.rule, a.kr, a.ki, Sf);
+                        else if (OpTraits<OP>::is_complex && a.ki != 0.0)
                             pair_fp32<GOP, NTS, NSS, QT, KAHAN, true, DUAL, SYM>(T_, te, F32, d0, d1, d2, dist2, a.rule,
                                                                                        a.kr, a.ki, Sf, Sf2, Sf3);
                         else
@@ -942,6 +1042,8 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                         efie_block_f32(Sm, T_.ee[te], T_.el[te], T_.ej[te], ef, lf, jfi, ikr, iki, invr, invi, V);
                     else if constexpr (HSX)
                         hs_block_f32<CPLX>(Sm, T_.ee[te], T_.ej[te], T_.nx[te], ef, jfi, F32.ny, k2r, k2i, V);
+                    else if constexpr (MFX)
+                        mfie_block_f32(Sm, T_.el[te], T_.ej[te], lf, jfi, V);
                     else
                         scalar_block_f32<NTS, NSS>(Sm, V);
     #pragma unroll
@@ -1083,7 +1185,7 @@ static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
     // at 3 workgroups per CU
     // mixed EFIE: trial blocks of up to BEM_TILE_RWG_MIXED_TRIAL_DOFS edges (a 256-element patch),
     // owner lanes with two slots each, smaller test tiles to stay at 2 workgroups per CU
-    constexpr bool RWGM = OP == kMaxwellEFIE && MIXED;
+    constexpr bool RWGM = is_rwg_op<OP>() && MIXED;
     constexpr int TM = SYM ? BEM_TILE_SYM_TEST_ELEMENTS
                      : DUAL ? BEM_TILE_PAIR_TEST_ELEMENTS
                      : RWGM ? BEM_TILE_RWG_MIXED_TEST_ELEMENTS
@@ -1095,7 +1197,7 @@ static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
                      : RWGM ? BEM_TILE_RWG_MIXED_TEST_DOFS
                             : BEM_TILE_TEST_DOFS;
     constexpr int MS = RWGM ? BEM_TILE_RWG_MIXED_TRIAL_DOFS : BEM_TILE_TRIAL_DOFS;
-    constexpr int KMAX = OP == kMaxwellEFIE ? kRwgMaxInc : kMaxInc;
+    constexpr int KMAX = is_rwg_op<OP>() ? kRwgMaxInc : kMaxInc;
     (void)CPLX;
     dim3 grid;
     if (tile) {
@@ -1133,9 +1235,13 @@ static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
             set_error("the symmetric DP0 pass stores directly (no dof blocks)");
             return BEM_ERR_INVALID_ARGUMENT;
         }
-        if (MIXED && kahan) BEM_LAUNCH(true, true);
-        else BEM_LAUNCH(false, true);
-    } else if constexpr (!SYM) {
+        if constexpr (OP == kMaxwellMFIE) {
+            BEM_LAUNCH(false, true);
+        } else {
+            if (MIXED && kahan) BEM_LAUNCH(true, true);
+            else BEM_LAUNCH(false, true);
+        }
+    } else if constexpr (!SYM && OP != kMaxwellMFIE) {
         if (MIXED && kahan) BEM_LAUNCH(true, false);
         else BEM_LAUNCH(false, false);
     }
@@ -1219,6 +1325,8 @@ BEM_DECLARE_REGULAR_OP(7)
 BEM_DECLARE_REGULAR_OP(8)
 #undef BEM_DECLARE_REGULAR_OP
 // single layer + double layer in one pass (regular_dual<k>.hip, k = the single layer)
+// far pairs of the magnetic field operator in mixed precision (tiled, owner reduction; regular_mfie.hip)
+int regular_launch_mfie(const RegularArgs &a, int out_dtype, hipStream_t s);
 int regular_launch_dual_0(const RegularArgs &a, int precision, int kahan, bool tile, int out_dtype, hipStream_t s);
 int regular_launch_dual_3(const RegularArgs &a, int precision, int kahan, bool tile, int out_dtype, hipStream_t s);
 // symmetric single + double layer pass (regular_sym<k>.hip)

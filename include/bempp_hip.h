@@ -157,6 +157,9 @@ typedef struct bem_dof_blocks {
  *   distance is below factor * (diameter_test + diameter_trial) / 2 use FP64.
  * kahan: compensated FP32 summation in MIXED mode.
  * out_dtype must be BEM_F64/BEM_F32 for Laplace and BEM_C128/BEM_C64 otherwise.
+ * BEM_MAXWELL_MAGNETIC_FIELD (bempp_cl/core/numba_kernels.py:2151-2277) takes the 6-point rule
+ * and always adds to `out` through both entry points; in MIXED mode the blocked entry point
+ * runs its far pairs through the LDS-tiled kernel.
  */
 int bem_regular_assemble(int op, const bem_space_t *test, const bem_space_t *trial, int grids_identical,
                          const bem_quad_rule_t *rule, double k_re, double k_im, int precision,

@@ -128,6 +128,54 @@ def test_magnetic_field_vs_oracle(k, ico2, default_parameters, hip_available):
     assert rel_fro(u, uref) < 1e-12
 
 
+@pytest.mark.parametrize("k", [2.0, 1.5 + 0.4j])
+def test_magnetic_field_regular_passes(k, hip_available):
+    """The MFIE's regular pairs through the library's three paths on icosphere(3): FP64, mixed
+    without dof blocks (one lane per trial element, atomic scatter) and mixed with RWG dof blocks
+    (the tiled kernel's owner reduction). The mixed passes leave the near pairs to the FP64 near
+    pass (bem_near_assemble over candidate pairs); with it they agree with FP64 to FP32 rounding,
+    without it they miss the near pairs entirely."""
+    import torch
+
+    from bempp_amd.api import function_space
+    from bempp_amd.api.grid.shapes import icosphere
+    from bempp_amd.api.integration.triangle_gauss import rule as triangle_rule
+    from bempp_amd.core import hip_backend as hb
+    from bempp_amd.api.utils.parameters import DEFAULT_NEAR_FIELD_FACTOR as near
+    from bempp_amd.core.blocking import device_blocks
+    from bempp_amd.core.near_field import device_near_candidates
+
+    grid = icosphere(3)
+    rwg = function_space(grid, "RWG", 0)
+    snc = function_space(grid, "SNC", 0)
+    test, trial = hb.device_space(snc), hb.device_space(rwg)
+    rule = hb.quad_rule(*triangle_rule(4))
+    shape = (snc.global_dof_count, rwg.global_dof_count)
+    outs = {}
+    for name in ("fp64", "untiled", "tiled"):
+        out = torch.zeros(shape, dtype=torch.complex128 if name == "fp64" else torch.complex64, device=hb.device())
+        if name == "tiled":
+            hb.regular_assemble_blocked("maxwell_magnetic_field", device_blocks(snc, "test_rwg_mixed"),
+                                        device_blocks(rwg, "trial_rwg_mixed"), True, rule, k, hb.PRECISION_MIXED, out)
+        else:
+            mode = hb.PRECISION_FP64 if name == "fp64" else hb.PRECISION_MIXED
+            hb.regular_assemble("maxwell_magnetic_field", test, trial, True, rule, k, mode, out, accumulate=1)
+        if name != "fp64":
+            outs[name + "_far"] = out.cpu().numpy().astype(np.complex128)
+            hb.near_assemble("maxwell_magnetic_field", test, trial, device_near_candidates(snc, rwg, near), True, rule,
+                             k, near, out)
+        outs[name] = out.cpu().numpy().astype(np.complex128)
+    ref = outs["fp64"]
+    # the far passes alone leave out a visible part (the near pairs)
+    assert rel_fro(outs["tiled_far"], ref) > 1e-3 and rel_fro(outs["untiled_far"], ref) > 1e-3
+    err_untiled, err_tiled = rel_fro(outs["untiled"], ref), rel_fro(outs["tiled"], ref)
+    print(f"k={k}: mixed vs FP64 untiled {err_untiled:.2e} tiled {err_tiled:.2e}", flush=True)
+    assert err_untiled < 1e-6 and err_tiled < 1e-6
+    # entrywise against FP64, relative to the largest entry
+    scale = np.abs(ref).max()
+    assert np.abs(outs["tiled"] - ref).max() < 1e-5 * scale
+
+
 def test_maxwell_vs_oracle(ico2, default_parameters, hip_available):
     from bempp_amd.api import function_space
     from bempp_amd.api.operators.boundary.maxwell import electric_field
