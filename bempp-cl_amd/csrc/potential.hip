@@ -12,7 +12,8 @@
 //   3. reduce: sums the chunk partials in a fixed order (deterministic result).
 // Far fields (numba_kernels.py:402-449, 2437-2570): the targets are unit directions xh and
 // the kernel is K = e^{-i Re(k) xh.y} / (4 pi) (the reference puts Re k in the exponent); they
-// are evaluated in FP64 in both precision modes (far_field_eval_fp64).
+// are evaluated in FP64 (far_field_eval_fp64) or, MIXED, with FP32 offsets within groups of
+// sources (far_field_eval_mixed).
 // MIXED: per (target, tile) the distance r0 = |x - c| to the tile centre c and e^{ik r0}
 // are formed in FP64; per source only r - r0 = u.(2v + u) / (r + r0), with v = x - c and
 // u = c - y ... small vectors, is needed, so the FP32 phase error stays ~k * eps * tile size
@@ -461,6 +462,227 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_fp64(const PotArgs
     }
 }
 
+// MIXED far fields and magnetic field potential. Sources are staged in FP32 as offsets u = y - c
+// from a reference source c (FP64) of their group of 32.
+// Far fields: with q_j the source's integrand weight (c, -i Re(k) (xh.n) c, ik t1 - xh t2 or
+// ik t1) and theta_j = -Re(k) xh.u_j,
+//   sum_j K_j q_j = K_c [ sum_j q_j + sum_j (e^{i theta_j} - 1) q_j ],  K_c = e^{-i Re(k) xh.c} / (4 pi).
+// K_c and the group sums of the weights (hence sum_j q_j) are FP64; only the second sum, small
+// and free of cancellation (e^{i theta} - 1 from half-angle sines), is FP32. The weights cancel
+// over the surface (RWG divergences sum to zero per basis function, sum n c dS over a closed
+// surface vanishes for constant c), so the leading term must not carry FP32 rounding.
+// Magnetic potential: the reference-distance split of potential_eval_mixed, r0 = |x - c| and
+// e^{ik r0} in FP64 per (target, group), r - r0 in FP32 from the small offsets.
+template <int FF>
+__global__ __launch_bounds__(kPotThreads) void far_field_eval_mixed(const PotArgs a) {
+    constexpr bool POT = FF == kMagneticPotential;
+    constexpr bool MAXWELL = FF == kFarE || FF == kFarH || POT;
+    constexpr int KD = MAXWELL ? 3 : 1;
+    constexpr int NF = MAXWELL ? kSrcFields : 8;
+    constexpr int NW = FF == kFarSL ? 2 : FF == kFarDL ? 6 : 8;  // FP64 group-summed weights
+    constexpr int GS = 32;                                          // sources per reference group
+    constexpr int NG = kPotTile / GS;
+    constexpr double kInv2PiD = 0.15915494309189533577;
+    __shared__ float sh[NF][kPotTile];
+    __shared__ double sc[NG][3];
+    __shared__ double wsum[NG][POT ? 1 : NW];
+    const int m = blockIdx.x * kPotThreads + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int s0 = chunk * a.chunk_len;
+    const int s1 = min(s0 + a.chunk_len, a.nsrc);
+    double x0 = 0, x1 = 0, x2 = 0;
+    if (m < a.npts) {
+        x0 = a.pts[3 * (int64_t)m];
+        x1 = a.pts[3 * (int64_t)m + 1];
+        x2 = a.pts[3 * (int64_t)m + 2];
+    }
+    const double xh[3] = {x0, x1, x2};
+    const float xf0 = (float)x0, xf1 = (float)x1, xf2 = (float)x2;
+    double accr[KD], acci[KD];
+    for (int d = 0; d < KD; ++d) accr[d] = acci[d] = 0.0;
+    const float kr = (float)a.kr, ki = (float)a.ki;
+    const float kr_rev = (float)(a.kr * kInv2PiD);
+    const float ki_log2e = (float)(a.ki * 1.4426950408889634);
+    for (int t0 = s0; t0 < s1; t0 += kPotTile) {
+        const int nt = min(kPotTile, s1 - t0);
+        __syncthreads();
+        {
+            const int g = threadIdx.x / GS;
+            const int cidx = t0 + min(g * GS + GS / 2, nt - 1);
+            double c[3];
+            for (int d = 0; d < 3; ++d) c[d] = a.src[(int64_t)d * a.nsrc_pad + cidx];
+            if (threadIdx.x % GS == 0)
+                for (int d = 0; d < 3; ++d) sc[g][d] = c[d];
+            const int s = t0 + threadIdx.x;
+            const bool ok = threadIdx.x < nt;
+            for (int d = 0; d < 3; ++d) sh[d][threadIdx.x] = ok ? (float)(a.src[(int64_t)d * a.nsrc_pad + s] - c[d]) : 0.f;
+            for (int k = 3; k < NF; ++k) sh[k][threadIdx.x] = ok ? (float)a.src[(int64_t)k * a.nsrc_pad + s] : 0.f;
+            if constexpr (!POT) {
+                double v[NW];
+                if constexpr (FF == kFarDL) {
+                    // n_j c_j: [d][re, im]
+                    const double cr = ok ? a.src[(int64_t)6 * a.nsrc_pad + s] : 0.0;
+                    const double ci = ok ? a.src[(int64_t)7 * a.nsrc_pad + s] : 0.0;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const double n = ok ? a.src[(int64_t)(3 + d) * a.nsrc_pad + s] : 0.0;
+                        v[2 * d] = n * cr;
+                        v[2 * d + 1] = n * ci;
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < NW; ++k) v[k] = ok ? a.src[(int64_t)(6 + k) * a.nsrc_pad + s] : 0.0;
+                }
+#pragma unroll
+                for (int k = 0; k < NW; ++k) {
+#pragma unroll
+                    for (int off = GS / 2; off >= 1; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+                }
+                if (threadIdx.x % GS == 0)
+#pragma unroll
+                    for (int k = 0; k < NW; ++k) wsum[g][k] = v[k];
+            }
+        }
+        __syncthreads();
+        for (int g = 0; g < NG && g * GS < nt; ++g) {
+            const int j0 = g * GS, j1 = min(j0 + GS, nt);
+            float Er[KD], Ei[KD];
+            for (int d = 0; d < KD; ++d) Er[d] = Ei[d] = 0.f;
+            double pr, pi;  // the group's FP64 factor
+            if constexpr (POT) {
+                const double v0d = x0 - sc[g][0], v1d = x1 - sc[g][1], v2d = x2 - sc[g][2];
+                const double r0sq_d = v0d * v0d + v1d * v1d + v2d * v2d;
+                const double r0d = sqrt(r0sq_d);
+                double sn, cs;
+                sincos(a.kr * r0d, &sn, &cs);
+                const double dmp = a.ki != 0.0 ? exp(-a.ki * r0d) : 1.0;
+                pr = cs * dmp;
+                pi = sn * dmp;
+                const float v0 = (float)v0d, v1 = (float)v1d, v2 = (float)v2d;
+                const float r0 = (float)r0d, r0sq = (float)r0sq_d;
+                for (int j = j0; j < j1; ++j) {
+                    const float u0 = sh[0][j], u1 = sh[1][j], u2 = sh[2][j];
+                    const float w0 = v0 - u0, w1 = v1 - u1, w2 = v2 - u2;  // x - y
+                    const float tt = u0 * (u0 - 2.f * v0) + u1 * (u1 - 2.f * v1) + u2 * (u2 - 2.f * v2);
+                    const float r2 = r0sq + tt;
+                    const float rinv = __builtin_amdgcn_rsqf(r2);
+                    const float r = r2 * rinv;
+                    const float delta = tt * __builtin_amdgcn_rcpf(r + r0);  // r - r0
+                    const float ph = kr_rev * delta;
+                    float c = __builtin_amdgcn_cosf(ph), sn1 = __builtin_amdgcn_sinf(ph);
+                    if (ki != 0.f) {
+                        const float dm = __builtin_amdgcn_exp2f(-ki_log2e * delta);
+                        c *= dm;
+                        sn1 *= dm;
+                    }
+                    // e^{ik delta} (ik r - 1) / (4 pi r^3)
+                    const float gg = float(kInv4Pi) * rinv * rinv * rinv;
+                    const float fr = -ki * r - 1.f, fi = kr * r;
+                    const float vr = gg * (c * fr - sn1 * fi), vi = gg * (c * fi + sn1 * fr);
+                    float wr[3], wi[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const float t1r = sh[8 + 2 * d][j], t1i = sh[9 + 2 * d][j];
+                        wr[d] = vr * t1r - vi * t1i;
+                        wi[d] = vr * t1i + vi * t1r;
+                    }
+                    Er[0] += w1 * wr[2] - w2 * wr[1];
+                    Ei[0] += w1 * wi[2] - w2 * wi[1];
+                    Er[1] += w2 * wr[0] - w0 * wr[2];
+                    Ei[1] += w2 * wi[0] - w0 * wi[2];
+                    Er[2] += w0 * wr[1] - w1 * wr[0];
+                    Ei[2] += w0 * wi[1] - w1 * wi[0];
+                }
+            } else {
+                double sn, cs;
+                sincos(-a.kr * (x0 * sc[g][0] + x1 * sc[g][1] + x2 * sc[g][2]), &sn, &cs);
+                pr = kInv4Pi * cs;
+                pi = kInv4Pi * sn;
+                for (int j = j0; j < j1; ++j) {
+                    // e^{i theta} - 1 = -2 sin^2(theta / 2) + 2 i sin(theta / 2) cos(theta / 2)
+                    const float half = -0.5f * kr_rev * (xf0 * sh[0][j] + xf1 * sh[1][j] + xf2 * sh[2][j]);
+                    const float hs = __builtin_amdgcn_sinf(half), hc = __builtin_amdgcn_cosf(half);
+                    const float em1r = -2.f * hs * hs, em1i = 2.f * hs * hc;
+                    if constexpr (FF == kFarSL || FF == kFarDL) {
+                        float qr = sh[6][j], qi = sh[7][j];
+                        if constexpr (FF == kFarDL) {
+                            // -i Re(k) (xh.n) c
+                            const float f = -kr * (xf0 * sh[3][j] + xf1 * sh[4][j] + xf2 * sh[5][j]);
+                            const float cr = qr;
+                            qr = -f * qi;
+                            qi = f * cr;
+                        }
+                        Er[0] += em1r * qr - em1i * qi;
+                        Ei[0] += em1r * qi + em1i * qr;
+                    } else {
+                        const float xfd[3] = {xf0, xf1, xf2};
+                        const float t2r = sh[6][j], t2i = sh[7][j];
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            const float t1r = sh[8 + 2 * d][j], t1i = sh[9 + 2 * d][j];
+                            float qr = -ki * t1r - kr * t1i, qi = -ki * t1i + kr * t1r;  // ik t1
+                            if constexpr (FF == kFarE) {
+                                qr -= xfd[d] * t2r;
+                                qi -= xfd[d] * t2i;
+                            }
+                            Er[d] += em1r * qr - em1i * qi;
+                            Ei[d] += em1r * qi + em1i * qr;
+                        }
+                    }
+                }
+                // the group sums: sum_j q_j in FP64, added to the FP32 remainder
+                double Qr[KD], Qi[KD];
+                if constexpr (FF == kFarSL) {
+                    Qr[0] = wsum[g][0];
+                    Qi[0] = wsum[g][1];
+                } else if constexpr (FF == kFarDL) {
+                    const double nr = x0 * wsum[g][0] + x1 * wsum[g][2] + x2 * wsum[g][4];
+                    const double ni = x0 * wsum[g][1] + x1 * wsum[g][3] + x2 * wsum[g][5];
+                    Qr[0] = a.kr * ni;  // -i Re(k) (nr + i ni)
+                    Qi[0] = -a.kr * nr;
+                } else {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const double t1r = wsum[g][2 + 2 * d], t1i = wsum[g][3 + 2 * d];
+                        Qr[d] = -a.ki * t1r - a.kr * t1i;
+                        Qi[d] = -a.ki * t1i + a.kr * t1r;
+                        if constexpr (FF == kFarE) {
+                            Qr[d] -= xh[d] * wsum[g][0];
+                            Qi[d] -= xh[d] * wsum[g][1];
+                        }
+                    }
+                }
+                for (int d = 0; d < KD; ++d) {
+                    const double er = Qr[d] + (double)Er[d], ei = Qi[d] + (double)Ei[d];
+                    accr[d] += pr * er - pi * ei;
+                    acci[d] += pr * ei + pi * er;
+                }
+                continue;
+            }
+            for (int d = 0; d < KD; ++d) {
+                accr[d] += pr * Er[d] - pi * Ei[d];
+                acci[d] += pr * Ei[d] + pi * Er[d];
+            }
+        }
+    }
+    if (m < a.npts) {
+        if constexpr (FF == kFarH) {
+            const double vr[3] = {accr[0], accr[1], accr[2]}, vi[3] = {acci[0], acci[1], acci[2]};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+                accr[d] = xh[d1] * vr[d2] - xh[d2] * vr[d1];
+                acci[d] = xh[d1] * vi[d2] - xh[d2] * vi[d1];
+            }
+        }
+        for (int d = 0; d < KD; ++d) {
+            double *p = a.partial + 2 * (((int64_t)chunk * KD + d) * a.npts + m);
+            p[0] = accr[d];
+            p[1] = acci[d];
+        }
+    }
+}
+
 template <typename OutT>
 __global__ void potential_reduce_kernel(const PotArgs a, int kd) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -514,8 +736,12 @@ static int run_potential(PotArgs &a, int precision, int out_dtype, hipStream_t s
     int rc = check_launch("potential_prep_kernel");
     if (rc) return rc;
     dim3 grid((a.npts + kPotThreads - 1) / kPotThreads, a.nchunk);
-    if constexpr (FF >= 0)
-        hipLaunchKernelGGL((far_field_eval_fp64<FF>), grid, dim3(kPotThreads), 0, s, a);
+    if constexpr (FF >= 0) {
+        if (precision == BEM_PRECISION_MIXED)
+            hipLaunchKernelGGL((far_field_eval_mixed<FF>), grid, dim3(kPotThreads), 0, s, a);
+        else
+            hipLaunchKernelGGL((far_field_eval_fp64<FF>), grid, dim3(kPotThreads), 0, s, a);
+    }
     else if (precision == BEM_PRECISION_MIXED)
         hipLaunchKernelGGL((potential_eval_mixed<OP>), grid, dim3(kPotThreads), 0, s, a);
     else

@@ -528,17 +528,27 @@ def maxwell_extras_bench(level, k, reps=3):
            "mfie_ms": 1e3 * t_mfie, "mfie_entries_per_s": n * n / t_mfie, "mfie_fp64_ms": 1e3 * t64,
            "mfie_speedup_vs_fp64": t64 / t_mfie, "mfie_rel_err_vs_fp64": rel_fro(out, ref)}
     del ref
-    for name, opr, pts in (("magnetic_potential", mpot.magnetic_field(rwg, 10.0 * dirs, k, precision="single"), 10.0 * dirs),
-                           ("electric_far_field", ff.electric_field(rwg, dirs, k, precision="single"), dirs),
-                           ("magnetic_far_field", ff.magnetic_field(rwg, dirs, k, precision="single"), dirs)):
-        opr.evaluate(fun)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for _ in range(reps):
+    import numpy as np
+
+    for name, factory, pts in (("magnetic_potential", mpot.magnetic_field, 10.0 * dirs),
+                               ("electric_far_field", ff.electric_field, dirs),
+                               ("magnetic_far_field", ff.magnetic_field, dirs)):
+        vals = {}
+        for precision in ("single", "double"):
+            opr = factory(rwg, pts, k, precision=precision)
             opr.evaluate(fun)
-        dt = (time.perf_counter() - t) / reps
-        res[f"{name}_ms"] = 1e3 * dt
-        res[f"{name}_evals_per_s"] = pts.shape[1] * grid.number_of_elements * 6 / dt
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(reps):
+                vals[precision] = opr.evaluate(fun)
+            dt = (time.perf_counter() - t) / reps
+            if precision == "single":
+                res[f"{name}_ms"] = 1e3 * dt
+                res[f"{name}_evals_per_s"] = pts.shape[1] * grid.number_of_elements * 6 / dt
+            else:
+                res[f"{name}_fp64_ms"] = 1e3 * dt
+        u32, u64 = vals["single"].astype(np.complex128), vals["double"]
+        res[f"{name}_rel_err_vs_fp64"] = float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64))
     return res
 
 

@@ -105,8 +105,8 @@ def test_far_fields_vs_oracle(case, ico2, default_parameters, hip_available):
 
 @pytest.mark.parametrize("k", [2.0, 1.5 + 0.4j])
 def test_magnetic_field_vs_oracle(k, ico2, default_parameters, hip_available):
-    """MFIE RWG x SNC and the magnetic field potential: FP64 vs the oracle; single (FP64 computed,
-    complex64 stored) within 1e-6."""
+    """MFIE RWG x SNC and the magnetic field potential: FP64 vs the oracle within 1e-12; single
+    precision (far pairs / sources in FP32) within 1e-6."""
     from bempp_amd.api import GridFunction, function_space
     from bempp_amd.api.operators.boundary.maxwell import magnetic_field
     from bempp_amd.api.operators.potential import maxwell as mpot
@@ -126,6 +126,39 @@ def test_magnetic_field_vs_oracle(k, ico2, default_parameters, hip_available):
     u = mpot.magnetic_field(rwg, pts, k, parameters=default_parameters).evaluate(GridFunction(rwg, coefficients=c))
     uref = O.potential("maxwell_magnetic_field", ico2.vertices, ico2.elements, rwg, c, pts, k)
     assert rel_fro(u, uref) < 1e-12
+    u32 = mpot.magnetic_field(rwg, pts, k, parameters=default_parameters, precision="single").evaluate(
+        GridFunction(rwg, coefficients=c))
+    assert u32.dtype == np.complex64 and rel_fro(u32.astype(np.complex128), uref) < 1e-6
+
+
+@pytest.mark.parametrize("case", ["helmholtz_dl_constant", "maxwell_e_random", "maxwell_h_random"])
+def test_far_field_low_frequency_mixed(case, default_parameters, hip_available):
+    """k = 0.05 on icosphere(3): the far-field weights nearly cancel over the surface (sum n c dS = 0
+    for a constant density; RWG divergences sum to zero per basis function), so the result is
+    O(k) smaller than its terms. The mixed kernels keep the group sums of the weights in FP64 and
+    only e^{i theta} - 1 times the weights in FP32; single must stay within 2e-6 of double."""
+    import bempp_amd.api.operators.far_field as FF
+    from bempp_amd.api import GridFunction, function_space
+    from bempp_amd.api.grid.shapes import icosphere
+
+    grid = icosphere(3)
+    rng = np.random.default_rng(4)
+    pts = rng.standard_normal((3, 300))
+    pts /= np.linalg.norm(pts, axis=0)
+    if case == "helmholtz_dl_constant":
+        space = function_space(grid, "P", 1)
+        c = np.ones(space.global_dof_count, dtype=np.complex128)
+        factory = FF.helmholtz.double_layer
+    else:
+        space = function_space(grid, "RWG", 0)
+        c = rng.standard_normal(space.global_dof_count) + 1j * rng.standard_normal(space.global_dof_count)
+        factory = FF.maxwell.electric_field if case == "maxwell_e_random" else FF.maxwell.magnetic_field
+    fun = GridFunction(space, coefficients=c)
+    u64 = factory(space, pts, 0.05, parameters=default_parameters, precision="double").evaluate(fun)
+    u32 = factory(space, pts, 0.05, parameters=default_parameters, precision="single").evaluate(fun)
+    err = rel_fro(u32.astype(np.complex128), u64)
+    print(f"{case}: |u| {np.linalg.norm(u64):.3e}, single vs double {err:.2e}", flush=True)
+    assert err < 2e-6
 
 
 @pytest.mark.parametrize("k", [2.0, 1.5 + 0.4j])
