@@ -14,8 +14,8 @@
 //    the far pairs go through the tiled regular kernel (regular_mfie.hip) instead;
 //  * near pairs of the mixed mode (pair_is_near), FP64: one lane per candidate pair
 //    (bem_near_assemble);
-//  * singular, FP64: one 64-lane wavefront per Sauter-Schwab pair, its Duffy points strided over
-//    the lanes, the 3 x 3 block reduced across the wave.
+//  * singular, FP64: eight lanes per Sauter-Schwab pair, its Duffy points strided over the
+//    group, the 3 x 3 block reduced across the group.
 #include "bem_internal.h"
 #include "far_split.h"
 #include "mfie_block.h"
@@ -113,19 +113,6 @@ __device__ __forceinline__ void scatter_block(const Args &a, const int *dt, cons
         }
 }
 
-template <typename OutT>
-__device__ __forceinline__ void scatter(const Args &a, int e, int f, const double *gt, const double *gf,
-                                        const double (&ar)[3][3], const double (&ai)[3][3], int i, int j) {
-    double lt[3], lf[3];
-    edge_lengths(gt, lt);
-    edge_lengths(gf, lf);
-    const double m = a.test.mult[3 * (int64_t)e + i] * a.trial.mult[3 * (int64_t)f + j] * lt[i] * lf[j];
-    if (m == 0.0) return;
-    const int64_t row = a.test.l2g[3 * (int64_t)e + i], col = a.trial.l2g[3 * (int64_t)f + j];
-    OutT *o = reinterpret_cast<OutT *>(a.out) + 2 * (row * a.ld + col);
-    atomic_add(o, (OutT)(ar[i][j] * m));
-    atomic_add(o + 1, (OutT)(ai[i][j] * m));
-}
 
 // The 3 x 3 block of a regular pair in FP64 (moments form of mfie_block.h): ap(p) the offsets of
 // the test points from the test centroid, e1 / e2 the test edge vectors, b the trial offsets, kap
@@ -398,18 +385,23 @@ __global__ __launch_bounds__(kThreads) void mfie_regular_mixed_kernel(const Args
     }
 }
 
+constexpr int kSingGroup = 8;  // lanes per singular pair
+
 template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
+__device__ __forceinline__ T group_sum(T v) {
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    for (int off = kSingGroup / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kSingGroup);
     return v;
 }
 
+// kSingGroup lanes per Sauter-Schwab pair (eight pairs per wavefront): lane g of the group takes
+// the Duffy points g, g + 8, ...; the 3 x 3 block is summed over the group with xor shuffles and
+// its entries are scattered by the group's lanes (lane g entry g, lane 0 also entry 8).
 template <typename OutT>
 __global__ __launch_bounds__(kThreads) void mfie_singular_kernel(const Args a) {
-    const int lane = threadIdx.x & 63;
-    const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (pair >= a.n_pairs) return;  // uniform per wavefront
+    const int g = threadIdx.x & (kSingGroup - 1);
+    const int pair = (blockIdx.x * kThreads + threadIdx.x) / kSingGroup;
+    if (pair >= a.n_pairs) return;  // uniform per group
     const int32_t *pd = a.pairs + 6 * (int64_t)pair;
     const int e = pd[0], f = pd[1], toff = pd[2], soff = pd[3], woff = pd[4], np = pd[5];
     const double *gt = a.test.geom + (int64_t)e * BEM_GEOM_STRIDE;
@@ -419,7 +411,7 @@ __global__ __launch_bounds__(kThreads) void mfie_singular_kernel(const Args a) {
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) ar[i][j] = ai[i][j] = 0.0;
-    for (int k = lane; k < np; k += 64) {
+    for (int k = g; k < np; k += kSingGroup) {
         const double xi0 = a.tpts[2 * (int64_t)(toff + k)], xi1 = a.tpts[2 * (int64_t)(toff + k) + 1];
         const double et0 = a.spts[2 * (int64_t)(soff + k)], et1 = a.spts[2 * (int64_t)(soff + k) + 1];
         double A[3][3], B[3][3];
@@ -435,15 +427,26 @@ __global__ __launch_bounds__(kThreads) void mfie_singular_kernel(const Args a) {
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            ar[i][j] = wave_sum(ar[i][j]);
-            ai[i][j] = wave_sum(ai[i][j]);
+            ar[i][j] = group_sum(ar[i][j]);
+            ai[i][j] = group_sum(ai[i][j]);
         }
-    // lanes 0..8 add one entry each (compile-time indices into the register blocks)
+    int tdof[3], fdof[3];
+    double tml[3], fml[3];
+    dof_factors(a.test, e, tdof, tml);
+    dof_factors(a.trial, f, fdof, fml);
+    // compile-time indices into the register blocks
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-            if (lane == 3 * i + j) scatter<OutT>(a, e, f, gt, gf, ar, ai, i, j);
+        for (int j = 0; j < 3; ++j) {
+            const int entry = 3 * i + j;
+            if (g != entry % kSingGroup) continue;
+            const double m = tml[i] * fml[j];
+            if (m == 0.0) continue;
+            OutT *o = reinterpret_cast<OutT *>(a.out) + 2 * ((int64_t)tdof[i] * a.ld + fdof[j]);
+            atomic_add(o, (OutT)(ar[i][j] * m));
+            atomic_add(o + 1, (OutT)(ai[i][j] * m));
+        }
 }
 
 }  // namespace mfie
@@ -537,7 +540,7 @@ int mfie_singular_launch(const SpaceArgs &test, const SpaceArgs &trial, int n_pa
     a.tpts = tpts;
     a.spts = spts;
     a.wts = wts;
-    const dim3 grid((n_pairs + 3) / 4);
+    const dim3 grid((unsigned)(((int64_t)n_pairs * mfie::kSingGroup + mfie::kThreads - 1) / mfie::kThreads));
     if (out_dtype == BEM_C128)
         hipLaunchKernelGGL(mfie::mfie_singular_kernel<double>, grid, dim3(mfie::kThreads), 0, s, a);
     else if (out_dtype == BEM_C64)
