@@ -7,6 +7,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include "bem_device.h"
+#include "bem_internal.h"
+
 namespace bem {
 namespace mfie {
 
@@ -106,6 +109,56 @@ struct PairMoments {
             }
     }
 };
+
+// edge lengths |v1 - v0|, |v2 - v0|, |v1 - v2| of a geometry row
+__device__ __forceinline__ void edge_lengths(const double *g, double (&l)[3]) {
+    l[0] = sqrt(g[3] * g[3] + g[4] * g[4] + g[5] * g[5]);
+    l[1] = sqrt(g[6] * g[6] + g[7] * g[7] + g[8] * g[8]);
+    const double d0 = g[3] - g[6], d1 = g[4] - g[7], d2 = g[5] - g[8];
+    l[2] = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+}
+
+// The 3 x 3 block V_ij / (l_i l_j) of a regular pair in FP64 with the plain rule weights (the
+// Jacobians cancel the Piola maps): ap(p) the offsets of the test points from the test centroid,
+// e1 / e2 the test edge vectors, b the trial offsets, kap c_f - Q_j of the trial element,
+// fd = c_f - c_e.
+template <int Q, typename AP>
+__device__ __forceinline__ void pair_block_fp64(AP ap_of, const double *e1, const double *e2, const double (&b)[Q][3],
+                                                const double (&kap)[3][3], const double (&fd)[3], const RuleArgs &rule,
+                                                double kr, double ki, double (&ar)[3][3], double (&ai)[3][3]) {
+    PairMoments<double> M;
+    M.clear();
+#pragma unroll 1
+    for (int p = 0; p < Q; ++p) {
+        double ap[3];
+        ap_of(p, ap);
+        const double d0 = ap[0] - fd[0], d1 = ap[1] - fd[1], d2 = ap[2] - fd[2];
+        double Rp[2] = {0.0, 0.0}, Yp[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const double u0 = d0 - b[q][0], u1 = d1 - b[q][1], u2 = d2 - b[q][2];  // x_p - y_q
+            const double r2 = u0 * u0 + u1 * u1 + u2 * u2;
+            const double r = sqrt(r2);
+            double sn, cs;
+            sincos(kr * r, &sn, &cs);
+            const double damp = ki != 0.0 ? exp(-ki * r) : 1.0;
+            const double g = rule.w[p] * rule.w[q] * kInv4Pi * damp / (r * r2);
+            const double fr = -ki * r - 1.0, fi = kr * r;
+            const double vr = g * (cs * fr - sn * fi), vi = g * (cs * fi + sn * fr);
+            Rp[0] += vr;
+            Rp[1] += vi;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                Yp[0][d] += vr * b[q][d];
+                Yp[1][d] += vi * b[q][d];
+            }
+        }
+        M.add_point(Rp, Yp, ap, fd);
+    }
+    double pi[3][3];
+    rwg_offsets(e1, e2, pi);
+    M.block(pi, kap, fd, ar, ai);
+}
 
 }  // namespace mfie
 }  // namespace bem
