@@ -8,11 +8,12 @@ algorithms (scipy/sparse/linalg/_isolve/iterative.py, gmres and cg, scipy 1.12+)
 GMRES with Givens rotations, the same inner/outer tolerance logic (rtol relative to ||b||,
 atol = 0) and the same callback values, so that the iteration counts, residual histories and
 iterates agree with the reference to rounding. The Arnoldi step orthogonalises with two
-classical Gram-Schmidt passes (two device products each) where scipy uses modified
-Gram-Schmidt: the Krylov basis is the same to rounding, and the device does 4 products per
-step instead of 2 (restart) small vector operations.
+classical Gram-Schmidt passes where scipy uses modified Gram-Schmidt: the Krylov basis is the
+same to rounding. On the GPU the step is one library call (bem_gmres_cgs2: three passes over
+the basis, one host copy of the coefficients) instead of 2 (restart) small vector operations.
 """
 
+import math
 import time as _time
 
 import numpy as _np
@@ -53,15 +54,17 @@ class IterationCounter(object):
 
 
 def _lartg(f, g):
-    """Givens rotation (c real, s) with [c, s; -conj(s), c] [f; g] = [r; 0] (LAPACK zlartg)."""
+    """Givens rotation (c real, s) with [c, s; -conj(s), c] [f; g] = [r; 0] (LAPACK zlartg), on
+    Python complex numbers (the per-iteration host work of GMRES)."""
+    f, g = complex(f), complex(g)
     af, ag = abs(f), abs(g)
     if ag == 0.0:
-        return 1.0, 0.0 * f, f
+        return 1.0, 0j, f
     if af == 0.0:
-        return 0.0, _np.conj(g) / ag, ag
-    d = _np.hypot(af, ag)
+        return 0.0, g.conjugate() / ag, complex(ag)
+    d = math.hypot(af, ag)
     ph = f / af
-    return af / d, ph * _np.conj(g) / d, ph * d
+    return af / d, ph * g.conjugate() / d, ph * d
 
 
 def device_gmres(tensor, b, rtol=1e-5, restart=None, maxiter=None, callback=None):
@@ -93,8 +96,15 @@ def krylov_gmres(matvec, b, dev, rtol=1e-5, restart=None, maxiter=None, callback
     ptol = bnrm2 * min(ptol_max_factor, atol / bnrm2)
     presid = 0.0
     V = torch.empty((restart + 1, n), dtype=torch.complex128, device=dev)
+    # on the GPU the CGS2 step is one library call (three passes over V, one host copy of the
+    # coefficients); on the CPU it is the same arithmetic in torch
+    hip = dev.type == "cuda"
+    if hip:
+        from bempp_amd.core import hip_backend as hb
+
+        gs_out = torch.empty(2 * restart + 4, dtype=torch.complex128, device=dev)
     h = _np.zeros((restart, restart + 1), dtype=_np.complex128)
-    givens = _np.zeros((restart, 2), dtype=_np.complex128)
+    gc, gs = [0.0] * restart, [0j] * restart  # Givens rotations (c real, s complex)
     inner_iter = 0
     rnorm = bnrm2
     r = bt.clone()
@@ -103,22 +113,30 @@ def krylov_gmres(matvec, b, dev, rtol=1e-5, restart=None, maxiter=None, callback
             return x, 0
         tmp = float(torch.linalg.vector_norm(r))
         V[0] = r / tmp
-        S = _np.zeros(restart + 1, dtype=_np.complex128)
-        S[0] = tmp
+        S = [0j] * (restart + 1)
+        S[0] = complex(tmp)
         breakdown = False
         col = 0
         for col in range(restart):
             w = matvec(V[col])
-            h0 = torch.linalg.vector_norm(w)
-            Vk = V[: col + 1]
-            c1 = torch.mv(Vk.conj(), w)
-            w = w - torch.mv(Vk.T, c1)
-            c2 = torch.mv(Vk.conj(), w)
-            w = w - torch.mv(Vk.T, c2)
-            h1 = torch.linalg.vector_norm(w)
-            hv = torch.cat([c1 + c2, h1.to(torch.complex128)[None], h0.to(torch.complex128)[None]]).cpu().numpy()
-            h[col, : col + 1] = hv[: col + 1]
-            h1v, h0v = hv[col + 1].real, hv[col + 2].real
+            nv = col + 1
+            if hip:
+                w = w.to(torch.complex128).contiguous()
+                hb.gmres_cgs2(V, nv, w, gs_out)
+                g = gs_out[: 2 * nv + 2].cpu().numpy()
+                h[col, :nv] = g[:nv] + g[nv + 1 : 2 * nv + 1]
+                h0v, h1v = math.sqrt(g[nv].real), math.sqrt(g[2 * nv + 1].real)
+            else:
+                h0 = torch.linalg.vector_norm(w)
+                Vk = V[:nv]
+                c1 = torch.mv(Vk.conj(), w)
+                w = w - torch.mv(Vk.T, c1)
+                c2 = torch.mv(Vk.conj(), w)
+                w = w - torch.mv(Vk.T, c2)
+                h1 = torch.linalg.vector_norm(w)
+                hv = torch.cat([c1 + c2, h1.to(torch.complex128)[None], h0.to(torch.complex128)[None]]).cpu().numpy()
+                h[col, :nv] = hv[:nv]
+                h1v, h0v = float(hv[nv].real), float(hv[nv + 1].real)
             h[col, col + 1] = h1v
             if h1v <= eps * h0v:
                 h[col, col + 1] = 0.0
@@ -126,14 +144,16 @@ def krylov_gmres(matvec, b, dev, rtol=1e-5, restart=None, maxiter=None, callback
                 breakdown = True
             else:
                 V[col + 1] = w / h1v
-            for k in range(col):
-                c, s = givens[k, 0].real, givens[k, 1]
-                n0, n1 = h[col, k], h[col, k + 1]
-                h[col, k], h[col, k + 1] = c * n0 + s * n1, -_np.conj(s) * n0 + c * n1
-            c, s, mag = _lartg(h[col, col], h[col, col + 1])
-            givens[col] = [c, s]
-            h[col, col], h[col, col + 1] = mag, 0.0
-            tmp = -_np.conj(s) * S[col]
+            row = h[col, : col + 2].tolist()
+            for i in range(col):
+                c, s = gc[i], gs[i]
+                n0, n1 = row[i], row[i + 1]
+                row[i], row[i + 1] = c * n0 + s * n1, -s.conjugate() * n0 + c * n1
+            c, s, mag = _lartg(row[col], row[col + 1])
+            gc[col], gs[col] = c, s
+            row[col], row[col + 1] = mag, 0.0
+            h[col, : col + 2] = row
+            tmp = -s.conjugate() * S[col]
             S[col], S[col + 1] = c * S[col], tmp
             presid = abs(tmp)
             inner_iter += 1

@@ -135,6 +135,8 @@ def lib():
         L.bem_adjoint_potential_single_layer.restype = i
         L.bem_dense_matvec.argtypes = [i, P, i, i64, i64, i64, P, P, P]
         L.bem_dense_matvec.restype = i
+        L.bem_gmres_cgs2.argtypes = [P, i64, i, i64, P, P, P]
+        L.bem_gmres_cgs2.restype = i
         L.bem_regular_assemble_sl_dl.argtypes = [i, SP, SP, BP, BP, i, QP, d, d, i, d, i, P, P, i64, i, i, P]
         L.bem_regular_assemble_sl_dl.restype = i
         L.bem_regular_assemble_sl_dl_symmetric.argtypes = [i, SP, BP, BP, i, P, i64, QP, d, d, i, d, i, P, P, i64, i, P]
@@ -421,6 +423,18 @@ def dense_matvec(A, x, trans=0, out=None):
     check(lib().bem_dense_matvec(int(trans), ptr(A), _dtype_code(A), rows, cols, A.stride(0), ptr(x), ptr(out),
                                  stream_handle()))
     return out
+
+
+def gmres_cgs2(V, k, w, out):
+    """CGS2 of w (complex128, length n, updated in place) against rows [0, k) of V ((m, n) complex128,
+    row-major); out: complex128 device buffer of >= 2k + 2 values receiving c1, |w|^2, c2, |w_out|^2."""
+    torch = _torch()
+    n = w.shape[0]
+    if V.dtype != torch.complex128 or V.dim() != 2 or V.stride(1) != 1 or V.shape[1] != n or not 1 <= k <= V.shape[0]:
+        raise ValueError("V must be a row-major (m, n) complex128 tensor with k <= m rows used")
+    if w.dtype != torch.complex128 or not w.is_contiguous() or out.dtype != torch.complex128 or out.numel() < 2 * k + 2:
+        raise ValueError("w must be a contiguous complex128 vector and out hold 2k + 2 complex128 values")
+    check(lib().bem_gmres_cgs2(ptr(V), V.stride(0), int(k), int(n), ptr(w), ptr(out), stream_handle()))
 
 
 def regular_assemble_sl_dl(op, test, trial, test_blocks, trial_blocks, grids_identical, rule, wavenumber, precision,

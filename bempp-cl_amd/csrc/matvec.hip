@@ -352,9 +352,116 @@ int square_matrix_op(int what, void *A, int dtype, int64_t n, int64_t ld, hipStr
     }
 }
 
+// ------------------------------------------------------------------ GMRES orthogonalisation
+// One classical Gram-Schmidt step with reorthogonalisation (CGS2) of w against the k rows of V
+// (complex128, row stride ldv), three passes that each read V once:
+//   1: c1 = conj(V) w and |w|^2;
+//   2: w <- w - V^T c1 and, from the updated elements (the update is elementwise), c2 = conj(V) w;
+//   3: w <- w - V^T c2 and |w|^2.
+// A thread owns elements j of w (grid-stride); its contributions to the k dot products are summed
+// over the block (wave shuffles, then LDS) and added to out with FP64 atomics.
+// out (complex128): c1[0, k), |w_in|^2 at k, c2[k + 1, 2k + 1), |w_out|^2 at 2k + 1.
+namespace gs {
+
+constexpr int kThreads = 256;
+
+// (re, im) summed over the block; every thread gets the total
+__device__ __forceinline__ void block_sum2(double &re, double &im, double (*red)[2]) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        re += __shfl_xor(re, off, 64);
+        im += __shfl_xor(im, off, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wave][0] = re;
+        red[wave][1] = im;
+    }
+    __syncthreads();
+    re = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+    im = red[0][1] + red[1][1] + red[2][1] + red[3][1];
+    __syncthreads();  // red is reused by the next sum
+}
+
+// PASS 1: dots of w; PASS 2: update by c1 (out[0, k)), then dots; PASS 3: update by c2, norm
+template <int PASS>
+__global__ __launch_bounds__(kThreads) void cgs2_kernel(const double2 *V, int64_t ldv, int k, int64_t n, double2 *w,
+                                                        double *out) {
+    __shared__ double red[kThreads / 64][2];
+    const int64_t stride = (int64_t)gridDim.x * kThreads;
+    const int64_t j0 = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const double2 *cin = reinterpret_cast<const double2 *>(out) + (PASS == 2 ? 0 : k + 1);
+    // update: w_j <- w_j - sum_i V_ij c_i (the thread's elements, kept in place)
+    if (PASS >= 2) {
+        for (int64_t j = j0; j < n; j += stride) {
+            double2 v = w[j];
+            for (int i = 0; i < k; ++i) {
+                const double2 a = V[(int64_t)i * ldv + j], c = cin[i];
+                v.x -= a.x * c.x - a.y * c.y;
+                v.y -= a.x * c.y + a.y * c.x;
+            }
+            w[j] = v;
+        }
+    }
+    if (PASS <= 2) {
+        // dots conj(V_i) . w into out[0, k) (pass 1) or out[k + 1, 2k + 1) (pass 2)
+        double *dst = out + 2 * (PASS == 1 ? 0 : k + 1);
+        for (int i = 0; i < k; ++i) {
+            double re = 0.0, im = 0.0;
+            for (int64_t j = j0; j < n; j += stride) {
+                const double2 a = V[(int64_t)i * ldv + j], v = w[j];
+                re += a.x * v.x + a.y * v.y;
+                im += a.x * v.y - a.y * v.x;
+            }
+            block_sum2(re, im, red);
+            if (threadIdx.x == 0) {
+                atomic_add(dst + 2 * i, re);
+                atomic_add(dst + 2 * i + 1, im);
+            }
+        }
+    }
+    if (PASS != 2) {
+        // |w|^2: the input (pass 1, at k) or the result (pass 3, at 2k + 1)
+        double re = 0.0, im = 0.0;
+        for (int64_t j = j0; j < n; j += stride) {
+            const double2 v = w[j];
+            re += v.x * v.x + v.y * v.y;
+        }
+        block_sum2(re, im, red);
+        if (threadIdx.x == 0) atomic_add(out + 2 * (PASS == 1 ? k : 2 * k + 1), re);
+    }
+}
+
+}  // namespace gs
+
 }  // namespace bem
 
 using namespace bem;
+
+extern "C" int bem_gmres_cgs2(const void *V, int64_t ldv, int k, int64_t n, void *w, void *out, void *stream) {
+    clear_error();
+    if (!V || !w || !out || k < 1 || n < 1 || ldv < n) {
+        set_error("invalid argument");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    if (reinterpret_cast<uintptr_t>(V) % 16 || reinterpret_cast<uintptr_t>(w) % 16 || reinterpret_cast<uintptr_t>(out) % 8) {
+        set_error("V and w must be 16-byte aligned complex128 arrays");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t nout = 2 * (2 * (int64_t)k + 2);
+    hipLaunchKernelGGL(mv::zero_kernel, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, s, (double *)out, nout);
+    int64_t blocks = (n + gs::kThreads - 1) / gs::kThreads;
+    if (blocks > 512) blocks = 512;
+    const dim3 grid((unsigned)blocks);
+    const double2 *Vd = (const double2 *)V;
+    double2 *wd = (double2 *)w;
+    double *od = (double *)out;
+    hipLaunchKernelGGL(gs::cgs2_kernel<1>, grid, dim3(gs::kThreads), 0, s, Vd, ldv, k, n, wd, od);
+    hipLaunchKernelGGL(gs::cgs2_kernel<2>, grid, dim3(gs::kThreads), 0, s, Vd, ldv, k, n, wd, od);
+    hipLaunchKernelGGL(gs::cgs2_kernel<3>, grid, dim3(gs::kThreads), 0, s, Vd, ldv, k, n, wd, od);
+    return check_launch("cgs2_kernel");
+}
 
 extern "C" int bem_dense_add_transpose(void *A, int dtype, int64_t n, int64_t ld, void *stream) {
     clear_error();

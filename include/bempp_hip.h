@@ -11,7 +11,7 @@
  *   bem_potential_evaluate <- potential_assembler(...) bempp_cl/core/numba_assemblers.py:115-164,
  *                                                      bempp_cl/core/opencl_assemblers.py:287-465
  * Extra entry points with no reference counterpart: bem_kbatch_assemble (many Helmholtz
- * wavenumbers in one persistent launch), bem_adjoint_* (gradients), bem_dense_matvec,
+ * wavenumbers in one persistent launch), bem_adjoint_* (gradients), bem_dense_matvec, bem_gmres_cgs2,
  * bem_fill_zero, bem_potential_workspace_size, bem_last_error, bem_version.
  *
  * Conventions
@@ -334,6 +334,17 @@ int bem_adjoint_potential_single_layer(const bem_space_t *space, const bem_quad_
                                        const void *coefficients, int n_points, const double *points,
                                        const void *cotangent, void *grad_coefficients, double *grad_vertices,
                                        void *stream);
+
+/*
+ * One classical Gram-Schmidt step with reorthogonalisation (CGS2) for GMRES on device vectors
+ * (no reference counterpart: the reference runs scipy.sparse.linalg.gmres on the host,
+ * bempp_cl/api/linalg/iterative_solvers.py:137-140). V: k >= 1 complex128 rows of length n with
+ * row stride ldv; w: complex128 vector of length n, overwritten with
+ *   w2 = w1 - V^T c2,  w1 = w - V^T c1,  c1 = conj(V) w,  c2 = conj(V) w1.
+ * out: device buffer of 2k + 2 complex128 values, overwritten with c1 (k values), |w|^2 (real
+ * part), c2 (k values), |w2|^2. Sums are FP64 with atomics (summation order not fixed).
+ */
+int bem_gmres_cgs2(const void *V, int64_t ldv, int k, int64_t n, void *w, void *out, void *stream);
 
 /*
  * y = op(A) x for a dense row-major matrix in HBM (the product of the reference's

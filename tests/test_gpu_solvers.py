@@ -197,3 +197,37 @@ def test_discrete_operator_products(dtype, default_parameters, hip_available):
     S = (0.5 * sparse.identity(p1, p1, p1) + op).weak_form()
     M = sparse.identity(p1, p1, p1).weak_form().to_dense()
     np.testing.assert_allclose(S.to_dense(), (0.5 * M + A).astype(S.dtype), rtol=1e-6, atol=1e-12)
+
+
+@pytest.mark.parametrize("k,n", [(1, 37), (5, 1000), (21, 10242), (40, 200000)])
+def test_gmres_cgs2_entry_point(k, n, hip_available):
+    """bem_gmres_cgs2 against the same CGS2 arithmetic in numpy (FP64): the coefficients, both
+    norms and the updated vector; n past one block, past the 512-block grid cap (grid-stride) and
+    k past the restart length used elsewhere."""
+    import torch
+
+    from bempp_amd.core import hip_backend as hb
+
+    rng = np.random.default_rng(k + n)
+    V = rng.standard_normal((k + 2, n)) + 1j * rng.standard_normal((k + 2, n))
+    V[:k] = np.linalg.qr(V[:k].T)[0].T  # orthonormal rows, as in GMRES
+    w = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    c1 = V[:k].conj() @ w
+    w1 = w - V[:k].T @ c1
+    c2 = V[:k].conj() @ w1
+    w2 = w1 - V[:k].T @ c2
+    dev = hb.device()
+    Vt = torch.as_tensor(V, device=dev)
+    wt = torch.as_tensor(w, device=dev)
+    out = torch.full((2 * k + 2,), 7.0 + 7.0j, dtype=torch.complex128, device=dev)  # overwritten
+    hb.gmres_cgs2(Vt, k, wt, out)
+    g = out.cpu().numpy()
+    scale = np.linalg.norm(w)
+    np.testing.assert_allclose(g[:k], c1, rtol=0, atol=1e-12 * scale)
+    np.testing.assert_allclose(g[k + 1 : 2 * k + 1], c2, rtol=0, atol=1e-12 * scale)
+    assert abs(g[k].real - np.vdot(w, w).real) <= 1e-12 * scale**2 and g[k].imag == 0.0
+    assert abs(g[2 * k + 1].real - np.vdot(w2, w2).real) <= 1e-12 * scale**2
+    np.testing.assert_allclose(wt.cpu().numpy(), w2, rtol=0, atol=1e-12 * scale)
+    # rows past k are untouched by the products
+    np.testing.assert_array_equal(Vt.cpu().numpy(), V)
+
