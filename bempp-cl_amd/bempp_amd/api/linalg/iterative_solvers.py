@@ -10,7 +10,9 @@ atol = 0) and the same callback values, so that the iteration counts, residual h
 iterates agree with the reference to rounding. The Arnoldi step orthogonalises with two
 classical Gram-Schmidt passes where scipy uses modified Gram-Schmidt: the Krylov basis is the
 same to rounding. On the GPU the step is one library call (bem_gmres_cgs2: three passes over
-the basis, one host copy of the coefficients) instead of 2 (restart) small vector operations.
+the basis and the next basis vector, one host copy of the coefficients) instead of 2 (restart)
+small vector operations, and the next step is issued before the host reads this one's
+coefficients.
 """
 
 import math
@@ -96,13 +98,27 @@ def krylov_gmres(matvec, b, dev, rtol=1e-5, restart=None, maxiter=None, callback
     ptol = bnrm2 * min(ptol_max_factor, atol / bnrm2)
     presid = 0.0
     V = torch.empty((restart + 1, n), dtype=torch.complex128, device=dev)
-    # on the GPU the CGS2 step is one library call (three passes over V, one host copy of the
-    # coefficients); on the CPU it is the same arithmetic in torch
+    # On the GPU the CGS2 step is one library call that also writes the next basis vector, and
+    # step col + 1 is issued before the host reads the coefficients of step col (alternating
+    # device / pinned host buffers), so the host's rotations overlap the next product. A step
+    # issued past the end of the inner loop is discarded. On the CPU the same arithmetic runs in
+    # torch, one step at a time.
     hip = dev.type == "cuda"
     if hip:
         from bempp_amd.core import hip_backend as hb
 
-        gs_out = torch.empty(2 * restart + 4, dtype=torch.complex128, device=dev)
+        nbuf = 2 * restart + 4
+        gs_out = [torch.empty(nbuf, dtype=torch.complex128, device=dev) for _ in range(2)]
+        gs_host = [torch.empty(nbuf, dtype=torch.complex128, pin_memory=True) for _ in range(2)]
+        gs_event = [torch.cuda.Event() for _ in range(2)]
+
+        def issue(c):
+            wv = matvec(V[c]).to(torch.complex128).contiguous()
+            b = c % 2
+            hb.gmres_cgs2(V, c + 1, wv, gs_out[b], V[c + 1], eps)
+            gs_host[b][: 2 * c + 4].copy_(gs_out[b][: 2 * c + 4], non_blocking=True)
+            gs_event[b].record()
+
     h = _np.zeros((restart, restart + 1), dtype=_np.complex128)
     gc, gs = [0.0] * restart, [0j] * restart  # Givens rotations (c real, s complex)
     inner_iter = 0
@@ -117,16 +133,19 @@ def krylov_gmres(matvec, b, dev, rtol=1e-5, restart=None, maxiter=None, callback
         S[0] = complex(tmp)
         breakdown = False
         col = 0
+        if hip:
+            issue(0)
         for col in range(restart):
-            w = matvec(V[col])
             nv = col + 1
             if hip:
-                w = w.to(torch.complex128).contiguous()
-                hb.gmres_cgs2(V, nv, w, gs_out)
-                g = gs_out[: 2 * nv + 2].cpu().numpy()
+                if col + 1 < restart:
+                    issue(col + 1)
+                gs_event[col % 2].synchronize()
+                g = gs_host[col % 2][: 2 * nv + 2].numpy().copy()
                 h[col, :nv] = g[:nv] + g[nv + 1 : 2 * nv + 1]
                 h0v, h1v = math.sqrt(g[nv].real), math.sqrt(g[2 * nv + 1].real)
             else:
+                w = matvec(V[col])
                 h0 = torch.linalg.vector_norm(w)
                 Vk = V[:nv]
                 c1 = torch.mv(Vk.conj(), w)
@@ -138,12 +157,11 @@ def krylov_gmres(matvec, b, dev, rtol=1e-5, restart=None, maxiter=None, callback
                 h[col, :nv] = hv[:nv]
                 h1v, h0v = float(hv[nv].real), float(hv[nv + 1].real)
             h[col, col + 1] = h1v
-            if h1v <= eps * h0v:
+            breakdown = h1v <= eps * h0v
+            if breakdown:
                 h[col, col + 1] = 0.0
-                V[col + 1] = w
-                breakdown = True
-            else:
-                V[col + 1] = w / h1v
+            if not hip:  # the device step wrote V[col + 1] itself, by the same test
+                V[col + 1] = w if breakdown else w / h1v
             row = h[col, : col + 2].tolist()
             for i in range(col):
                 c, s = gc[i], gs[i]

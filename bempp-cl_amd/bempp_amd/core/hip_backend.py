@@ -135,7 +135,7 @@ def lib():
         L.bem_adjoint_potential_single_layer.restype = i
         L.bem_dense_matvec.argtypes = [i, P, i, i64, i64, i64, P, P, P]
         L.bem_dense_matvec.restype = i
-        L.bem_gmres_cgs2.argtypes = [P, i64, i, i64, P, P, P]
+        L.bem_gmres_cgs2.argtypes = [P, i64, i, i64, P, P, P, ctypes.c_double, P]
         L.bem_gmres_cgs2.restype = i
         L.bem_regular_assemble_sl_dl.argtypes = [i, SP, SP, BP, BP, i, QP, d, d, i, d, i, P, P, i64, i, i, P]
         L.bem_regular_assemble_sl_dl.restype = i
@@ -425,16 +425,21 @@ def dense_matvec(A, x, trans=0, out=None):
     return out
 
 
-def gmres_cgs2(V, k, w, out):
+def gmres_cgs2(V, k, w, out, v_next=None, eps=0.0):
     """CGS2 of w (complex128, length n, updated in place) against rows [0, k) of V ((m, n) complex128,
-    row-major); out: complex128 device buffer of >= 2k + 2 values receiving c1, |w|^2, c2, |w_out|^2."""
+    row-major); out: complex128 device buffer of >= 2k + 2 values receiving c1, |w|^2, c2, |w_out|^2.
+    v_next (optional complex128 vector, e.g. V[k]): receives w_out / |w_out|, or w_out when
+    |w_out| <= eps |w|."""
     torch = _torch()
     n = w.shape[0]
     if V.dtype != torch.complex128 or V.dim() != 2 or V.stride(1) != 1 or V.shape[1] != n or not 1 <= k <= V.shape[0]:
         raise ValueError("V must be a row-major (m, n) complex128 tensor with k <= m rows used")
     if w.dtype != torch.complex128 or not w.is_contiguous() or out.dtype != torch.complex128 or out.numel() < 2 * k + 2:
         raise ValueError("w must be a contiguous complex128 vector and out hold 2k + 2 complex128 values")
-    check(lib().bem_gmres_cgs2(ptr(V), V.stride(0), int(k), int(n), ptr(w), ptr(out), stream_handle()))
+    if v_next is not None and (v_next.dtype != torch.complex128 or not v_next.is_contiguous() or v_next.shape != (n,)):
+        raise ValueError("v_next must be a contiguous complex128 vector of length n")
+    check(lib().bem_gmres_cgs2(ptr(V), V.stride(0), int(k), int(n), ptr(w), ptr(out),
+                               None if v_next is None else ptr(v_next), float(eps), stream_handle()))
 
 
 def regular_assemble_sl_dl(op, test, trial, test_blocks, trial_blocks, grids_identical, rule, wavenumber, precision,

@@ -432,20 +432,37 @@ __global__ __launch_bounds__(kThreads) void cgs2_kernel(const double2 *V, int64_
     }
 }
 
+// next basis vector: v = w / |w| (or w itself when |w| <= eps |w_in|, the breakdown test of GMRES)
+__global__ __launch_bounds__(kThreads) void cgs2_scale_kernel(const double2 *w, int64_t n, int k, const double *out,
+                                                              double eps, double2 *v) {
+    const double h0 = sqrt(out[2 * k]), h1 = sqrt(out[2 * (2 * k + 1)]);
+    const bool keep = h1 <= eps * h0;
+    for (int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x; j < n; j += (int64_t)gridDim.x * kThreads) {
+        double2 x = w[j];
+        if (!keep) {
+            x.x /= h1;
+            x.y /= h1;
+        }
+        v[j] = x;
+    }
+}
+
 }  // namespace gs
 
 }  // namespace bem
 
 using namespace bem;
 
-extern "C" int bem_gmres_cgs2(const void *V, int64_t ldv, int k, int64_t n, void *w, void *out, void *stream) {
+extern "C" int bem_gmres_cgs2(const void *V, int64_t ldv, int k, int64_t n, void *w, void *out, void *v_next,
+                              double eps, void *stream) {
     clear_error();
     if (!V || !w || !out || k < 1 || n < 1 || ldv < n) {
         set_error("invalid argument");
         return BEM_ERR_INVALID_ARGUMENT;
     }
-    if (reinterpret_cast<uintptr_t>(V) % 16 || reinterpret_cast<uintptr_t>(w) % 16 || reinterpret_cast<uintptr_t>(out) % 8) {
-        set_error("V and w must be 16-byte aligned complex128 arrays");
+    if (reinterpret_cast<uintptr_t>(V) % 16 || reinterpret_cast<uintptr_t>(w) % 16 || reinterpret_cast<uintptr_t>(out) % 8 ||
+        reinterpret_cast<uintptr_t>(v_next) % 16) {
+        set_error("V, w and v_next must be 16-byte aligned complex128 arrays");
         return BEM_ERR_INVALID_ARGUMENT;
     }
     hipStream_t s = (hipStream_t)stream;
@@ -460,6 +477,8 @@ extern "C" int bem_gmres_cgs2(const void *V, int64_t ldv, int k, int64_t n, void
     hipLaunchKernelGGL(gs::cgs2_kernel<1>, grid, dim3(gs::kThreads), 0, s, Vd, ldv, k, n, wd, od);
     hipLaunchKernelGGL(gs::cgs2_kernel<2>, grid, dim3(gs::kThreads), 0, s, Vd, ldv, k, n, wd, od);
     hipLaunchKernelGGL(gs::cgs2_kernel<3>, grid, dim3(gs::kThreads), 0, s, Vd, ldv, k, n, wd, od);
+    if (v_next)
+        hipLaunchKernelGGL(gs::cgs2_scale_kernel, grid, dim3(gs::kThreads), 0, s, wd, n, k, od, eps, (double2 *)v_next);
     return check_launch("cgs2_kernel");
 }
 

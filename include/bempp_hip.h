@@ -343,8 +343,12 @@ int bem_adjoint_potential_single_layer(const bem_space_t *space, const bem_quad_
  *   w2 = w1 - V^T c2,  w1 = w - V^T c1,  c1 = conj(V) w,  c2 = conj(V) w1.
  * out: device buffer of 2k + 2 complex128 values, overwritten with c1 (k values), |w|^2 (real
  * part), c2 (k values), |w2|^2. Sums are FP64 with atomics (summation order not fixed).
+ * v_next (optional, length n, may be a row of V past k): receives the next basis vector
+ * w2 / |w2|, or w2 itself when |w2| <= eps |w| (breakdown), so that the next step can start
+ * before the host has read out.
  */
-int bem_gmres_cgs2(const void *V, int64_t ldv, int k, int64_t n, void *w, void *out, void *stream);
+int bem_gmres_cgs2(const void *V, int64_t ldv, int k, int64_t n, void *w, void *out, void *v_next, double eps,
+                   void *stream);
 
 /*
  * y = op(A) x for a dense row-major matrix in HBM (the product of the reference's

@@ -202,8 +202,9 @@ def test_discrete_operator_products(dtype, default_parameters, hip_available):
 @pytest.mark.parametrize("k,n", [(1, 37), (5, 1000), (21, 10242), (40, 200000)])
 def test_gmres_cgs2_entry_point(k, n, hip_available):
     """bem_gmres_cgs2 against the same CGS2 arithmetic in numpy (FP64): the coefficients, both
-    norms and the updated vector; n past one block, past the 512-block grid cap (grid-stride) and
-    k past the restart length used elsewhere."""
+    norms, the updated vector and the next basis vector (row k of V: w2 / |w2|, or w2 itself when
+    the breakdown test |w2| <= eps |w| holds); n past one block, past the 512-block grid cap
+    (grid-stride) and k past the restart length used elsewhere."""
     import torch
 
     from bempp_amd.core import hip_backend as hb
@@ -220,7 +221,7 @@ def test_gmres_cgs2_entry_point(k, n, hip_available):
     Vt = torch.as_tensor(V, device=dev)
     wt = torch.as_tensor(w, device=dev)
     out = torch.full((2 * k + 2,), 7.0 + 7.0j, dtype=torch.complex128, device=dev)  # overwritten
-    hb.gmres_cgs2(Vt, k, wt, out)
+    hb.gmres_cgs2(Vt, k, wt, out, Vt[k], 1e-14)
     g = out.cpu().numpy()
     scale = np.linalg.norm(w)
     np.testing.assert_allclose(g[:k], c1, rtol=0, atol=1e-12 * scale)
@@ -228,6 +229,13 @@ def test_gmres_cgs2_entry_point(k, n, hip_available):
     assert abs(g[k].real - np.vdot(w, w).real) <= 1e-12 * scale**2 and g[k].imag == 0.0
     assert abs(g[2 * k + 1].real - np.vdot(w2, w2).real) <= 1e-12 * scale**2
     np.testing.assert_allclose(wt.cpu().numpy(), w2, rtol=0, atol=1e-12 * scale)
-    # rows past k are untouched by the products
-    np.testing.assert_array_equal(Vt.cpu().numpy(), V)
+    Vh = Vt.cpu().numpy()
+    np.testing.assert_allclose(Vh[k], w2 / np.linalg.norm(w2), rtol=0, atol=1e-12)
+    # the other rows are untouched
+    np.testing.assert_array_equal(Vh[:k], V[:k])
+    np.testing.assert_array_equal(Vh[k + 1], V[k + 1])
+    # breakdown test (eps so large that it holds): row k + 1 receives w_out unscaled
+    w3 = wt.clone()
+    hb.gmres_cgs2(Vt, k, w3, out, Vt[k + 1], 1e300)
+    np.testing.assert_array_equal(Vt[k + 1].cpu().numpy(), w3.cpu().numpy())
 
