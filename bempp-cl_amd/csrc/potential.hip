@@ -174,6 +174,19 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_fp64(const PotArgs
 
 // MIXED evaluation: FP32 per source, FP64 per (target, group of sources). EFIE uses groups of
 // 32 sources (a few neighbouring elements) as references, the scalar kernels one group per tile.
+// sin and cos of a half angle x (radians) with relative accuracy, for e^{i 2x} - 1 = -2 sin^2 x +
+// 2i sin x cos x, a small value that must not inherit the absolute error of the hardware sine:
+// Taylor series for |x| <= 0.5 (remainders x^11 / 11! and x^10 / 10! < 3e-10), sincosf beyond.
+__device__ __forceinline__ void half_angle_sincos(float x, float &s, float &c) {
+    if (fabsf(x) <= 0.5f) {
+        const float x2 = x * x;
+        s = x * (1.f - x2 * (1.f / 6.f) * (1.f - x2 * (1.f / 20.f) * (1.f - x2 * (1.f / 42.f) * (1.f - x2 * (1.f / 72.f)))));
+        c = 1.f - x2 * 0.5f * (1.f - x2 * (1.f / 12.f) * (1.f - x2 * (1.f / 30.f) * (1.f - x2 * (1.f / 56.f))));
+    } else {
+        sincosf(x, &s, &c);
+    }
+}
+
 template <int OP>
 __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed(const PotArgs a) {
     constexpr int KD = OP == kMaxwellEFIE ? 3 : 1;
@@ -196,6 +209,8 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed(const PotArg
     double accr[KD], acci[KD];
     for (int d = 0; d < KD; ++d) accr[d] = acci[d] = 0.0;
     const float kr = (float)a.kr, ki = (float)a.ki;
+    // phases in revolutions for the hardware sin / cos (|k (r - r0)| stays within a few radians)
+    const float kr_rev = (float)(a.kr * 0.15915494309189533577);
     for (int t0 = s0; t0 < s1; t0 += kPotTile) {
         const int nt = min(kPotTile, s1 - t0);
         __syncthreads();
@@ -256,9 +271,8 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed(const PotArg
             float r2 = r0sq + tt;
             float rinv = rsqrt_t(r2);
             float r = r2 * rinv;
-            float delta = tt / (r + r0);
-            float s, c;
-            sincos_t(kr * delta, &s, &c);
+            float delta = tt * __builtin_amdgcn_rcpf(r + r0);
+            const float s = __builtin_amdgcn_sinf(kr_rev * delta), c = __builtin_amdgcn_cosf(kr_rev * delta);
             float damp = ki != 0.f ? __expf(-ki * delta) : 1.f;
             // e^{ik(r - r0)} / (4 pi r) ...
             float g = float(kInv4Pi) * rinv * damp;
@@ -281,7 +295,7 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed(const PotArg
                 (void)ei;
                 // E - 1 = e^{ik delta} - 1 from half-angle sin/cos (no cancellation for small delta)
                 float sh2, ch2;
-                sincos_t(0.5f * kr * delta, &sh2, &ch2);
+                half_angle_sincos(0.5f * kr * delta, sh2, ch2);
                 float em1r = -2.f * sh2 * sh2, em1i = 2.f * sh2 * ch2;
                 if (ki != 0.f) {
                     const float dmp = __expf(-ki * delta), dm1 = expm1f(-ki * delta);
@@ -600,8 +614,8 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_mixed(const PotArg
                 pi = kInv4Pi * sn;
                 for (int j = j0; j < j1; ++j) {
                     // e^{i theta} - 1 = -2 sin^2(theta / 2) + 2 i sin(theta / 2) cos(theta / 2)
-                    const float half = -0.5f * kr_rev * (xf0 * sh[0][j] + xf1 * sh[1][j] + xf2 * sh[2][j]);
-                    const float hs = __builtin_amdgcn_sinf(half), hc = __builtin_amdgcn_cosf(half);
+                    float hs, hc;
+                    half_angle_sincos(-0.5f * kr * (xf0 * sh[0][j] + xf1 * sh[1][j] + xf2 * sh[2][j]), hs, hc);
                     const float em1r = -2.f * hs * hs, em1i = 2.f * hs * hc;
                     if constexpr (FF == kFarSL || FF == kFarDL) {
                         float qr = sh[6][j], qi = sh[7][j];
