@@ -358,12 +358,24 @@ int square_matrix_op(int what, void *A, int dtype, int64_t n, int64_t ld, hipStr
 //   1: c1 = conj(V) w and |w|^2;
 //   2: w <- w - V^T c1 and, from the updated elements (the update is elementwise), c2 = conj(V) w;
 //   3: w <- w - V^T c2 and |w|^2.
-// A thread owns elements j of w (grid-stride); its contributions to the k dot products are summed
-// over the block (wave shuffles, then LDS) and added to out with FP64 atomics.
+// A thread owns elements j of w (grid-stride). For the k dot products each wave sums its lanes
+// with shuffles into an LDS row per dot product; after one barrier per 256 dot products the
+// block adds the waves' sums to out with one FP64 atomic per value.
 // out (complex128): c1[0, k), |w_in|^2 at k, c2[k + 1, 2k + 1), |w_out|^2 at 2k + 1.
 namespace gs {
 
 constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kDotChunk = 256;  // dot products staged in LDS per barrier
+
+// (re, im) summed over the wavefront (every lane gets the total)
+__device__ __forceinline__ void wave_sum2(double &re, double &im) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        re += __shfl_xor(re, off, 64);
+        im += __shfl_xor(im, off, 64);
+    }
+}
 
 // (re, im) summed over the block; every thread gets the total
 __device__ __forceinline__ void block_sum2(double &re, double &im, double (*red)[2]) {
@@ -405,19 +417,36 @@ __global__ __launch_bounds__(kThreads) void cgs2_kernel(const double2 *V, int64_
     }
     if (PASS <= 2) {
         // dots conj(V_i) . w into out[0, k) (pass 1) or out[k + 1, 2k + 1) (pass 2)
+        __shared__ double part[kWaves][kDotChunk][2];
         double *dst = out + 2 * (PASS == 1 ? 0 : k + 1);
-        for (int i = 0; i < k; ++i) {
-            double re = 0.0, im = 0.0;
-            for (int64_t j = j0; j < n; j += stride) {
-                const double2 a = V[(int64_t)i * ldv + j], v = w[j];
-                re += a.x * v.x + a.y * v.y;
-                im += a.x * v.y - a.y * v.x;
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        for (int i0 = 0; i0 < k; i0 += kDotChunk) {
+            const int m = min(kDotChunk, k - i0);
+            for (int i = 0; i < m; ++i) {
+                double re = 0.0, im = 0.0;
+                for (int64_t j = j0; j < n; j += stride) {
+                    const double2 a = V[(int64_t)(i0 + i) * ldv + j], v = w[j];
+                    re += a.x * v.x + a.y * v.y;
+                    im += a.x * v.y - a.y * v.x;
+                }
+                wave_sum2(re, im);
+                if (lane == 0) {
+                    part[wave][i][0] = re;
+                    part[wave][i][1] = im;
+                }
             }
-            block_sum2(re, im, red);
-            if (threadIdx.x == 0) {
-                atomic_add(dst + 2 * i, re);
-                atomic_add(dst + 2 * i + 1, im);
+            __syncthreads();
+            for (int i = threadIdx.x; i < m; i += kThreads) {
+                double re = 0.0, im = 0.0;
+#pragma unroll
+                for (int wv = 0; wv < kWaves; ++wv) {
+                    re += part[wv][i][0];
+                    im += part[wv][i][1];
+                }
+                atomic_add(dst + 2 * (i0 + i), re);
+                atomic_add(dst + 2 * (i0 + i) + 1, im);
             }
+            __syncthreads();  // part is reused by the next chunk
         }
     }
     if (PASS != 2) {

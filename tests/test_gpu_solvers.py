@@ -199,12 +199,13 @@ def test_discrete_operator_products(dtype, default_parameters, hip_available):
     np.testing.assert_allclose(S.to_dense(), (0.5 * M + A).astype(S.dtype), rtol=1e-6, atol=1e-12)
 
 
-@pytest.mark.parametrize("k,n", [(1, 37), (5, 1000), (21, 10242), (40, 200000)])
+@pytest.mark.parametrize("k,n", [(1, 37), (5, 1000), (21, 10242), (40, 200000), (300, 2000)])
 def test_gmres_cgs2_entry_point(k, n, hip_available):
     """bem_gmres_cgs2 against the same CGS2 arithmetic in numpy (FP64): the coefficients, both
     norms, the updated vector and the next basis vector (row k of V: w2 / |w2|, or w2 itself when
     the breakdown test |w2| <= eps |w| holds); n past one block, past the 512-block grid cap
-    (grid-stride) and k past the restart length used elsewhere."""
+    (grid-stride), k past the restart length used elsewhere and past the 256 dot products staged
+    per barrier."""
     import torch
 
     from bempp_amd.core import hip_backend as hb
