@@ -121,9 +121,13 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     else:
         if tuple(out.shape) != (rows, cols) or out.dtype != tdtype or not out.is_contiguous():
             raise ValueError(f"out must be a contiguous ({rows}, {cols}) {dtype} tensor")
-    if not accumulate and not full_cover:
-        hb.fill_zero(out)
     hip_opts = getattr(parameters, "hip", None)
+    sym_opt = getattr(hip_opts, "symmetric", True) if hip_opts is not None else True
+    # a single layer on one P1 space: every unordered pair once; the library zeroes the output and
+    # adds the transpose itself
+    sym_sl = not accumulate and sym_opt and op in _SYMMETRIC_SINGLE_LAYERS and use_symmetric_pass(domain, dual_to_range)
+    if not accumulate and not full_cover and not sym_sl:
+        hb.fill_zero(out)
     near = getattr(hip_opts, "mixed_near_field_factor", DEFAULT_NEAR_FIELD_FACTOR) if hip_opts is not None else DEFAULT_NEAR_FIELD_FACTOR
     kahan = getattr(hip_opts, "kahan", False) if hip_opts is not None else False
     srule = None
@@ -148,6 +152,11 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     elif upper:
         hb.regular_assemble_symmetric(op, test, rule, _wavenumber(operator_descriptor), mode, out, near_factor=near,
                                       kahan=kahan)
+    elif sym_sl:
+        from bempp_amd.core.blocking import device_symmetric_blocks
+
+        hb.regular_assemble_sl_dl_symmetric(op, device_symmetric_blocks(domain), rule, _wavenumber(operator_descriptor),
+                                            mode, out, None, near_factor=near, kahan=kahan)
     elif direct:
         hb.regular_assemble(
             op, test, trial, grids_identical, rule, _wavenumber(operator_descriptor), mode, out,

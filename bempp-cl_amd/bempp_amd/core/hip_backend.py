@@ -467,11 +467,11 @@ def regular_assemble_sl_dl_symmetric(op, sym, rule, wavenumber, precision, out_s
     """Single layer (op) and its double layer on one P1 space, every unordered pair once.
 
     sym: blocking.DeviceSymmetricBlocks of the space. Both outputs are overwritten with the
-    regular part (square, contiguous, same dtype).
+    regular part (square, contiguous, same dtype). out_double None: the single layer alone.
     """
     k = complex(wavenumber)
     n = out_single.shape[0]
-    for o in (out_single, out_double):
+    for o in (out_single,) if out_double is None else (out_single, out_double):
         if o.dim() != 2 or tuple(o.shape) != (n, n) or not o.is_contiguous() or o.dtype != out_single.dtype:
             raise ValueError("outputs must be contiguous square matrices of one dtype")
     check(

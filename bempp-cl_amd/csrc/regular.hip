@@ -83,8 +83,12 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
         return BEM_ERR_UNSUPPORTED;
     }
     if (work) {
-        if (op == BEM_LAPLACE_SINGLE_LAYER) return regular_launch_sym_0(a, precision, kahan, out_dtype, s);
-        if (op == BEM_HELMHOLTZ_SINGLE_LAYER) return regular_launch_sym_3(a, precision, kahan, out_dtype, s);
+        if (op == BEM_LAPLACE_SINGLE_LAYER)
+            return out2 ? regular_launch_sym_0(a, precision, kahan, out_dtype, s)
+                        : regular_launch_symsl_0(a, precision, kahan, out_dtype, s);
+        if (op == BEM_HELMHOLTZ_SINGLE_LAYER)
+            return out2 ? regular_launch_sym_3(a, precision, kahan, out_dtype, s)
+                        : regular_launch_symsl_3(a, precision, kahan, out_dtype, s);
         set_error("the symmetric single/double layer pass needs a single-layer operator, got %d", op);
         return BEM_ERR_UNSUPPORTED;
     }
@@ -157,7 +161,7 @@ extern "C" int bem_regular_assemble_sl_dl_symmetric(int op, const bem_space_t *s
                                                     int kahan, void *out_single, void *out_double, int64_t ld,
                                                     int out_dtype, void *stream) {
     clear_error();
-    if (!space || !test_tiles || !trial_blocks || !out_single || !out_double || (n_work > 0 && !work)) {
+    if (!space || !test_tiles || !trial_blocks || !out_single || (n_work > 0 && !work)) {
         set_error("null argument");
         return BEM_ERR_INVALID_ARGUMENT;
     }
@@ -174,7 +178,7 @@ extern "C" int bem_regular_assemble_sl_dl_symmetric(int op, const bem_space_t *s
     // both outputs are overwritten: zero, then the pairs (e, f) with e before f in the element
     // order, then the single layer's transpose
     if ((rc = square_matrix_op(0, out_single, out_dtype, n_dofs, ld, s))) return rc;
-    if ((rc = square_matrix_op(0, out_double, out_dtype, n_dofs, ld, s))) return rc;
+    if (out_double && (rc = square_matrix_op(0, out_double, out_dtype, n_dofs, ld, s))) return rc;
     if (n_work > 0) {
         rc = regular_entry(op, space, space, test_tiles, trial_blocks, 1, rule, k_re, k_im, precision,
                            mixed_near_factor, kahan, out_single, out_double, ld, out_dtype, 0, stream, work, n_work);

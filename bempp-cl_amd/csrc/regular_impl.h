@@ -706,19 +706,20 @@ __device__ __forceinline__ void store_direct(void *out, int64_t ld, int row, int
 // ------------------------------------------------------------------ kernel
 // DUAL: OP is a single layer and the kernel also assembles the double layer of the same family
 // into a.out2 from the same point-pair geometry and phase factors.
-// SYM (DUAL, TILE, test space == trial space on one grid): every unordered element pair is
+// SYM (TILE, test space == trial space on one grid): every unordered element pair is
 // evaluated once. The test tiles are cut inside the trial blocks of one element order; a work
 // item is a trial block with a run of test tiles, all of them in this block or in earlier ones
 // (the workgroup keeps its trial elements in registers across the run). In the tile's own
 // block only the pairs with list position of the trial element > that of the test element
 // count. Each pair (e, f) adds its single-layer block S_ef to a.out at (e, f) only (the caller
-// adds the transpose afterwards: S_fe = S_ef^T), its double-layer block to a.out2 at (e, f)
-// and the double layer of the reversed pair (test normal) to a.out2 at (f, e).
-// waves per SIMD the launch bound asks for: the FP32 fused pass fits 3 (168 VGPRs, 52.9 KB LDS);
-// the symmetric pass and the mixed EFIE pass keep more state (2); the others leave it to the compiler
+// adds the transpose afterwards: S_fe = S_ef^T); with DUAL also its double-layer block to a.out2
+// at (e, f) and the double layer of the reversed pair (test normal) to a.out2 at (f, e) (REV).
+// waves per SIMD the launch bound asks for: the FP32 fused pass and the symmetric single layer
+// fit 3; the symmetric single + double layer pass and the mixed EFIE pass keep more state (2);
+// the others leave it to the compiler
 template <int OP, bool MIXED, bool DUAL, bool SYM>
 constexpr int regular_waves() {
-    return !MIXED ? 1 : SYM ? 2 : DUAL ? 3 : is_rwg_op<OP>() ? 2 : 1;
+    return !MIXED ? 1 : SYM ? (DUAL ? 2 : 3) : DUAL ? 3 : is_rwg_op<OP>() ? 2 : 1;
 }
 
 // UPPER (direct DP0 stores, a symmetric operator on one space): only the pairs whose trial element
@@ -728,7 +729,9 @@ template <int OP, int NTS, int NSS, int QT, int TM, bool MIXED, bool KAHAN, bool
           typename OutT, bool DUAL = false, bool SYM = false, bool UPPER = false>
 __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>())) void regular_kernel(
     const RegularArgs a) {
-    static_assert(!SYM || (DUAL && TILE), "the symmetric pass is a tiled single + double layer pass");
+    static_assert(!SYM || TILE, "the symmetric pass is tiled");
+    // REV: the double layer of the reversed pair (symmetric single + double layer pass)
+    constexpr bool REV = SYM && DUAL;
     static_assert(!UPPER || (!TILE && !DUAL && NTS == 1 && NSS == 1), "UPPER stores DP0 blocks directly");
     constexpr bool CPLX = OpTraits<OP>::is_complex;
     constexpr int NT = NTS, NS = NSS;
@@ -756,7 +759,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
     __shared__ TestTile<TMC, MIXED ? 1 : TMC, QC, EFX || HSX || MFX> T_;
     __shared__ AccT acc[TILE ? MAXU_T * MAXU_S * NC : 1];
     __shared__ AccT acc2[TILE && DUAL ? MAXU_T * MAXU_S * NC : 1];
-    __shared__ AccT acc3[TILE && SYM ? MAXU_T * MAXU_S * NC : 1];
+    __shared__ AccT acc3[TILE && REV ? MAXU_T * MAXU_S * NC : 1];
     // owner-pass stage, [(j * NT + i)][lane][re, im]; column kRegThreads of the rows j = 0 stays
     // zero and stands in for missing incidences, so the gather below needs no branches
     constexpr int SROW = kRegThreads + 1;
@@ -954,7 +957,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
             for (int i = tid; i < MAXU_T * MAXU_S * (CPLX ? 2 : 1); i += kRegThreads) {
                 acc[i] = (AccT)0;
                 if constexpr (DUAL) acc2[i] = (AccT)0;
-                if constexpr (SYM) acc3[i] = (AccT)0;
+                if constexpr (REV) acc3[i] = (AccT)0;
             }
         }
         if constexpr (!SYM) stage_tests(t0, ntile);
@@ -994,10 +997,10 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                 if constexpr (!MIXED) {
                     double tnm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
                     if constexpr (QT > 0)
-                        pair_fp64_regs<GOP, NTS, NSS, QT, DUAL, SYM>(T_, te, gt, tnm, yq, wq64, ny64,
+                        pair_fp64_regs<GOP, NTS, NSS, QT, DUAL, REV>(T_, te, gt, tnm, yq, wq64, ny64,
                                                                                       a.rule, a.kr, a.ki, S, S2, S3);
                     else
-                        pair_fp64_geom<GOP, NTS, NSS, QT, DUAL, SYM>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2, S3);
+                        pair_fp64_geom<GOP, NTS, NSS, QT, DUAL, REV>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2, S3);
                 } else {
                     const double cfv[3] = {cf0, cf1, cf2};
                     double dist2;
@@ -1007,10 +1010,10 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                         if constexpr (MFX)
                             mfie_pair_fp32<QT>(T_, te, F32, kapf, d0, d1, d2, dist2, a.rule, a.kr, a.ki, Sf);
                         else if (OpTraits<OP>::is_complex && a.ki != 0.0)
-                            pair_fp32<GOP, NTS, NSS, QT, KAHAN, true, DUAL, SYM>(T_, te, F32, d0, d1, d2, dist2, a.rule,
+                            pair_fp32<GOP, NTS, NSS, QT, KAHAN, true, DUAL, REV>(T_, te, F32, d0, d1, d2, dist2, a.rule,
                                                                                        a.kr, a.ki, Sf, Sf2, Sf3);
                         else
-                            pair_fp32<GOP, NTS, NSS, QT, KAHAN, false, DUAL, SYM>(T_, te, F32, d0, d1, d2, dist2,
+                            pair_fp32<GOP, NTS, NSS, QT, KAHAN, false, DUAL, REV>(T_, te, F32, d0, d1, d2, dist2,
                                                                                         a.rule, a.kr, a.ki, Sf, Sf2, Sf3);
                     }
                 }
@@ -1026,7 +1029,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                             S2[i][j].re = (double)Sf2[i][j].re;
                             S2[i][j].im = (double)Sf2[i][j].im;
                         }
-                        if constexpr (SYM) {
+                        if constexpr (REV) {
                             S3[i][j].re = (double)Sf3[i][j].re;
                             S3[i][j].im = (double)Sf3[i][j].im;
                         }
@@ -1096,7 +1099,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                 };
                 owner_pass(Sf, acc);
                 if constexpr (DUAL) owner_pass(Sf2, acc2);
-                if constexpr (SYM) owner_pass(Sf3, acc3);
+                if constexpr (REV) owner_pass(Sf3, acc3);
             } else if (valid) {
     #pragma unroll
                 for (int i = 0; i < NT; ++i) {
@@ -1115,7 +1118,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                                     unsafeAtomicAdd(acc2 + off, (AccT)(v2.re * m));
                                     if constexpr (CPLX) unsafeAtomicAdd(acc2 + off + 1, (AccT)(v2.im * m));
                                 }
-                                if constexpr (SYM) {
+                                if constexpr (REV) {
                                     C2d v3 = local_value<OP2, NTS, NSS>(S3, i, j, gt, gf, a.kr, a.ki);
                                     unsafeAtomicAdd(acc3 + off, (AccT)(v3.re * m));
                                     if constexpr (CPLX) unsafeAtomicAdd(acc3 + off + 1, (AccT)(v3.im * m));
@@ -1170,7 +1173,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
             };
             flush(acc, a.out, false);
             if constexpr (DUAL) flush(acc2, a.out2, false);
-            if constexpr (SYM) flush(acc3, a.out2, true);
+            if constexpr (REV) flush(acc3, a.out2, true);
         }
         // the next tile reuses the test tile and the LDS tiles
         if (tile_id + 1 < tile_end) __syncthreads();
@@ -1277,7 +1280,7 @@ static int launch1(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
             set_error("the symmetric pass needs a space with three shape functions per element (P1)");
             return BEM_ERR_INVALID_ARGUMENT;
         }
-        return launch2<OP, 3, 3, MIXED, OutT, true, true>(a, kahan, tile, stream);
+        return launch2<OP, 3, 3, MIXED, OutT, DUAL, true>(a, kahan, tile, stream);
     } else {
         int nt = nshape_of(a.test.shape), ns = nshape_of(a.trial.shape);
         if (nt == 1 && ns == 1) return launch2<OP, 1, 1, MIXED, OutT, DUAL>(a, kahan, tile, stream);
@@ -1332,6 +1335,9 @@ int regular_launch_dual_3(const RegularArgs &a, int precision, int kahan, bool t
 // symmetric single + double layer pass (regular_sym<k>.hip)
 int regular_launch_sym_0(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
 int regular_launch_sym_3(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
+// the symmetric pass of the single layer alone
+int regular_launch_symsl_0(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
+int regular_launch_symsl_3(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
 // symmetric DP0 single layer, upper triangle (regular_upper.hip)
 int regular_launch_upper_0(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
 int regular_launch_upper_3(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
