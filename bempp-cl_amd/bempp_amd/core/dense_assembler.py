@@ -123,9 +123,9 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
             raise ValueError(f"out must be a contiguous ({rows}, {cols}) {dtype} tensor")
     hip_opts = getattr(parameters, "hip", None)
     sym_opt = getattr(hip_opts, "symmetric", True) if hip_opts is not None else True
-    # a single layer on one P1 space: every unordered pair once; the library zeroes the output and
-    # adds the transpose itself
-    sym_sl = not accumulate and sym_opt and op in _SYMMETRIC_SINGLE_LAYERS and use_symmetric_pass(domain, dual_to_range)
+    # a single layer or hypersingular operator on one P1 space: every unordered pair once; the
+    # library zeroes the output and adds the transpose itself
+    sym_sl = not accumulate and sym_opt and op in _SYMMETRIC_P1_OPERATORS and use_symmetric_pass(domain, dual_to_range)
     if not accumulate and not full_cover and not sym_sl:
         hb.fill_zero(out)
     near = getattr(hip_opts, "mixed_near_field_factor", DEFAULT_NEAR_FIELD_FACTOR) if hip_opts is not None else DEFAULT_NEAR_FIELD_FACTOR
@@ -212,6 +212,8 @@ def can_fuse_sl_dl(first, second):
 
 
 _SYMMETRIC_SINGLE_LAYERS = ("laplace_single_layer", "helmholtz_single_layer")
+# operators whose matrix on one space is symmetric (the symmetric P1 traversal)
+_SYMMETRIC_P1_OPERATORS = _SYMMETRIC_SINGLE_LAYERS + ("laplace_hypersingular", "helmholtz_hypersingular")
 
 
 def use_upper_triangle(domain, dual_to_range, op):

@@ -1267,7 +1267,7 @@ static int launch1(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
             set_error("hypersingular operators need P1 shape functions on both sides");
             return BEM_ERR_INVALID_ARGUMENT;
         }
-        return launch2<OP, 3, 3, MIXED, OutT>(a, kahan, tile, stream);
+        return launch2<OP, 3, 3, MIXED, OutT, false, SYM>(a, kahan, tile, stream);
     } else if constexpr (UPPER) {
         if (nshape_of(a.test.shape) != 1 || nshape_of(a.trial.shape) != 1) {
             set_error("the symmetric DP0 pass needs piecewise constant spaces");
@@ -1335,9 +1335,11 @@ int regular_launch_dual_3(const RegularArgs &a, int precision, int kahan, bool t
 // symmetric single + double layer pass (regular_sym<k>.hip)
 int regular_launch_sym_0(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
 int regular_launch_sym_3(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
-// the symmetric pass of the single layer alone
+// the symmetric pass of one symmetric operator alone (single layers, hypersingular operators)
 int regular_launch_symsl_0(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
 int regular_launch_symsl_3(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
+int regular_launch_symsl_7(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
+int regular_launch_symsl_8(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
 // symmetric DP0 single layer, upper triangle (regular_upper.hip)
 int regular_launch_upper_0(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
 int regular_launch_upper_3(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);

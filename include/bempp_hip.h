@@ -205,8 +205,9 @@ int bem_regular_assemble_sl_dl(int op, const bem_space_t *test, const bem_space_
  * with the trial element later in the list than the test element are evaluated. out_single and out_double (n_dofs x n_dofs,
  * leading dimension ld) are OVERWRITTEN with the regular part: pairs sharing a vertex are
  * excluded, and in MIXED precision so are the near pairs (bem_near_assemble_sl_dl adds them).
- * out_double may be NULL: the single layer alone, at the cost of its own point pairs only (the
- * near pairs then come from bem_near_assemble).
+ * out_double may be NULL: the operator alone, at the cost of its own point pairs only (the near
+ * pairs then come from bem_near_assemble). op may then also be BEM_LAPLACE_HYPERSINGULAR or
+ * BEM_HELMHOLTZ_HYPERSINGULAR, symmetric like the single layers (W_fe = W_ef^T).
  */
 int bem_regular_assemble_sl_dl_symmetric(int op, const bem_space_t *space, const bem_dof_blocks_t *test_tiles,
                                          const bem_dof_blocks_t *trial_blocks, int n_work, const int32_t *work,
