@@ -321,6 +321,8 @@ __global__ __launch_bounds__(256) void zero_square_kernel(E *A, int64_t n, int64
 template <typename E>
 static int square_launch(int what, void *A, int64_t n, int64_t ld, hipStream_t s) {
     if (what == 0) {
+        // contiguous rows: one 16-byte store per lane over the whole matrix
+        if (ld == n && reinterpret_cast<uintptr_t>(A) % 16 == 0) return bem_fill_zero(A, (size_t)n * n * sizeof(E), s);
         const unsigned rows = (unsigned)(n < 4096 ? n : 4096);
         hipLaunchKernelGGL((zero_square_kernel<E>), dim3((unsigned)((n + 255) / 256), rows), dim3(256), 0, s, (E *)A, n,
                            ld);
