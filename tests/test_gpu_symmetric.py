@@ -159,3 +159,25 @@ def test_dp0_single_layer_upper_triangle(fam, k, default_parameters, hip_availab
     ref = O.assemble(f"{fam}_single_layer", grid.vertices, grid.elements, dp0, dp0, 0.0 if k is None else k)
     assert np.linalg.norm(res["double", True].cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-12
 
+
+@pytest.mark.parametrize("dtype", ["float32", "float64", "complex64", "complex128"])
+@pytest.mark.parametrize("n,ld", [(37, 37), (1000, 1000), (37, 41), (300, 320)])
+def test_dense_zero_contiguous_and_strided(dtype, n, ld, hip_available):
+    """bem_dense_zero: a contiguous matrix (ld == n) takes the 16-byte fill, including its tail
+    bytes (37^2 float32 values are not a multiple of 16 bytes); a strided one (ld > n) the
+    row kernel. The n x n block is zeroed and the padding columns keep their values."""
+    import ctypes
+
+    import torch
+
+    from bempp_amd.core import hip_backend as hb
+
+    torch.manual_seed(n + ld)
+    a = torch.randn((n, ld), dtype=getattr(torch, dtype), device=hb.device()) + 1
+    before = a.cpu()
+    rc = hb.lib().bem_dense_zero(ctypes.c_void_p(a.data_ptr()), hb._dtype_code(a), n, ld, hb.stream_handle())
+    assert rc == 0
+    after = a.cpu()
+    assert bool((after[:, :n] == 0).all())
+    assert torch.equal(after[:, n:], before[:, n:])
+
