@@ -117,6 +117,7 @@ def _fusion_key(op):
         p.assembly.always_promote_to_double,
         getattr(hip, "mixed_near_field_factor", DEFAULT_NEAR_FIELD_FACTOR),
         getattr(hip, "kahan", False),
+        getattr(hip, "symmetric", True),
         op.assembler._device_interface,
     )
 

@@ -62,12 +62,25 @@ class SingleLayerProblem(object):
             g = self._topology
         else:
             g = self._topology.with_vertices(v.T)
+        if self._last is not None:
+            # the previous vertex set is superseded: free its device copies (geometry, dof maps,
+            # near-field candidates) so that a loop over moving vertices does not accumulate them
+            from bempp_amd.core import release
+
+            old_space = self._space if self._space is not None and self._space.grid is self._last[1] else None
+            release(self._last[1], (old_space,))
+            if old_space is not None:
+                self._space = None
         self._last = (v.copy(), g)
         return g
 
     def space(self, grid):
         """The DP0 space of a grid from `grid` (kept for the last grid, so device copies are reused)."""
         if self._space is None or self._space.grid is not grid:
+            if self._space is not None:
+                from bempp_amd.core import release
+
+                release(None, (self._space,))
             self._space = _dp0_space(grid)
         return self._space
 

@@ -55,12 +55,32 @@ def make_key(op, test_shapeset, trial_shapeset, rows, cols, nk, precision, devic
     return "|".join(str(x) for x in (op, test_shapeset, trial_shapeset, rows, cols, nk, precision, device_name))
 
 
+def _valid_entry(entry):
+    """The launch shape of a persisted entry, or None if the entry is malformed or outside the
+    candidate set (a stale or hand-edited cache file then only costs a re-tune)."""
+    try:
+        cfg = entry["config"]
+        if len(cfg) != 2:
+            return None
+        tile, wg = cfg
+        if isinstance(tile, bool) or isinstance(wg, bool) or int(tile) != tile or int(wg) != wg:
+            return None
+        tile, wg = int(tile), int(wg)
+    except (TypeError, KeyError, ValueError, IndexError):
+        return None
+    if tile not in CANDIDATE_TILES or wg not in CANDIDATE_WG_PER_CU:
+        return None
+    return (tile, wg)
+
+
 def lookup(key):
     if key in _MEMORY:
         return _MEMORY[key]
     entry = _load().get(key)
     if entry is not None:
-        cfg = (int(entry["config"][0]), int(entry["config"][1]))
+        cfg = _valid_entry(entry)
+        if cfg is None:
+            return None
         _MEMORY[key] = cfg
         return cfg
     return None

@@ -67,12 +67,16 @@ class BemDofBlocks(ctypes.Structure):
     ]
 
 
+QUAD_MAX_POINTS = 79  # BEM_QUAD_MAX_POINTS: the longest symmetric rule (order 20)
+QUAD_FAST_POINTS = 16  # BEM_QUAD_FAST_POINTS: longer rules take the FP64 one-lane-per-pair kernel
+
+
 class BemQuadRule(ctypes.Structure):
     _fields_ = [
         ("n_points", ctypes.c_int32),
-        ("x", ctypes.c_double * 16),
-        ("y", ctypes.c_double * 16),
-        ("w", ctypes.c_double * 16),
+        ("x", ctypes.c_double * QUAD_MAX_POINTS),
+        ("y", ctypes.c_double * QUAD_MAX_POINTS),
+        ("w", ctypes.c_double * QUAD_MAX_POINTS),
     ]
 
 
@@ -145,6 +149,8 @@ def lib():
         L.bem_dense_add_transpose.restype = i
         L.bem_dense_zero.argtypes = [P, i, i64, i64, P]
         L.bem_dense_zero.restype = i
+        L.bem_dense_mirror_upper.argtypes = [P, i, i64, i64, P]
+        L.bem_dense_mirror_upper.restype = i
         L.bem_regular_assemble_symmetric.argtypes = [i, SP, QP, d, d, i, d, i, P, i64, i, P]
         L.bem_regular_assemble_symmetric.restype = i
         L.bem_near_assemble_sl_dl.argtypes = [i, SP, SP, i, P, i, QP, d, d, d, P, P, i64, i, P]
@@ -187,8 +193,8 @@ def ptr(t):
 def quad_rule(points, weights):
     r = BemQuadRule()
     n = len(weights)
-    if n > 16:
-        raise ValueError("quadrature rules with more than 16 points are not supported")
+    if n > QUAD_MAX_POINTS:
+        raise ValueError(f"quadrature rules with more than {QUAD_MAX_POINTS} points are not supported")
     r.n_points = n
     for k in range(n):
         r.x[k] = float(points[0, k])
@@ -285,7 +291,7 @@ def _dtype_code(t):
 
 
 def regular_assemble(op, test, trial, grids_identical, rule, wavenumber, precision, out, accumulate=0,
-                     near_factor=DEFAULT_NEAR_FIELD_FACTOR, kahan=True):
+                     near_factor=DEFAULT_NEAR_FIELD_FACTOR, kahan=False):
     """Regular pairs of localised spaces (DP0/DP1): direct stores into out."""
     k = complex(wavenumber)
     check(
@@ -298,7 +304,7 @@ def regular_assemble(op, test, trial, grids_identical, rule, wavenumber, precisi
 
 
 def regular_assemble_blocked(op, test_blocks, trial_blocks, grids_identical, rule, wavenumber, precision, out,
-                             near_factor=DEFAULT_NEAR_FIELD_FACTOR, kahan=True):
+                             near_factor=DEFAULT_NEAR_FIELD_FACTOR, kahan=False):
     """Regular pairs of spaces with shared dofs (P1, RWG/SNC): LDS-tiled sums added to out."""
     k = complex(wavenumber)
     check(
@@ -512,6 +518,13 @@ def dense_zero(a):
     if a.dim() != 2 or a.shape[0] != a.shape[1] or not a.is_contiguous():
         raise ValueError("expected a contiguous square matrix")
     check(lib().bem_dense_zero(ptr(a), _dtype_code(a), a.shape[0], a.shape[1], stream_handle()))
+
+
+def dense_mirror_upper(a):
+    """Lower triangle <- upper triangle, zero diagonal (square contiguous device matrix)."""
+    if a.dim() != 2 or a.shape[0] != a.shape[1] or not a.is_contiguous():
+        raise ValueError("expected a contiguous square matrix")
+    check(lib().bem_dense_mirror_upper(ptr(a), _dtype_code(a), a.shape[0], a.shape[1], stream_handle()))
 
 
 def near_assemble_sl_dl(op, test, trial, pairs, grids_identical, rule, wavenumber, near_factor, out_single,

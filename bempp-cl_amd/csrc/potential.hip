@@ -29,7 +29,7 @@ constexpr int kSrcFields = 14;  // px py pz nx ny nz c_re c_im t1 (6): EFIE uses
 
 struct PotArgs {
     SpaceArgs sp;
-    RuleArgs rule;
+    RuleArgsWide rule;  // read by the prep kernel only
     const double *coeffs;  // complex128 per (element, shape)
     int npts;
     const double *pts;
@@ -783,11 +783,11 @@ extern "C" size_t bem_potential_workspace_size(int op, const bem_space_t *space,
     return sizeof(double) * ((size_t)kSrcFields * nsrc_pad + (size_t)2 * nchunk * kd * n_points) + 256;
 }
 
-extern "C" int bem_potential_evaluate(int op, const bem_space_t *space, const void *coefficients, int n_points,
-                                      const double *points, const bem_quad_rule_t *rule, double k_re, double k_im,
-                                      int precision, void *out, int out_dtype, void *workspace, size_t workspace_bytes,
-                                      void *stream) {
-    clear_error();
+// Checks, workspace layout and launches of bem_potential_evaluate.
+static int potential_evaluate(int op, const bem_space_t *space, const void *coefficients, int n_points,
+                              const double *points, const bem_quad_rule_t *rule, double k_re, double k_im,
+                              int precision, void *out, int out_dtype, void *workspace, size_t workspace_bytes,
+                              hipStream_t s) {
     if (!space || !rule || !out || !coefficients || (n_points > 0 && !points) || !workspace) {
         set_error("null argument");
         return BEM_ERR_INVALID_ARGUMENT;
@@ -806,8 +806,8 @@ extern "C" int bem_potential_evaluate(int op, const bem_space_t *space, const vo
         set_error("potential output must be complex (BEM_C128 or BEM_C64)");
         return BEM_ERR_INVALID_ARGUMENT;
     }
-    if (rule->n_points < 1 || rule->n_points > 16) {
-        set_error("quadrature rule must have 1..16 points");
+    if (rule->n_points < 1 || rule->n_points > BEM_QUAD_MAX_POINTS) {
+        set_error("quadrature rule must have 1..%d points", BEM_QUAD_MAX_POINTS);
         return BEM_ERR_INVALID_ARGUMENT;
     }
     size_t need = bem_potential_workspace_size(op, space, rule, n_points, precision);
@@ -818,7 +818,7 @@ extern "C" int bem_potential_evaluate(int op, const bem_space_t *space, const vo
     if (n_points == 0) return BEM_OK;
     PotArgs a;
     a.sp = to_args(space);
-    a.rule = to_args(rule);
+    a.rule = to_args_wide(rule);
     a.coeffs = (const double *)coefficients;
     a.npts = n_points;
     a.pts = points;
@@ -830,7 +830,6 @@ extern "C" int bem_potential_evaluate(int op, const bem_space_t *space, const vo
     a.src = (double *)base;
     a.partial = a.src + (size_t)kSrcFields * a.nsrc_pad;
     a.out = out;
-    hipStream_t s = (hipStream_t)stream;
     switch (op) {
     case BEM_LAPLACE_SINGLE_LAYER: return run_potential<kLaplaceSL>(a, precision, out_dtype, s);
     case BEM_LAPLACE_DOUBLE_LAYER: return run_potential<kLaplaceDL>(a, precision, out_dtype, s);
@@ -843,4 +842,13 @@ extern "C" int bem_potential_evaluate(int op, const bem_space_t *space, const vo
     case BEM_MAXWELL_MAGNETIC_FIELD: return run_potential<kMaxwellEFIE, kMagneticPotential>(a, precision, out_dtype, s);
     default: return run_potential<kMaxwellEFIE>(a, precision, out_dtype, s);
     }
+}
+
+extern "C" int bem_potential_evaluate(int op, const bem_space_t *space, const void *coefficients, int n_points,
+                                      const double *points, const bem_quad_rule_t *rule, double k_re, double k_im,
+                                      int precision, void *out, int out_dtype, void *workspace, size_t workspace_bytes,
+                                      void *stream) {
+    clear_error();
+    return potential_evaluate(op, space, coefficients, n_points, points, rule, k_re, k_im, precision, out, out_dtype,
+                              workspace, workspace_bytes, (hipStream_t)stream);
 }

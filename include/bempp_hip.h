@@ -96,10 +96,16 @@ typedef struct bem_space {
     const int32_t *normal_multipliers; /* [n_grid_elements] +1/-1, or NULL for all +1      */
 } bem_space_t;
 
-/* A triangle quadrature rule, host memory, at most 16 points on the reference triangle. */
+/* A triangle quadrature rule, host memory, at most BEM_QUAD_MAX_POINTS points on the reference
+ * triangle (the reference's symmetric rules of orders 1-20 have 1..79 points,
+ * bempp_cl/api/integration/triangle_gauss.py:143-173). Rules of up to BEM_QUAD_FAST_POINTS points
+ * run through every kernel; longer rules are served by bem_regular_assemble,
+ * bem_regular_assemble_blocked (FP64 arithmetic only) and bem_potential_evaluate. */
+#define BEM_QUAD_MAX_POINTS 79
+#define BEM_QUAD_FAST_POINTS 16
 typedef struct bem_quad_rule {
     int32_t n_points;
-    double x[16], y[16], w[16];
+    double x[BEM_QUAD_MAX_POINTS], y[BEM_QUAD_MAX_POINTS], w[BEM_QUAD_MAX_POINTS];
 } bem_quad_rule_t;
 
 /* Blocking of one side for bem_regular_assemble_blocked (device arrays). The element list of
@@ -157,6 +163,10 @@ typedef struct bem_dof_blocks {
  *   distance is below factor * (diameter_test + diameter_trial) / 2 use FP64.
  * kahan: compensated FP32 summation in MIXED mode.
  * out_dtype must be BEM_F64/BEM_F32 for Laplace and BEM_C128/BEM_C64 otherwise.
+ * Rules of more than BEM_QUAD_FAST_POINTS points (regular orders 7-20): precision must be
+ * BEM_PRECISION_FP64 (any output dtype); one lane per element pair evaluates every point pair
+ * (blocked entry: atomic adds of the local block; dof blocks are not read). The magnetic field
+ * operator, the fused and the symmetric passes take at most BEM_QUAD_FAST_POINTS points.
  * BEM_MAXWELL_MAGNETIC_FIELD (bempp_cl/core/numba_kernels.py:2151-2277) takes the 6-point rule
  * and always adds to `out` through both entry points; in MIXED mode the blocked entry point
  * runs its far pairs through the LDS-tiled kernel.
@@ -382,6 +392,10 @@ int bem_dense_add_transpose(void *A, int dtype, int64_t n, int64_t ld, void *str
 /* A <- 0 for a square n x n matrix (leading dimension ld), with a fill kernel (no memset node,
  * so the call can be captured in a HIP graph). */
 int bem_dense_zero(void *A, int dtype, int64_t n, int64_t ld, void *stream);
+
+/* Copy the upper triangle of a square n x n matrix into its lower triangle and zero the diagonal
+ * (the second step of bem_regular_assemble_symmetric; no reference counterpart). */
+int bem_dense_mirror_upper(void *A, int dtype, int64_t n, int64_t ld, void *stream);
 
 /* Set `bytes` bytes at `ptr` to zero with a fill kernel (safe inside graph capture). */
 int bem_fill_zero(void *ptr, size_t bytes, void *stream);

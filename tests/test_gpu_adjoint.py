@@ -246,3 +246,31 @@ def test_autograd_loss(precision, default_parameters, hip_available):
         lm = float(loss(prob64, torch.from_numpy(_vtx(grid)), torch.from_numpy(c0 - hc * d)))
         fdc = (lp - lm) / (2 * hc)
         assert abs(np.real(np.conj(gc) @ d) - fdc) < tol * abs(fdc)
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_moving_vertices_do_not_accumulate_device_memory(hip_available, precision):
+    """A shape-optimisation loop (new vertex positions every step) keeps device memory flat:
+    the problem releases the previous grid's and space's cached device copies."""
+    import torch
+
+    import bempp_amd.api as bempp
+    from bempp_amd.api.differentiable import SingleLayerProblem, single_layer_apply, single_layer_potential
+
+    grid = bempp.shapes.icosphere(2)
+    prob = SingleLayerProblem(grid.elements, 2.0, precision=precision)
+    rng = np.random.default_rng(9)
+    F = grid.number_of_elements
+    c = torch.from_numpy(_complex(rng, F)).cuda()
+    pts = torch.from_numpy(3.0 * rng.standard_normal((64, 3))).cuda()
+    v0 = np.ascontiguousarray(grid.vertices.T)
+    usage = []
+    for step in range(5):
+        v = torch.from_numpy(v0 * (1.0 + 0.01 * step)).cuda().requires_grad_(True)
+        loss = single_layer_apply(v, c, prob).abs().pow(2).sum() + single_layer_potential(v, c, pts, prob).abs().pow(2).sum()
+        loss.backward()
+        del v, loss
+        torch.cuda.synchronize()
+        usage.append(torch.cuda.memory_allocated())
+    # the first step builds the caches; later steps replace them instead of adding to them
+    assert max(usage[1:]) <= usage[1], usage

@@ -66,3 +66,27 @@ def test_autotune_candidates_fit_the_abi():
     assert set(autotune.CANDIDATE_TILES) <= {8, 16, 32}
     assert all(1 <= w <= 16 for w in autotune.CANDIDATE_WG_PER_CU)
     assert autotune.DEFAULT_CONFIG[0] in autotune.CANDIDATE_TILES
+
+
+@pytest.mark.parametrize("entry", [
+    {"config": [12, 4]},          # tile outside the candidates
+    {"config": [16, 3]},          # workgroups per CU outside the candidates
+    {"config": [16]},             # wrong length
+    {"config": "16,4"},           # wrong type
+    {"config": [16.5, 4]},        # not an integer
+    {"config": [True, 4]},        # bool is not a tile size
+    {"ms": 1.0},                  # no config
+    [16, 4],                      # not a mapping
+])
+def test_autotune_treats_invalid_entries_as_a_miss(tmp_path, monkeypatch, entry):
+    from bempp_amd.core import autotune
+
+    cache = tmp_path / "tune.json"
+    monkeypatch.setenv("BEMPP_AMD_TUNE_CACHE", str(cache))
+    autotune.clear_memory()
+    key = autotune.make_key("helmholtz_single_layer", "p0", "p0", 320, 320, 4, "single", "dev")
+    cache.write_text(json.dumps({key: entry}))
+    assert autotune.lookup(key) is None
+    cache.write_text(json.dumps({key: {"config": [8, 2]}}))
+    assert autotune.lookup(key) == (8, 2)
+    autotune.clear_memory()
