@@ -10,13 +10,31 @@ same low-order symmetric rules from their closed forms instead of a table:
 * order 4: 6-point rule of Strang and Fix / Dunavant (two 3-point orbits);
 * order 5: 7-point Radon rule (centroid plus two 3-point orbits, closed form in sqrt(15)).
 
+Orders 6 to 20 (12 to 79 points) have no closed form; they are served from
+triangle_rules.npz, the reference's table of symmetric rules kept as data (same points,
+same order, triangle_gauss.py:143-173, 175-2925), written by make_triangle_table.py.
+
 The point order inside each rule follows the reference (triangle_gauss.py:2935-2939:
 points are the 2nd and 3rd barycentric coordinate of each table row), so that sums
 over quadrature points run in the same order. Weights sum to 1/2, the area of the
 reference triangle.
 """
 
+import os as _os
+
 import numpy as _np
+
+_TABLE_PATH = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "triangle_rules.npz")
+_TABLE = {}
+HIGHEST_ORDER = 20
+
+
+def _table_rule(order):
+    if order not in _TABLE:
+        with _np.load(_TABLE_PATH) as t:
+            _TABLE[order] = (_np.asfortranarray(t[f"p{order}"]), _np.array(t[f"w{order}"]))
+    p, w = _TABLE[order]
+    return p.copy(order="F"), w.copy()
 
 # Orbit parameters of the order-4 rule (Dunavant 1985, degree 4, 6 points), given to
 # 20 significant digits; exactness for all monomials of degree <= 4 is checked in tests.
@@ -60,12 +78,10 @@ def _barycentric_rule(order):
         pts = [(1.0 / 3, 1.0 / 3, 1.0 / 3), o1[0], o2[0], o1[1], o1[2], o2[1], o2[2]]
         wts = [9.0 / 40, w2, w1, w2, w2, w1, w1]
         return pts, wts
-    raise ValueError(
-        f"Symmetric triangle Gauss rules are available for orders 1 to 5 in this library. Provided: {order}"
-    )
+    raise ValueError(f"Symmetric Gauss quadrature order must be between 1 and 20. Provided: {order}")
 
 
-SUPPORTED_ORDERS = (1, 2, 3, 4, 5)
+SUPPORTED_ORDERS = tuple(range(1, HIGHEST_ORDER + 1))
 
 
 def rule(order):
@@ -74,7 +90,12 @@ def rule(order):
     Points are reference coordinates (xi, eta) = 2nd and 3rd barycentric coordinate,
     weights sum to 1/2 (triangle_gauss.py:2928-2940).
     """
-    bary, wts = _barycentric_rule(int(order))
+    order = int(order)
+    if order < 1 or order > HIGHEST_ORDER:
+        raise ValueError(f"Symmetric Gauss quadrature order must be between 1 and 20. Provided: {order}")
+    if order > 5:
+        return _table_rule(order)
+    bary, wts = _barycentric_rule(order)
     bary = _np.array(bary, dtype=_np.float64)
     points = _np.asfortranarray(_np.vstack([bary[:, 1], bary[:, 2]]))
     return points, 0.5 * _np.array(wts, dtype=_np.float64)
@@ -82,4 +103,4 @@ def rule(order):
 
 def get_number_of_quad_points(order):
     """Number of points of the rule of the given order (triangle_gauss.py:2943-2945)."""
-    return len(_barycentric_rule(int(order))[1])
+    return len(rule(order)[1])

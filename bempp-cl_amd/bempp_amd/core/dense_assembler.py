@@ -108,6 +108,15 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     points, weights = triangle_rule(parameters.quadrature.regular)
     rule = hb.quad_rule(points, weights)
     mode = hb.PRECISION_MIXED if precision == "single" else hb.PRECISION_FP64
+    # rules of more than 16 points (regular orders 7-20) take the library's one-lane-per-pair FP64
+    # kernel in both precisions (stored in the precision's dtype): no near-field split, no
+    # symmetric traversal
+    wide = len(weights) > hb.QUAD_FAST_POINTS
+    if wide:
+        if op == "maxwell_magnetic_field":
+            raise ValueError(f"the magnetic field operator takes regular rules of at most {hb.QUAD_FAST_POINTS} "
+                             f"points (order <= 8); got order {parameters.quadrature.regular}")
+        mode = hb.PRECISION_FP64
     direct = _is_localised_injective(domain) and _is_localised_injective(dual_to_range)
     full_cover = (
         direct
@@ -125,7 +134,8 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     sym_opt = getattr(hip_opts, "symmetric", True) if hip_opts is not None else True
     # a single layer or hypersingular operator on one P1 space: every unordered pair once; the
     # library zeroes the output and adds the transpose itself
-    sym_sl = not accumulate and sym_opt and op in _SYMMETRIC_P1_OPERATORS and use_symmetric_pass(domain, dual_to_range)
+    sym_sl = (not accumulate and not wide and sym_opt and op in _SYMMETRIC_P1_OPERATORS
+              and use_symmetric_pass(domain, dual_to_range))
     if not accumulate and not full_cover and not sym_sl:
         hb.fill_zero(out)
     near = getattr(hip_opts, "mixed_near_field_factor", DEFAULT_NEAR_FIELD_FACTOR) if hip_opts is not None else DEFAULT_NEAR_FIELD_FACTOR
@@ -137,6 +147,7 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     hip_opts_ = getattr(parameters, "hip", None)
     upper = (
         direct
+        and not wide
         and not accumulate
         and use_upper_triangle(domain, dual_to_range, op)
         and (getattr(hip_opts_, "symmetric", True) if hip_opts_ is not None else True)
@@ -265,6 +276,12 @@ def hip_dense_assemble_sl_dl(domain, dual_to_range, parameters, sl_descriptor, d
     test = hb.device_space(dual_to_range)
     trial = hb.device_space(domain)
     points, weights = triangle_rule(parameters.quadrature.regular)
+    if len(weights) > hb.QUAD_FAST_POINTS:
+        # rules of more than 16 points have no fused pass: one assembly per operator
+        results = []
+        for desc, o in ((sl_descriptor, out_single), (dl_descriptor, out_double)):
+            results.append(hip_dense_assemble(domain, dual_to_range, parameters, desc, out=o, accumulate=False))
+        return results[0], results[1]
     rule = hb.quad_rule(points, weights)
     mode = hb.PRECISION_MIXED if precision == "single" else hb.PRECISION_FP64
     direct = _is_localised_injective(domain) and _is_localised_injective(dual_to_range)

@@ -44,6 +44,24 @@ inline SpaceArgs to_args(const bem_space_t *s) {
     return a;
 }
 
+// Rules longer than the fast kernels' 16 points (regular_wide.hip, potential prep): kernel
+// arguments, read with uniform indices (scalar loads).
+struct RuleArgsWide {
+    int n;
+    double x[BEM_QUAD_MAX_POINTS], y[BEM_QUAD_MAX_POINTS], w[BEM_QUAD_MAX_POINTS];
+};
+
+inline RuleArgsWide to_args_wide(const bem_quad_rule_t *r) {
+    RuleArgsWide a;
+    a.n = r->n_points;
+    for (int i = 0; i < BEM_QUAD_MAX_POINTS; ++i) {
+        a.x[i] = i < r->n_points ? r->x[i] : 0.0;
+        a.y[i] = i < r->n_points ? r->y[i] : 0.0;
+        a.w[i] = i < r->n_points ? r->w[i] : 0.0;
+    }
+    return a;
+}
+
 inline RuleArgs to_args(const bem_quad_rule_t *r) {
     RuleArgs a;
     a.n = r->n_points;
@@ -62,6 +80,20 @@ int check_space(const bem_space_t *s, const char *which);
 
 // Map a HIP error to BEM_ERR_DEVICE with message.
 int check_launch(const char *what);
+
+// Regular pairs with a rule of more than BEM_QUAD_FAST_POINTS points, FP64 (regular_wide.hip):
+// one lane per (test, trial) element pair. atomic: add the local blocks with atomics (spaces with
+// shared dofs); otherwise every entry of a pair is stored (accumulate = 0) or added (accumulate = 1).
+struct RegularWideArgs {
+    SpaceArgs test, trial;
+    RuleArgsWide rule;
+    double kr, ki;
+    void *out;
+    int64_t ld;
+    int identical;
+    int accumulate;
+};
+int regular_wide_launch(int op, const RegularWideArgs &a, bool atomic, int out_dtype, hipStream_t s);
 
 // Square n x n matrix (leading dimension ld, dtype BEM_F32/F64/C64/C128) on stream s:
 // what = 0 sets it to zero, what = 1 replaces A by A + A^T, what = 2 copies the upper triangle

@@ -523,6 +523,11 @@ extern "C" int bem_dense_zero(void *A, int dtype, int64_t n, int64_t ld, void *s
     return square_matrix_op(0, A, dtype, n, ld, (hipStream_t)stream);
 }
 
+extern "C" int bem_dense_mirror_upper(void *A, int dtype, int64_t n, int64_t ld, void *stream) {
+    clear_error();
+    return square_matrix_op(2, A, dtype, n, ld, (hipStream_t)stream);
+}
+
 extern "C" int bem_dense_matvec(int trans, const void *A, int a_dtype, int64_t rows, int64_t cols, int64_t ld,
                                 const void *x, void *y, void *stream) {
     clear_error();

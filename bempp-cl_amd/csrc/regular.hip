@@ -16,12 +16,23 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
     }
     int rc;
     if ((rc = check_space(test, "test")) || (rc = check_space(trial, "trial"))) return rc;
-    if (rule->n_points < 1 || rule->n_points > 16) {
-        set_error("quadrature rule must have 1..16 points, got %d", rule->n_points);
+    if (rule->n_points < 1 || rule->n_points > BEM_QUAD_MAX_POINTS) {
+        set_error("quadrature rule must have 1..%d points, got %d", BEM_QUAD_MAX_POINTS, rule->n_points);
         return BEM_ERR_INVALID_ARGUMENT;
     }
     if (precision != BEM_PRECISION_FP64 && precision != BEM_PRECISION_MIXED) {
         set_error("unknown precision mode %d", precision);
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    const bool wide = rule->n_points > BEM_QUAD_FAST_POINTS;
+    if (wide && (out2 || work || upper || op == BEM_MAXWELL_MAGNETIC_FIELD)) {
+        set_error("rules of more than %d points are served by bem_regular_assemble(_blocked) for operators 0-8 only",
+                  BEM_QUAD_FAST_POINTS);
+        return BEM_ERR_UNSUPPORTED;
+    }
+    if (wide && precision != BEM_PRECISION_FP64) {
+        set_error("rules of more than %d points run in FP64 (BEM_PRECISION_FP64, any output dtype)",
+                  BEM_QUAD_FAST_POINTS);
         return BEM_ERR_INVALID_ARGUMENT;
     }
     const bool maxwell = op == BEM_MAXWELL_ELECTRIC_FIELD || op == BEM_MAXWELL_MAGNETIC_FIELD;
@@ -38,6 +49,19 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
         return BEM_ERR_UNSUPPORTED;
     }
     const bool tile = tblocks != nullptr;
+    if (wide) {
+        RegularWideArgs w;
+        w.test = to_args(test);
+        w.trial = to_args(trial);
+        w.rule = to_args_wide(rule);
+        w.kr = k_re;
+        w.ki = k_im;
+        w.out = out;
+        w.ld = ld;
+        w.identical = grids_identical;
+        w.accumulate = accumulate;
+        return regular_wide_launch(op, w, tile, out_dtype, (hipStream_t)stream);
+    }
     if (op == BEM_MAXWELL_MAGNETIC_FIELD && !(tile && precision == BEM_PRECISION_MIXED)) {
         // adds to the output: FP64, or without dof blocks the untiled mixed kernel (far pairs)
         return mfie_regular_launch(to_args(test), to_args(trial), to_args(rule), k_re, k_im, grids_identical,

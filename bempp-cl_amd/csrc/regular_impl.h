@@ -154,9 +154,9 @@ __device__ __forceinline__ void green3(double dx, double dy, double dz, double r
 
 // FP64 pair from the geometry table (any rule). DUAL: S2 receives the double layer; SYM: S3
 // receives the double layer of the reversed pair (indexed like S: test side first).
-template <int OP, int NTS, int NSS, int QT, bool DUAL = false, bool SYM = false>
+template <int OP, int NTS, int NSS, int QT, bool DUAL = false, bool SYM = false, typename R = RuleArgs>
 __device__ __forceinline__ void pair_fp64_geom(const double *__restrict__ gt, const double *__restrict__ gf, double tnm,
-                                               double snm, const RuleArgs &rule, double kr, double ki,
+                                               double snm, const R &rule, double kr, double ki,
                                                C2d (&S)[NTS][NSS], C2d (&S2)[NTS][NSS], C2d (&S3)[NTS][NSS]) {
     const int nq = QT > 0 ? QT : rule.n;
     double nx[3] = {gt[9] * tnm, gt[10] * tnm, gt[11] * tnm};

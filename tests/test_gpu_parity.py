@@ -338,3 +338,69 @@ def test_efie_potential_far_mixed(k, density, default_parameters, hip_available)
     u64 = mpot.electric_field(rwg, pts, k, parameters=default_parameters, precision="double").evaluate(fun)
     u32 = mpot.electric_field(rwg, pts, k, parameters=default_parameters, precision="single").evaluate(fun)
     assert np.linalg.norm(u32 - u64) / np.linalg.norm(u64) < 1e-6
+
+
+# Regular orders beyond 5 (the reference's rules of orders 1-20, 12..79 points): order 6 has 12
+# points (the tiled kernels' runtime-Q path), orders 10 and 20 have 25 and 79 (the one-lane-per-pair
+# FP64 kernel, direct stores for DP0/DP1 and atomic adds for P1/RWG).
+HIGH_ORDER_CASES = [
+    ("laplace", "single_layer", ("DP", 0), ("DP", 0), None),
+    ("helmholtz", "double_layer", ("P", 1), ("P", 1), 3.0 + 0.5j),
+    ("helmholtz", "adjoint_double_layer", ("DP", 1), ("P", 1), 2.0),
+    ("helmholtz", "hypersingular", ("P", 1), ("P", 1), 2.5),
+    ("maxwell", "electric_field", ("RWG", 0), ("SNC", 0), 2.0),
+]
+
+
+@pytest.mark.parametrize("order", [6, 10, 20])
+@pytest.mark.parametrize("case", HIGH_ORDER_CASES,
+                         ids=[f"{c[0]}_{c[1]}_{c[2][0]}{c[2][1]}_{c[3][0]}{c[3][1]}" for c in HIGH_ORDER_CASES])
+def test_high_regular_orders_vs_oracle(case, order, hip_available):
+    import bempp_amd.api.operators.boundary as B
+    from bempp_amd.api import function_space
+    from bempp_amd.api.grid.shapes import icosphere
+    from bempp_amd.api.utils.parameters import DefaultParameters
+    from oracle import oracle as O
+
+    mod, fn, dk, tk, k = case
+    grid = icosphere(1)
+    params = DefaultParameters()
+    params.quadrature.regular = order
+    params.quadrature.singular = 4
+    dom = function_space(grid, *dk)
+    dual = function_space(grid, *tk)
+    factory = getattr(getattr(B, mod), fn)
+    rng = dom if mod == "maxwell" else dom
+    args = (dom, rng, dual) if k is None else (dom, rng, dual, k)
+    print(f"high order case {mod}_{fn} order {order}", flush=True)
+    A64 = factory(*args, parameters=params, precision="double").weak_form().to_dense()
+    op_name = f"{mod}_{fn}"
+    ref = O.assemble(op_name, grid.vertices, grid.elements, dom, dual, 0.0 if k is None else k, regular_order=order)
+    assert rel_fro(A64, ref) < 1e-12
+    A32 = factory(*args, parameters=params, precision="single").weak_form().to_dense()
+    assert rel_fro(A32.astype(A64.dtype), A64) < 1e-6
+
+
+@pytest.mark.parametrize("order", [10, 20])
+def test_high_order_potentials_vs_oracle(order, hip_available):
+    from bempp_amd.api import GridFunction, function_space
+    from bempp_amd.api.grid.shapes import icosphere
+    from bempp_amd.api.operators.potential import helmholtz as hpot
+    from bempp_amd.api.operators.potential import maxwell as mpot
+    from bempp_amd.api.utils.parameters import DefaultParameters
+    from oracle import oracle as O
+
+    grid = icosphere(1)
+    params = DefaultParameters()
+    params.quadrature.regular = order
+    rng = np.random.default_rng(4)
+    pts = rng.standard_normal((3, 50))
+    pts = 2.5 * pts / np.linalg.norm(pts, axis=0)
+    p1 = function_space(grid, "P", 1)
+    rwg = function_space(grid, "RWG", 0)
+    for space, factory, name, k in ((p1, hpot.double_layer, "helmholtz_double_layer", 2.0),
+                                    (rwg, mpot.electric_field, "maxwell_electric_field", 1.5)):
+        c = rng.standard_normal(space.global_dof_count) + 1j * rng.standard_normal(space.global_dof_count)
+        u = factory(space, pts, k, parameters=params, precision="double").evaluate(GridFunction(space, coefficients=c))
+        ref = O.potential(name, grid.vertices, grid.elements, space, c, pts, k, order=order)
+        assert rel_fro(u, ref) < 1e-12, name
