@@ -42,7 +42,7 @@ enum { SHAPE_P0 = 0, SHAPE_P1 = 1, SHAPE_RWG = 2 };
 
 #define M_INV_4PI_ 0.079577471545947667884
 #define PI_ 3.14159265358979323846
-#define MAXQ 64
+#define MAXQ 80
 
 typedef struct { double re, im; } cplx;
 
@@ -187,7 +187,18 @@ static inline cplx kernel_eval(int op, const double *x, const double *y, const d
 }
 
 /* ------------------------------------------------------------------ quadrature */
+/* Rules of orders 6-20 are data, not closed forms (the reference's HyENA table,
+ * bempp_cl/api/integration/triangle_gauss.py:175-2925): the caller passes them through
+ * bem_oracle_set_regular_rule and selects them with order 0. */
+static int g_user_n = 0;
+static double g_user_x[MAXQ], g_user_y[MAXQ], g_user_w[MAXQ];
+
 static int triangle_rule(int order, double *px, double *py, double *w) {
+    if (order == 0) {
+        if (g_user_n < 1) return -1;
+        for (int i = 0; i < g_user_n; ++i) { px[i] = g_user_x[i]; py[i] = g_user_y[i]; w[i] = g_user_w[i]; }
+        return g_user_n;
+    }
     double b[8][3], wt[8];
     int n = 0;
 #define S21(a, ww) do { double a_ = (a); double c_ = 1.0 - 2.0 * a_; \
@@ -486,14 +497,25 @@ typedef struct {
 
 static int nshape_of(int shape) { return shape == SHAPE_P0 ? 1 : 3; }
 
+/* Optional row selection (bem_oracle_set_row_map): global test dof -> row of `out`, or -1 to
+ * drop the entry, so that sampled rows of a large operator need only their own storage. */
+static const int32_t *g_row_map = NULL;
+
 static void scatter_block(double *out, int cols, int is_complex, const Spaces *S, const Elem *te, const Elem *se, int e, int f, cplx loc[3][3]) {
     for (int i = 0; i < S->nt; ++i) for (int j = 0; j < S->ns; ++j) {
+        int row = S->tl2g[e * S->nt + i];
+        if (g_row_map) {
+            row = g_row_map[row];
+            if (row < 0) continue;
+        }
         double m = S->tmult[e * S->nt + i] * S->smult[f * S->ns + j];
         if (S->op == OP_MAXWELL_EFIE || S->op == OP_MAXWELL_MFIE) m *= te->len[i] * se->len[j];
         cplx v = {loc[i][j].re * m, loc[i][j].im * m};
-        accumulate(out, cols, is_complex, S->tl2g[e * S->nt + i], S->sl2g[f * S->ns + j], v);
+        accumulate(out, cols, is_complex, row, S->sl2g[f * S->ns + j], v);
     }
 }
+
+void bem_oracle_set_row_map(const int32_t *map) { g_row_map = map; }
 
 /*
  * Dense assembly of one operator on one grid (test and trial spaces on the same grid).
@@ -714,6 +736,14 @@ int bem_oracle_potential(int op, int nv, const double *vtx, int nf, const int32_
 
 /* 1-D Gauss nodes/weights, for tests of the rule generator */
 void bem_oracle_gauss01(int n, double *x, double *w) { gauss01(n, x, w); }
+
+/* Regular rule used by order 0 (points on the reference triangle, weights summing to 1/2). */
+int bem_oracle_set_regular_rule(int n, const double *x, const double *y, const double *w) {
+    if (n < 1 || n > MAXQ) return -1;
+    for (int i = 0; i < n; ++i) { g_user_x[i] = x[i]; g_user_y[i] = y[i]; g_user_w[i] = w[i]; }
+    g_user_n = n;
+    return 0;
+}
 
 /* Duffy rule points for tests: returns number of points; arrays must hold 6 n^4 */
 int bem_oracle_duffy(int order, int kind, double *tx, double *ty, double *sx, double *sy, double *w) {

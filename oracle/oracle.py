@@ -57,6 +57,10 @@ def lib():
         L.bem_oracle_potential.argtypes = [i, i, P, i, P, i, P, P, P, i, P, i, d, d, P, i]
         L.bem_oracle_potential.restype = i
         L.bem_oracle_gauss01.argtypes = [i, P, P]
+        L.bem_oracle_set_regular_rule.argtypes = [i, P, P, P]
+        L.bem_oracle_set_regular_rule.restype = i
+        L.bem_oracle_set_row_map.argtypes = [P]
+        L.bem_oracle_set_row_map.restype = None
         L.bem_oracle_duffy.argtypes = [i, i, P, P, P, P, P]
         L.bem_oracle_duffy.restype = i
         _LIB = L
@@ -96,9 +100,31 @@ def _mesh(vertices, elements):
     return v, e
 
 
+_TABLE = os.path.join(os.path.dirname(_HERE), "bempp-cl_amd", "bempp_amd", "api", "integration", "triangle_rules.npz")
+
+
+def _regular_order(order):
+    """Orders 1-5 are the oracle's own closed forms; orders 6-20 come from the rule table (data
+    kept alike with the reference's), passed to the C library and selected as order 0."""
+    order = int(order)
+    if order <= 5:
+        return order
+    with np.load(_TABLE) as t:
+        p = np.ascontiguousarray(t[f"p{order}"])
+        w = np.ascontiguousarray(t[f"w{order}"])
+    x, y = np.ascontiguousarray(p[0]), np.ascontiguousarray(p[1])
+    if lib().bem_oracle_set_regular_rule(len(w), _ptr(x), _ptr(y), _ptr(w)) != 0:
+        raise ValueError(f"rule of order {order} does not fit the oracle")
+    return 0
+
+
 def assemble(op, vertices, elements, domain, dual_to_range, wavenumber=0.0, regular_order=4, singular_order=4,
-             parts=3, test_range=None, nthreads=0):
-    """FP64 dense matrix (rows = dual_to_range dofs, cols = domain dofs) of the operator `op`."""
+             parts=3, test_range=None, nthreads=0, rows_subset=None):
+    """FP64 dense matrix (rows = dual_to_range dofs, cols = domain dofs) of the operator `op`.
+
+    rows_subset: optional test dof indices; the result then holds only those rows, in that order
+    (combine with test_range to skip the work of the other test elements)."""
+    regular_order = _regular_order(regular_order)
     v, e = _mesh(vertices, elements)
     if not isinstance(domain, SpaceArrays):
         domain = SpaceArrays.from_space(domain)
@@ -106,17 +132,22 @@ def assemble(op, vertices, elements, domain, dual_to_range, wavenumber=0.0, regu
         dual_to_range = SpaceArrays.from_space(dual_to_range)
     is_complex = op.startswith("helmholtz") or op.startswith("maxwell")
     rows, cols = dual_to_range.global_dof_count, domain.global_dof_count
-    out = np.zeros((rows, cols, 2) if is_complex else (rows, cols), dtype=np.float64)
+    row_map = None
+    if rows_subset is not None:
+        row_map = np.full(rows, -1, dtype=np.int32)
+        row_map[np.asarray(rows_subset, dtype=np.int64)] = np.arange(len(rows_subset), dtype=np.int32)
+        out_rows = len(rows_subset)
+    else:
+        out_rows = rows
+    out = np.zeros((out_rows, cols, 2) if is_complex else (out_rows, cols), dtype=np.float64)
     k = complex(wavenumber)
     t0, t1 = (0, -1) if test_range is None else test_range
-    rc = lib().bem_oracle_assemble(
-        OPS[op], v.shape[0], _ptr(v), e.shape[0], _ptr(e),
-        SHAPES[dual_to_range.shapeset], _ptr(dual_to_range.local2global), _ptr(dual_to_range.local_multipliers),
-        _ptr(dual_to_range.normal_multipliers), _ptr(dual_to_range.support),
-        SHAPES[domain.shapeset], _ptr(domain.local2global), _ptr(domain.local_multipliers),
-        _ptr(domain.normal_multipliers), _ptr(domain.support),
-        regular_order, singular_order, k.real, k.imag, rows, cols, int(is_complex), _ptr(out), parts, t0, t1, nthreads,
-    )
+    lib().bem_oracle_set_row_map(None if row_map is None else _ptr(row_map))
+    try:
+        rc = _assemble_call(op, v, e, dual_to_range, domain, regular_order, singular_order, k, rows, cols, is_complex,
+                            out, parts, t0, t1, nthreads)
+    finally:
+        lib().bem_oracle_set_row_map(None)
     if rc != 0:
         raise ValueError(f"oracle assembly failed with code {rc}")
     if is_complex:
@@ -124,8 +155,21 @@ def assemble(op, vertices, elements, domain, dual_to_range, wavenumber=0.0, regu
     return out
 
 
+def _assemble_call(op, v, e, dual_to_range, domain, regular_order, singular_order, k, rows, cols, is_complex, out, parts,
+                   t0, t1, nthreads):
+    return lib().bem_oracle_assemble(
+        OPS[op], v.shape[0], _ptr(v), e.shape[0], _ptr(e),
+        SHAPES[dual_to_range.shapeset], _ptr(dual_to_range.local2global), _ptr(dual_to_range.local_multipliers),
+        _ptr(dual_to_range.normal_multipliers), _ptr(dual_to_range.support),
+        SHAPES[domain.shapeset], _ptr(domain.local2global), _ptr(domain.local_multipliers),
+        _ptr(domain.normal_multipliers), _ptr(domain.support),
+        regular_order, singular_order, k.real, k.imag, rows, cols, int(is_complex), _ptr(out), parts, t0, t1, nthreads,
+    )
+
+
 def potential(op, vertices, elements, space, coefficients, points, wavenumber=0.0, order=4, nthreads=0):
     """FP64 potential (kdim, M) of a density given by its space coefficients (dof vector)."""
+    order = _regular_order(order)
     v, e = _mesh(vertices, elements)
     if not isinstance(space, SpaceArrays):
         space = SpaceArrays.from_space(space)
