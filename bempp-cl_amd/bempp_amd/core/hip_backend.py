@@ -124,6 +124,8 @@ def lib():
         L.bem_potential_workspace_size.restype = sz
         L.bem_potential_evaluate.argtypes = [i, SP, P, i, P, QP, d, d, i, P, i, P, sz, P]
         L.bem_potential_evaluate.restype = i
+        L.bem_potential_evaluate_noalloc.argtypes = [i, SP, P, i, P, QP, d, d, i, P, i, P]
+        L.bem_potential_evaluate_noalloc.restype = i
         L.bem_fill_zero.argtypes = [P, sz, P]
         L.bem_fill_zero.restype = i
         L.bem_last_error.restype = ctypes.c_char_p

@@ -21,6 +21,8 @@
 #include "bem_device.h"
 #include "bem_internal.h"
 
+#include <mutex>
+
 namespace bem {
 
 constexpr int kPotThreads = 256;
@@ -851,4 +853,67 @@ extern "C" int bem_potential_evaluate(int op, const bem_space_t *space, const vo
     clear_error();
     return potential_evaluate(op, space, coefficients, n_points, points, rule, k_re, k_im, precision, out, out_dtype,
                               workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------ allocation-free variant
+// bem_potential_evaluate_noalloc: the workspace of bem_potential_evaluate lives in this
+// translation unit's device array (one copy per GPU, held until the process ends). Its device
+// address is looked up once per device with hipGetSymbolAddress; host code never passes the
+// array's own name to a kernel (on the host that name is a stand-in, not a device address).
+// Nothing in the workspace needs to start at zero: the prep kernel writes every source slot
+// (padding included) and the eval kernels write every (chunk, target) partial. Calls that
+// overlap on one GPU share the array and race.
+namespace {
+
+static __device__ __align__(256) unsigned char scratch[BEM_POTENTIAL_SCRATCH_BYTES];
+
+constexpr int kMaxScratchDevices = 64;
+void *g_scratch_ptr[kMaxScratchDevices];
+std::mutex g_scratch_mutex;
+
+// device address of `scratch` on the current device (looked up on the first call per device)
+int scratch_pointer(void **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= kMaxScratchDevices) {
+        set_error("no usable current device for the potential scratch buffer (%s)", hipGetErrorString(e));
+        return BEM_ERR_DEVICE;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+            return BEM_ERR_DEVICE;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return BEM_OK;
+}
+
+}  // namespace
+
+extern "C" int bem_potential_evaluate_noalloc(int op, const bem_space_t *space, const void *coefficients,
+                                              int n_points, const double *points, const bem_quad_rule_t *rule,
+                                              double k_re, double k_im, int precision, void *out, int out_dtype,
+                                              void *stream) {
+    clear_error();
+    if (!space || !rule) {
+        set_error("null argument");
+        return BEM_ERR_INVALID_ARGUMENT;
+    }
+    size_t need = bem_potential_workspace_size(op, space, rule, n_points, precision);
+    if (need > (size_t)BEM_POTENTIAL_SCRATCH_BYTES) {
+        set_error("potential needs %zu bytes of workspace, more than the %zu-byte scratch buffer of "
+                  "bem_potential_evaluate_noalloc; use bem_potential_evaluate with a caller workspace",
+                  need, (size_t)BEM_POTENTIAL_SCRATCH_BYTES);
+        return BEM_ERR_WORKSPACE;
+    }
+    void *ws = nullptr;
+    int rc = scratch_pointer(&ws);
+    if (rc) return rc;
+    return potential_evaluate(op, space, coefficients, n_points, points, rule, k_re, k_im, precision, out, out_dtype,
+                              ws, (size_t)BEM_POTENTIAL_SCRATCH_BYTES, (hipStream_t)stream);
 }

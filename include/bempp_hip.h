@@ -285,6 +285,21 @@ int bem_potential_evaluate(int op, const bem_space_t *space, const void *coeffic
                            int precision, void *out, int out_dtype, void *workspace, size_t workspace_bytes,
                            void *stream);
 
+/*
+ * Allocation-free bem_potential_evaluate. No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Same arguments and results as bem_potential_evaluate without the
+ * workspace: it uses a library-owned static device buffer of BEM_POTENTIAL_SCRATCH_BYTES per
+ * GPU (held until the process ends) and calls no allocator. Returns BEM_ERR_WORKSPACE when
+ * bem_potential_workspace_size(...) exceeds that size (the level-5 P1 potential at 16,384
+ * points needs 29.5 MB, the level-4 RWG potential at 65,536 points 50.6 MB). All calls on one
+ * GPU share the buffer: calls that overlap on one GPU (other streams, other threads) race.
+ */
+#define BEM_POTENTIAL_SCRATCH_BYTES ((size_t)64 << 20)
+int bem_potential_evaluate_noalloc(int op, const bem_space_t *space, const void *coefficients, int n_points,
+                                   const double *points, const bem_quad_rule_t *rule, double k_re, double k_im,
+                                   int precision, void *out, int out_dtype, void *stream);
+
 /* Launch shape of bem_kbatch_assemble, picked by the autotuner (bempp_amd/core/autotune.py). */
 typedef struct bem_kbatch_config {
     int32_t test_tile;          /* test elements staged in LDS per tile: 8, 16 or 32 */

@@ -73,3 +73,23 @@ def test_error_path_without_gpu():
     assert rc == 1  # ld < n_dofs
     rc = L.bem_dense_add_transpose(ctypes.c_void_p(16), 2, 8, 4, None)
     assert rc == 1
+
+
+def test_noalloc_potential_refuses_oversize_without_gpu():
+    """The allocation-free potential refuses a workspace beyond its static buffer on the host,
+    before it touches a device (only n_elements and the rule's size enter the check)."""
+    from bempp_amd.core import hip_backend
+
+    L = hip_backend.lib()
+    rule = hip_backend.BemQuadRule()
+    rule.n_points = 6
+    sp = hip_backend.BemSpace()
+    sp.n_elements = 1 << 22
+    n_points = 1 << 20
+    need = L.bem_potential_workspace_size(3, ctypes.byref(sp), ctypes.byref(rule), n_points, 1)
+    assert need > (64 << 20)
+    dummy = ctypes.c_void_p(256)
+    rc = L.bem_potential_evaluate_noalloc(3, ctypes.byref(sp), dummy, n_points, dummy, ctypes.byref(rule), 1.0,
+                                          0.0, 1, dummy, 2, None)
+    assert rc == 4
+    assert b"scratch" in L.bem_last_error()
