@@ -764,6 +764,13 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
     // zero and stands in for missing incidences, so the gather below needs no branches
     constexpr int SROW = kRegThreads + 1;
     __shared__ __align__(16) float stage[OWNER ? NT * NS * SROW * NC : 1];
+    // TILE: the trial block's dofs and, double-buffered, the current and the next test tile's dofs
+    // and (start, end, dof count). The tile loop reads them from LDS only: on gfx9 a wait for a
+    // global load also waits for every older global atomic, so a global load between two tile
+    // flushes would stall the wave until the previous flush's atomics have completed.
+    __shared__ int32_t sh_sdofs[TILE ? MAXU_S : 1];
+    __shared__ int32_t sh_tdofs[TILE ? 2 : 1][TILE ? MAXU_T : 1];
+    __shared__ int32_t sh_tmeta[TILE ? 2 : 1][3];
 
     const int tid = threadIdx.x;
     int block_id = blockIdx.x, tile_begin = blockIdx.y, tile_end = blockIdx.y + 1;
@@ -939,11 +946,39 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
         if (tid < NT * NC) stage[(tid / NC) * SROW * NC + kRegThreads * NC + tid % NC] = 0.f;  // zero column
     }
 
+    // load test tile `tl`'s dofs and (start, end, dof count) into LDS buffer `buf` (no atomics of this
+    // workgroup are outstanding when it runs: before the tile loop, or before a flush's atomics)
+    auto load_tile_meta = [&](int tl, int buf) {
+        if constexpr (TILE) {
+            const int nmax = min(a.tb.maxu, MAXU_T);
+            if (tid < nmax) sh_tdofs[buf][tid] = a.tb.dofs[(int64_t)tl * a.tb.maxu + tid];
+            if (tid == kRegThreads - 1) {
+                sh_tmeta[buf][0] = a.tb.start[tl];
+                sh_tmeta[buf][1] = a.tb.start[tl + 1];
+                sh_tmeta[buf][2] = min(max(a.tb.ndofs[tl], 0), nmax);
+            }
+        }
+    };
+    int ndt = 0, nds = 0;
+    if constexpr (TILE) {
+        nds = min(max(a.sb.ndofs[block_id], 0), min(a.sb.maxu, MAXU_S));
+        for (int s = tid; s < nds; s += kRegThreads) sh_sdofs[s] = a.sb.dofs[(int64_t)block_id * a.sb.maxu + s];
+        load_tile_meta(tile_begin, 0);  // published by the first tile's barrier
+    }
+
     for (int tile_id = tile_begin; tile_id < tile_end; ++tile_id) {
         int t0, t1;
+        const int buf = (tile_id - tile_begin) & 1;
         if constexpr (TILE) {
-            t0 = a.tb.start[tile_id];
-            t1 = a.tb.start[tile_id + 1];
+            if (tile_id == tile_begin) {  // the LDS copy is published by the barrier below
+                t0 = a.tb.start[tile_id];
+                t1 = a.tb.start[tile_id + 1];
+                ndt = min(max(a.tb.ndofs[tile_id], 0), min(a.tb.maxu, MAXU_T));
+            } else {  // loaded during the previous tile's flush, published by its closing barrier
+                t0 = __builtin_amdgcn_readfirstlane(sh_tmeta[buf][0]);
+                t1 = __builtin_amdgcn_readfirstlane(sh_tmeta[buf][1]);
+                ndt = __builtin_amdgcn_readfirstlane(sh_tmeta[buf][2]);
+            }
         } else {
             t0 = tile_id * TM;
             t1 = min(t0 + TM, a.test.n);
@@ -1145,11 +1180,13 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
 
         if constexpr (TILE) {
             __syncthreads();
-            const int ndt = min(a.tb.ndofs[tile_id], MAXU_T), nds = min(a.sb.ndofs[block_id], MAXU_S);
-            const int32_t *dt = a.tb.dofs + (int64_t)tile_id * a.tb.maxu;
-            const int32_t *ds = a.sb.dofs + (int64_t)block_id * a.sb.maxu;
+            // the next tile's dofs and bounds, before this flush issues its atomics
+            if (tile_id + 1 < tile_end) load_tile_meta(tile_id + 1, buf ^ 1);
+            const int32_t *dt = sh_tdofs[buf];
+            const int32_t *ds = sh_sdofs;
             // transposed: entry (test slot ia, trial slot is) goes to (ds[is], dt[ia]); consecutive
-            // lanes then walk one output row
+            // lanes then walk one output row. Only LDS reads and global atomics: nothing waits for
+            // the atomics, which complete while the next tile is computed.
             auto flush = [&](const AccT *tile, void *out, bool transposed) {
                 OutT *o = reinterpret_cast<OutT *>(out);
                 for (int idx = tid; idx < ndt * nds; idx += kRegThreads) {
