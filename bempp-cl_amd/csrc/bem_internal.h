@@ -29,7 +29,16 @@ struct SpaceArgs {
 struct RuleArgs {
     int n;
     double x[16], y[16], w[16];
+    // a 6-point rule made of two symmetric orbits, reordered to A0 A1 A2 B0 B1 B2 where point k of
+    // orbit o has barycentric coordinate a_o at index k and b_o at the other two:
+    // orb = (b_A, a_A - b_A, b_B, a_B - b_B) (set by orbit_rule; the kernels that use it check it)
+    float orb[4];
 };
+
+// Reorders a 6-point rule into the orbit order above when it has that structure (two orbits of
+// three points with equal weights, permutations of (a, b, b)); returns false and leaves it unchanged
+// otherwise. The order of the points does not change any sum beyond rounding.
+bool orbit_rule(RuleArgs &r);
 
 inline SpaceArgs to_args(const bem_space_t *s) {
     SpaceArgs a;
@@ -65,6 +74,7 @@ inline RuleArgsWide to_args_wide(const bem_quad_rule_t *r) {
 inline RuleArgs to_args(const bem_quad_rule_t *r) {
     RuleArgs a;
     a.n = r->n_points;
+    a.orb[0] = a.orb[1] = a.orb[2] = a.orb[3] = 0.f;
     for (int i = 0; i < 16; ++i) {
         a.x[i] = i < r->n_points ? r->x[i] : 0.0;
         a.y[i] = i < r->n_points ? r->y[i] : 0.0;

@@ -31,6 +31,62 @@ int check_space(const bem_space_t *s, const char *which) {
     return BEM_OK;
 }
 
+bool orbit_rule(RuleArgs &r) {
+    if (r.n != 6) return false;
+    double lam[6][3];
+    for (int q = 0; q < 6; ++q) {
+        lam[q][0] = 1.0 - r.x[q] - r.y[q];
+        lam[q][1] = r.x[q];
+        lam[q][2] = r.y[q];
+    }
+    auto close = [](double u, double v) { return fabs(u - v) <= 1e-12 * (fabs(u) + fabs(v) + 1e-300); };
+    // special index of each point: the one barycentric coordinate that differs from the other two
+    int special[6];
+    double av[6], bv[6];
+    for (int q = 0; q < 6; ++q) {
+        special[q] = -1;
+        for (int k = 0; k < 3; ++k) {
+            const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+            if (close(lam[q][k1], lam[q][k2]) && !close(lam[q][k], lam[q][k1])) special[q] = k;
+        }
+        if (special[q] < 0) return false;
+        av[q] = lam[q][special[q]];
+        bv[q] = lam[q][(special[q] + 1) % 3];
+    }
+    // two orbits: same (a, b) and weight, special indices 0, 1, 2
+    int order[6], used[6] = {0, 0, 0, 0, 0, 0}, n = 0;
+    for (int o = 0; o < 2; ++o) {
+        int first = -1;
+        for (int q = 0; q < 6 && first < 0; ++q)
+            if (!used[q]) first = q;
+        if (first < 0) return false;
+        for (int k = 0; k < 3; ++k) {
+            int found = -1;
+            for (int q = 0; q < 6; ++q)
+                if (!used[q] && special[q] == k && close(av[q], av[first]) && close(bv[q], bv[first]) &&
+                    close(r.w[q], r.w[first]))
+                    found = q;
+            if (found < 0) return false;
+            used[found] = 1;
+            order[n++] = found;
+        }
+        r.orb[2 * o] = (float)bv[first];
+        r.orb[2 * o + 1] = (float)(av[first] - bv[first]);
+    }
+    double x[6], y[6], w[6];
+    for (int q = 0; q < 6; ++q) {
+        x[q] = r.x[order[q]];
+        y[q] = r.y[order[q]];
+        w[q] = r.w[order[q]];
+    }
+    for (int q = 0; q < 6; ++q) {
+        r.x[q] = x[q];
+        r.y[q] = y[q];
+        r.w[q] = w[q];
+    }
+    return true;
+}
+
 int check_launch(const char *what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {

@@ -248,10 +248,17 @@ __device__ __forceinline__ float cos_rev(float x) { return __builtin_amdgcn_cosf
 // DUAL (OP a single layer): S2 receives the double layer of the same family, sharing the
 // distance, 1/r and e^{ik delta} of every point pair. SYM (with DUAL): S3 receives the double
 // layer of the reversed pair (normal of the test element), from the same values again.
-template <int OP, int NTS, int NSS, int QT, bool KAHAN, bool DAMP, bool DUAL, bool SYM = false, typename TT>
+// ORB (6-point rule in the orbit order of orbit_rule, P1 trial side): the trial moments of every
+// point value G come from the two orbit sums S_A, S_B: sum_q G_q = S_A + S_B and
+//   sum_q G_q xi_q  = b_A S_A + b_B S_B + (a_A - b_A) G_A1 + (a_B - b_B) G_B1
+//   sum_q G_q eta_q = b_A S_A + b_B S_B + (a_A - b_A) G_A2 + (a_B - b_B) G_B2,
+// 11 operations per value and test point instead of 18.
+template <int OP, int NTS, int NSS, int QT, bool KAHAN, bool DAMP, bool DUAL, bool SYM = false, bool ORB = false,
+          typename TT>
 __device__ __forceinline__ void pair_fp32(const TT &T_, int t, const TrialF32<QT> &F, double d0, double d1,
                                           double d2, double dist2, const RuleArgs &rule, double kr, double ki,
                                           C2f (&S)[NTS][NSS], C2f (&S2)[NTS][NSS], C2f (&S3)[NTS][NSS]) {
+    static_assert(!ORB || (QT == 6 && NSS == 3), "the orbit moments need the 6-point rule and a P1 trial side");
     constexpr bool HELM = OP >= kHelmholtzSL;
     static_assert(!DUAL || OP == kLaplaceSL || OP == kHelmholtzSL, "DUAL pairs a single layer with its double layer");
     static_assert(!SYM || DUAL, "SYM extends the DUAL pass");
@@ -265,6 +272,8 @@ __device__ __forceinline__ void pair_fp32(const TT &T_, int t, const TrialF32<QT
     const float kr_rev = (float)(kr * kInv2Pi);
     const float kr_f = (float)kr, ki_f = (float)ki;
     const float ki_log2e = (float)(ki * 1.4426950408889634);
+    // K = 1 / kr_rev and r0 K (infinite for k = 0: the phase is then 0)
+    const float rev_inv = 1.f / kr_rev, r0_rev_inv = r0 * rev_inv;
 
     // test normal projections of the trial offsets (ADL)
     float bnx[TrialF32<QT>::Q];
@@ -311,6 +320,8 @@ __device__ __forceinline__ void pair_fp32(const TT &T_, int t, const TrialF32<QT
             if constexpr (DUAL) Tr2[b] = Ti2[b] = 0.f;
             if constexpr (SYM) Tr3[b] = Ti3[b] = 0.f;
         }
+        constexpr int QO = ORB ? 6 : 1;
+        float Gv[6][QO];  // ORB: this test point's values at the 6 trial points (gre gim g2re g2im g3re g3im)
 #pragma unroll(QT > 0 ? QT : 1)
         for (int q = 0; q < nq; ++q) {
             // t = |y - x|^2 - r0^2
@@ -337,8 +348,14 @@ __device__ __forceinline__ void pair_fp32(const TT &T_, int t, const TrialF32<QT
                 gim = 0.f;
             } else {
                 const float r = r2 * rinv;
-                const float delta = tt * __builtin_amdgcn_rcpf(r + r0);
-                const float ph = kr_rev * delta;
+                // phase in revolutions, k (r - r0) / (2 pi) = t / ((r + r0) K) with K = 2 pi / k
+                float ph, delta = 0.f;
+                if constexpr (DAMP) {
+                    delta = tt * __builtin_amdgcn_rcpf(r + r0);
+                    ph = kr_rev * delta;
+                } else {
+                    ph = tt * __builtin_amdgcn_rcpf(r * rev_inv + r0_rev_inv);
+                }
                 const float s = sin_rev(ph), c = cos_rev(ph);
                 float g = rinv * F.wq[q];
                 if constexpr (DAMP) g *= __builtin_amdgcn_exp2f(-ki_log2e * delta);
@@ -369,6 +386,15 @@ __device__ __forceinline__ void pair_fp32(const TT &T_, int t, const TrialF32<QT
                     gre = (ar * c - ai * s) * g;
                     gim = (ai * c + ar * s) * g;
                 }
+            }
+            if constexpr (ORB) {
+                Gv[0][q] = gre;
+                Gv[1][q] = gim;
+                Gv[2][q] = g2re;
+                Gv[3][q] = g2im;
+                Gv[4][q] = g3re;
+                Gv[5][q] = g3im;
+                continue;
             }
             Tr[0] += gre;
             if constexpr (HELM) Ti[0] += gim;
@@ -404,6 +430,25 @@ __device__ __forceinline__ void pair_fp32(const TT &T_, int t, const TrialF32<QT
                         Ti3[2] += g3im * s1;
                     }
                 }
+            }
+        }
+        if constexpr (ORB) {
+            auto orbit = [&](const float(&G)[QO], float(&T)[NSS]) {
+                const float sa = G[0] + G[1] + G[2], sb = G[3] + G[4] + G[5];
+                const float base = rule.orb[0] * sa + rule.orb[2] * sb;
+                T[0] = sa + sb;
+                T[1] = rule.orb[1] * G[1] + (rule.orb[3] * G[4] + base);
+                T[2] = rule.orb[1] * G[2] + (rule.orb[3] * G[5] + base);
+            };
+            orbit(Gv[0], Tr);
+            if constexpr (HELM) orbit(Gv[1], Ti);
+            if constexpr (DUAL) {
+                orbit(Gv[2], Tr2);
+                if constexpr (HELM) orbit(Gv[3], Ti2);
+            }
+            if constexpr (SYM) {
+                orbit(Gv[4], Tr3);
+                if constexpr (HELM) orbit(Gv[5], Ti3);
             }
         }
         const float wp = T_.wp[t][p];
@@ -725,8 +770,9 @@ constexpr int regular_waves() {
 // UPPER (direct DP0 stores, a symmetric operator on one space): only the pairs whose trial element
 // comes later in the list than the test element are evaluated and stored; the caller mirrors the
 // upper triangle into the lower one (bem_regular_assemble_symmetric).
+// ORB: a.rule is a 6-point rule in orbit order (orbit_rule); the FP32 pairs use the orbit moments.
 template <int OP, int NTS, int NSS, int QT, int TM, bool MIXED, bool KAHAN, bool TILE, int MAXU_T, int MAXU_S,
-          typename OutT, bool DUAL = false, bool SYM = false, bool UPPER = false>
+          typename OutT, bool DUAL = false, bool SYM = false, bool UPPER = false, bool ORB = false>
 __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>())) void regular_kernel(
     const RegularArgs a) {
     static_assert(!SYM || TILE, "the symmetric pass is tiled");
@@ -1045,10 +1091,10 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                         if constexpr (MFX)
                             mfie_pair_fp32<QT>(T_, te, F32, kapf, d0, d1, d2, dist2, a.rule, a.kr, a.ki, Sf);
                         else if (OpTraits<OP>::is_complex && a.ki != 0.0)
-                            pair_fp32<GOP, NTS, NSS, QT, KAHAN, true, DUAL, REV>(T_, te, F32, d0, d1, d2, dist2, a.rule,
+                            pair_fp32<GOP, NTS, NSS, QT, KAHAN, true, DUAL, REV, ORB>(T_, te, F32, d0, d1, d2, dist2, a.rule,
                                                                                        a.kr, a.ki, Sf, Sf2, Sf3);
                         else
-                            pair_fp32<GOP, NTS, NSS, QT, KAHAN, false, DUAL, REV>(T_, te, F32, d0, d1, d2, dist2,
+                            pair_fp32<GOP, NTS, NSS, QT, KAHAN, false, DUAL, REV, ORB>(T_, te, F32, d0, d1, d2, dist2,
                                                                                         a.rule, a.kr, a.ki, Sf, Sf2, Sf3);
                     }
                 }
@@ -1274,6 +1320,16 @@ static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
         if constexpr (UPPER) {
             set_error("the symmetric DP0 pass stores directly (no dof blocks)");
             return BEM_ERR_INVALID_ARGUMENT;
+        }
+        if constexpr (SYM && MIXED && QT == 6 && NSS == 3 && (OP == kLaplaceSL || OP == kHelmholtzSL)) {
+            // the symmetric single layer (and double layer) pass with the orbit moments when the rule allows
+            RegularArgs b = a;
+            if (!kahan && orbit_rule(b.rule)) {
+                hipLaunchKernelGGL((regular_kernel<OP, NTS, NSS, QT, TM, MIXED, false, true, MT, MS, OutT, DUAL, true,
+                                                   false, true>),
+                                   grid, dim3(kRegThreads), 0, stream, b);
+                return check_launch("regular_kernel");
+            }
         }
         if constexpr (OP == kMaxwellMFIE) {
             BEM_LAUNCH(false, true);
