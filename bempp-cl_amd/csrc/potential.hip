@@ -363,6 +363,147 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed(const PotArg
     }
 }
 
+// MIXED scalar potentials (Laplace and Helmholtz single and double layer). The sources of a tile
+// of 256 are staged once as FP32 records relative to the tile's reference source c:
+//   u = c - y, |u|^2, the weights c_j / (4 pi) and, for the double layer, n_y and u.n_y,
+// read by every lane as LDS broadcasts. Per (target, tile) the FP64 part forms v = x - c,
+// r0^2 = |v|^2, r0 and e^{i k r0}; per source only
+//   t = r^2 - r0^2 = 2 v.u + |u|^2,  r = sqrt(r0^2 + t),  k (r - r0) / 2 pi = t / ((r + r0) K)
+// (K = 2 pi / k) are needed in FP32, with the hardware rsq, rcp, sin and cos (revolutions). DAMP:
+// complex wavenumber, e^{-Im(k) (r - r0)} per source (exp2), e^{-Im(k) r0} per tile.
+template <int OP, bool DAMP>
+__global__ __launch_bounds__(kPotThreads) void potential_eval_mixed_scalar(const PotArgs a) {
+    constexpr bool HELM = OP == kHelmholtzSL || OP == kHelmholtzDL;
+    constexpr bool DL = OP == kLaplaceDL || OP == kHelmholtzDL;
+    static_assert(!DAMP || HELM, "only Helmholtz wavenumbers are complex");
+    __shared__ float4 shu[kPotTile];           // u, |u|^2
+    __shared__ float4 shw[kPotTile];           // Re c / 4 pi, Im c / 4 pi, u.n_y, -
+    __shared__ float4 shn[DL ? kPotTile : 1];  // n_y, -
+    __shared__ double sc[3];
+    const int m = blockIdx.x * kPotThreads + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int s0 = chunk * a.chunk_len;
+    const int s1 = min(s0 + a.chunk_len, a.nsrc);
+    double x0 = 0, x1 = 0, x2 = 0;
+    if (m < a.npts) {
+        x0 = a.pts[3 * (int64_t)m];
+        x1 = a.pts[3 * (int64_t)m + 1];
+        x2 = a.pts[3 * (int64_t)m + 2];
+    }
+    double accr = 0.0, acci = 0.0;
+    const float kr_f = (float)a.kr, ki_f = (float)a.ki;
+    const float kr_rev = (float)(a.kr * 0.15915494309189533577);
+    const float ki_log2e = (float)(a.ki * 1.4426950408889634);
+    const float kinv = 1.f / kr_rev;  // K (infinite for k = 0: the phase is then 0)
+    const int P = a.nsrc_pad;
+    for (int t0 = s0; t0 < s1; t0 += kPotTile) {
+        const int nt = min(kPotTile, s1 - t0);
+        __syncthreads();
+        {
+            const int cidx = t0 + min(kPotTile / 2, nt - 1);  // reference: the tile's middle source
+            double c[3];
+            for (int d = 0; d < 3; ++d) c[d] = a.src[(int64_t)d * P + cidx];
+            if (threadIdx.x == 0)
+                for (int d = 0; d < 3; ++d) sc[d] = c[d];
+            const int j = threadIdx.x, s = t0 + j;
+            float4 U = make_float4(0.f, 0.f, 0.f, 0.f), W = make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 N = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (j < nt) {
+                const double u0 = c[0] - a.src[s], u1 = c[1] - a.src[(int64_t)P + s], u2 = c[2] - a.src[2 * (int64_t)P + s];
+                U = make_float4((float)u0, (float)u1, (float)u2, (float)(u0 * u0 + u1 * u1 + u2 * u2));
+                W.x = (float)(kInv4Pi * a.src[6 * (int64_t)P + s]);
+                W.y = (float)(kInv4Pi * a.src[7 * (int64_t)P + s]);
+                if constexpr (DL) {
+                    const double n0 = a.src[3 * (int64_t)P + s], n1 = a.src[4 * (int64_t)P + s],
+                                 n2 = a.src[5 * (int64_t)P + s];
+                    N = make_float4((float)n0, (float)n1, (float)n2, 0.f);
+                    W.z = (float)(u0 * n0 + u1 * n1 + u2 * n2);
+                }
+            }
+            shu[j] = U;
+            shw[j] = W;
+            if constexpr (DL) shn[j] = N;
+        }
+        __syncthreads();
+        // per (target, tile): FP64 reference distance and phase
+        const double v0d = x0 - sc[0], v1d = x1 - sc[1], v2d = x2 - sc[2];
+        const double r0sq_d = v0d * v0d + v1d * v1d + v2d * v2d;
+        const double r0d = sqrt(r0sq_d);
+        const float tv0 = (float)(2.0 * v0d), tv1 = (float)(2.0 * v1d), tv2 = (float)(2.0 * v2d);
+        const float v0 = (float)v0d, v1 = (float)v1d, v2 = (float)v2d;
+        const float r0sq = (float)r0sq_d, r0 = (float)r0d;
+        const float r0k = r0 * kinv;
+        float Ar = 0.f, Ai = 0.f;
+#pragma unroll 4
+        for (int j = 0; j < nt; ++j) {
+            const float4 U = shu[j];
+            const float4 W = shw[j];
+            const float tt = tv0 * U.x + (tv1 * U.y + (tv2 * U.z + U.w));  // r^2 - r0^2
+            const float r2 = r0sq + tt;
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            float er = rinv, ei = 0.f;  // e^{ik(r - r0)} / r (Laplace: 1 / r)
+            float r = 0.f;
+            if constexpr (HELM) {
+                r = r2 * rinv;
+                float ph, amp = rinv;
+                if constexpr (DAMP) {
+                    const float delta = tt * __builtin_amdgcn_rcpf(r + r0);
+                    ph = kr_rev * delta;
+                    amp *= __builtin_amdgcn_exp2f(-ki_log2e * delta);
+                } else {
+                    ph = tt * __builtin_amdgcn_rcpf(r * kinv + r0k);
+                }
+                const float cs = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
+                if constexpr (!DL) {
+                    // single layer: the weight times e^{ik(r - r0)} first, then one FMA by the amplitude
+                    Ar += amp * (cs * W.x - sn * W.y);
+                    Ai += amp * (cs * W.y + sn * W.x);
+                    continue;
+                }
+                er = cs * amp;
+                ei = sn * amp;
+            }
+            if constexpr (DL) {
+                // (i k r - 1) (y - x).n_y / r^2 with y - x = -(v + u)
+                const float4 N = shn[j];
+                const float dot = -(v0 * N.x + (v1 * N.y + (v2 * N.z + W.z)));
+                const float f = dot * rinv * rinv;
+                if constexpr (HELM) {
+                    const float ar = DAMP ? -1.f - ki_f * r : -1.f, ai = kr_f * r;
+                    const float pr = (ar * er - ai * ei) * f, pi = (ar * ei + ai * er) * f;
+                    er = pr;
+                    ei = pi;
+                } else {
+                    er = -er * f;
+                }
+            }
+            if constexpr (HELM) {
+                Ar += er * W.x - ei * W.y;
+                Ai += er * W.y + ei * W.x;
+            } else {
+                Ar += er * W.x;
+                Ai += er * W.y;
+            }
+        }
+        // multiply the tile sum by e^{i k r0} (e^{-Im(k) r0}) in FP64
+        double pr = 1.0, pi = 0.0;
+        if constexpr (HELM) {
+            double sn, cs;
+            sincos(a.kr * r0d, &sn, &cs);
+            const double damp0 = DAMP ? exp(-a.ki * r0d) : 1.0;
+            pr = cs * damp0;
+            pi = sn * damp0;
+        }
+        accr += pr * Ar - pi * Ai;
+        acci += pr * Ai + pi * Ar;
+    }
+    if (m < a.npts) {
+        double *p = a.partial + 2 * ((int64_t)chunk * a.npts + m);
+        p[0] = accr;
+        p[1] = acci;
+    }
+}
+
 // Far-field integrands per source (sources from potential_prep_kernel: c for scalar spaces,
 // t1 and t2 of the EFIE for RWG):
 //   Helmholtz single layer  K c
@@ -758,7 +899,16 @@ static int run_potential(PotArgs &a, int precision, int out_dtype, hipStream_t s
         else
             hipLaunchKernelGGL((far_field_eval_fp64<FF>), grid, dim3(kPotThreads), 0, s, a);
     }
-    else if (precision == BEM_PRECISION_MIXED)
+    else if (precision == BEM_PRECISION_MIXED && OP != kMaxwellEFIE) {
+        if constexpr (OP == kHelmholtzSL || OP == kHelmholtzDL) {
+            if (a.ki != 0.0)
+                hipLaunchKernelGGL((potential_eval_mixed_scalar<OP, true>), grid, dim3(kPotThreads), 0, s, a);
+            else
+                hipLaunchKernelGGL((potential_eval_mixed_scalar<OP, false>), grid, dim3(kPotThreads), 0, s, a);
+        } else if constexpr (OP != kMaxwellEFIE) {
+            hipLaunchKernelGGL((potential_eval_mixed_scalar<OP, false>), grid, dim3(kPotThreads), 0, s, a);
+        }
+    } else if (precision == BEM_PRECISION_MIXED)
         hipLaunchKernelGGL((potential_eval_mixed<OP>), grid, dim3(kPotThreads), 0, s, a);
     else
         hipLaunchKernelGGL((potential_eval_fp64<OP>), grid, dim3(kPotThreads), 0, s, a);
