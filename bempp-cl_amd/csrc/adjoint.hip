@@ -324,7 +324,7 @@ __device__ __forceinline__ void pair_terms_fp64(double k, double Wr, double Wi, 
     const double dx = yq[0] - xp[0], dy = yq[1] - xp[1], dz = yq[2] - xp[2];
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
     double s, c;
-    sincos(k * r, &s, &c);
+    sincos_fast(k * r, &s, &c);
     const double Zr = Wr * c - Wi * s, Zi = Wr * s + Wi * c;
     const double amp = wpJ * wq * kInv4Pi / r;
     S += amp * Zr;
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(256) void adjoint_singular_kernel(const Args a, int
             dv[d] = (gf[d] + gf[3 + d] * et0 + gf[6 + d] * et1) - (gt[d] + gt[3 + d] * xi0 + gt[6 + d] * xi1);
         const double r = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
         double s, c;
-        sincos(a.k * r, &s, &c);
+        sincos_fast(a.k * r, &s, &c);
         const double Zr = Wr * c - Wi * s, Zi = Wr * s + Wi * c;
         const double amp = w[kk] * kInv4Pi / r;
         S += amp * Zr;
@@ -583,7 +583,7 @@ __global__ __launch_bounds__(kThreads) void potential_adjoint_kernel(const PotAr
                 const double dx = yq[q][0] - P_[t][0], dy = yq[q][1] - P_[t][1], dz = yq[q][2] - P_[t][2];
                 const double r = sqrt(dx * dx + dy * dy + dz * dz);
                 double s, c;
-                sincos(a.k * r, &s, &c);
+                sincos_fast(a.k * r, &s, &c);
                 const double amp = a.rule.w[q] * kInv4Pi / r;  // without J_e
                 // grad_c: conj(G) g
                 hr += amp * (c * gr + s * gi);

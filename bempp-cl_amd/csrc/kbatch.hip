@@ -358,7 +358,7 @@ __global__ __launch_bounds__(kRegThreads) void kbatch_fp64_kernel(const Args a) 
                                          dz = yq[q][2] - T_.xp[t][p][2];
                             const double r = sqrt(dx * dx + dy * dy + dz * dz);
                             double s, c;
-                            sincos(k * r, &s, &c);
+                            sincos_fast(k * r, &s, &c);
                             const double g = wq[q] * wp / r;
                             sr += c * g;
                             si += s * g;
@@ -437,7 +437,7 @@ __device__ __forceinline__ void near_pair_moments(const Args &a, const NearPair 
         const double k = a.k[km];
         double re, im, s, cs;
         taylor_eval<NM>(c, k, re, im);
-        sincos(k * P.r0, &s, &cs);
+        sincos_fast(k * P.r0, &s, &cs);
         add_c(a.out, km * a.kstride + P.base, (float)(cs * re - s * im), (float)(cs * im + s * re));
     }
 }
@@ -449,7 +449,7 @@ __device__ __forceinline__ void near_pair_direct(const Args &a, const NearPair &
         double sr = 0.0, si = 0.0;
         near_points(a, P, [&](double amp, double r) {
             double s, c;
-            sincos(k * r, &s, &c);
+            sincos_fast(k * r, &s, &c);
             sr += c * amp;
             si += s * amp;
         });
@@ -558,7 +558,7 @@ __device__ __forceinline__ void singular_pair_moments(const Args &a, const Singu
         const double k = a.k[km];
         double re, im, s, cs;
         taylor_eval<NM>(c, k, re, im);
-        sincos(k * rc, &s, &cs);
+        sincos_fast(k * rc, &s, &cs);
         add_c(a.out, km * a.kstride + P.base, (OutT)(cs * re - s * im), (OutT)(cs * im + s * re));
     }
 }
@@ -570,7 +570,7 @@ __device__ __forceinline__ void singular_pair_direct(const Args &a, const Singul
         double sr = 0.0, si = 0.0;
         singular_points(P, lane, [&](double g, double r) {
             double s, c;
-            sincos(k * r, &s, &c);
+            sincos_fast(k * r, &s, &c);
             sr += c * g;
             si += s * g;
         });

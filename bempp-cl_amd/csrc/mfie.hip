@@ -58,7 +58,7 @@ __device__ __forceinline__ void accumulate(const double (&diff)[3], const double
     const double r2 = diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2];
     const double r = sqrt(r2);
     double sn, cs;
-    sincos(kr * r, &sn, &cs);
+    sincos_fast(kr * r, &sn, &cs);
     const double damp = ki != 0.0 ? exp(-ki * r) : 1.0;
     const double g = wgt * kInv4Pi * damp / (r * r2);  // |G| / r^2 times the weight
     // G (ik r - 1) / r^2 = g e^{i kr r} ((-ki r - 1) + i kr r)

@@ -140,7 +140,7 @@ __device__ __forceinline__ void pair_block_fp64(AP ap_of, const double *e1, cons
             const double r2 = u0 * u0 + u1 * u1 + u2 * u2;
             const double r = sqrt(r2);
             double sn, cs;
-            sincos(kr * r, &sn, &cs);
+            sincos_fast(kr * r, &sn, &cs);
             const double damp = ki != 0.0 ? exp(-ki * r) : 1.0;
             const double g = rule.w[p] * rule.w[q] * kInv4Pi * damp / (r * r2);
             const double fr = -ki * r - 1.0, fi = kr * r;

@@ -251,7 +251,7 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed(const PotArg
         double r0sq_d = v0d * v0d + v1d * v1d + v2d * v2d;
         double r0d = sqrt(r0sq_d);
         double ph_s, ph_c;
-        sincos(a.kr * r0d, &ph_s, &ph_c);
+        sincos_fast(a.kr * r0d, &ph_s, &ph_c);
         double damp0 = a.ki != 0.0 ? exp(-a.ki * r0d) : 1.0;
         const float v0 = (float)v0d, v1 = (float)v1d, v2 = (float)v2d;
         const float r0 = (float)r0d, r0sq = (float)r0sq_d;
@@ -489,7 +489,7 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed_scalar(const
         double pr = 1.0, pi = 0.0;
         if constexpr (HELM) {
             double sn, cs;
-            sincos(a.kr * r0d, &sn, &cs);
+            sincos_fast(a.kr * r0d, &sn, &cs);
             const double damp0 = DAMP ? exp(-a.ki * r0d) : 1.0;
             pr = cs * damp0;
             pi = sn * damp0;
@@ -549,7 +549,7 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_fp64(const PotArgs
                 const double d0 = x0 - sh[0][j], d1 = x1 - sh[1][j], d2 = x2 - sh[2][j];
                 const double r2 = d0 * d0 + d1 * d1 + d2 * d2, r = sqrt(r2);
                 double sn, cs;
-                sincos(kr * r, &sn, &cs);
+                sincos_fast(kr * r, &sn, &cs);
                 const double damp = ki != 0.0 ? exp(-ki * r) : 1.0;
                 const double g = kInv4Pi * damp / (r * r2);
                 const double fr = -ki * r - 1.0, fi = kr * r;  // ik r - 1
@@ -572,7 +572,7 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_fp64(const PotArgs
             // padding sources sit at 1e30 with zero weights: the phase stays finite
             const double dot = x0 * sh[0][j] + x1 * sh[1][j] + x2 * sh[2][j];
             double sn, cs;
-            sincos(-kr * dot, &sn, &cs);
+            sincos_fast(-kr * dot, &sn, &cs);
             const double gre = kInv4Pi * cs, gim = kInv4Pi * sn;
             if constexpr (FF == kFarSL || FF == kFarDL) {
                 double vr = gre, vi = gim;
@@ -711,7 +711,7 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_mixed(const PotArg
                 const double r0sq_d = v0d * v0d + v1d * v1d + v2d * v2d;
                 const double r0d = sqrt(r0sq_d);
                 double sn, cs;
-                sincos(a.kr * r0d, &sn, &cs);
+                sincos_fast(a.kr * r0d, &sn, &cs);
                 const double dmp = a.ki != 0.0 ? exp(-a.ki * r0d) : 1.0;
                 pr = cs * dmp;
                 pi = sn * dmp;
@@ -752,7 +752,7 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_mixed(const PotArg
                 }
             } else {
                 double sn, cs;
-                sincos(-a.kr * (x0 * sc[g][0] + x1 * sc[g][1] + x2 * sc[g][2]), &sn, &cs);
+                sincos_fast(-a.kr * (x0 * sc[g][0] + x1 * sc[g][1] + x2 * sc[g][2]), &sn, &cs);
                 pr = kInv4Pi * cs;
                 pi = kInv4Pi * sn;
                 for (int j = j0; j < j1; ++j) {
