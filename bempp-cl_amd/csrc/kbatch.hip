@@ -268,12 +268,15 @@ __global__ __launch_bounds__(kRegThreads) void kbatch_far_kernel(const Args a) {
             // expansion order for the largest pair of the wavefront (x is wave-uniform)
             const float x = kmax * (T_.rho[t] + rho_fw) * 1.0001f;
             const int N = taylor_order(x, kTol32, 16);
-            const int bucket = __builtin_amdgcn_readfirstlane(N <= 4 ? 0 : N <= 8 ? 1 : N <= 12 ? 2 : N <= 16 ? 3 : 4);
+            const int bucket = __builtin_amdgcn_readfirstlane(N <= 4 ? 0 : N <= 6 ? 1 : N <= 8 ? 2 : N <= 10 ? 3 : N <= 12 ? 4
+                                                             : N <= 16 ? 5 : 6);
             switch (bucket) {
             case 0: far_pair_moments<4>(a, T_, t, L, P, skip, m, valid, base); break;
-            case 1: far_pair_moments<8>(a, T_, t, L, P, skip, m, valid, base); break;
-            case 2: far_pair_moments<12>(a, T_, t, L, P, skip, m, valid, base); break;
-            case 3: far_pair_moments<16>(a, T_, t, L, P, skip, m, valid, base); break;
+            case 1: far_pair_moments<6>(a, T_, t, L, P, skip, m, valid, base); break;
+            case 2: far_pair_moments<8>(a, T_, t, L, P, skip, m, valid, base); break;
+            case 3: far_pair_moments<10>(a, T_, t, L, P, skip, m, valid, base); break;
+            case 4: far_pair_moments<12>(a, T_, t, L, P, skip, m, valid, base); break;
+            case 5: far_pair_moments<16>(a, T_, t, L, P, skip, m, valid, base); break;
             default: far_pair_direct(a, T_, t, L, P, skip, m, valid, base); break;
             }
         }

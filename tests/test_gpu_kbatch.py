@@ -57,14 +57,18 @@ def test_fp64_sweep_matches_single_wavenumber_and_oracle(ico2, dp0, default_para
 # the direct fallback: the order grows with k_max times the element size (level 2: h ~ 0.3,
 # level 3: h ~ 0.15).
 SWEEPS = [
+    (3, np.linspace(0.05, 0.25, 4)),
+    (2, np.linspace(0.1, 0.5, 4)),
     (2, np.linspace(0.1, 1.0, 8)),
     (2, np.linspace(1.0, 4.0, 8)),
+    (3, np.linspace(1.0, 8.0, 8)),
     (2, np.linspace(1.0, 16.0, 32)),
     (3, np.linspace(1.0, 16.0, 32)),
 ]
 
 
-@pytest.mark.parametrize("level,ks", SWEEPS, ids=["L2_k0.1-1", "L2_k1-4", "L2_k1-16", "L3_k1-16"])
+@pytest.mark.parametrize("level,ks", SWEEPS, ids=["L3_k0.05-0.25", "L2_k0.1-0.5", "L2_k0.1-1", "L2_k1-4", "L3_k1-8",
+                                                 "L2_k1-16", "L3_k1-16"])
 def test_mixed_sweep_vs_fp64(level, ks, default_parameters, hip_available):
     import torch
     from bempp_amd.api import function_space
