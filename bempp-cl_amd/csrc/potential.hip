@@ -22,12 +22,36 @@
 #include "bem_internal.h"
 
 #include <mutex>
+#include <type_traits>
 
 namespace bem {
 
 constexpr int kPotThreads = 256;
 constexpr int kPotTile = 256;
 constexpr int kSrcFields = 14;  // px py pz nx ny nz c_re c_im t1 (6): EFIE uses c as t2
+
+// Taylor coefficients of e^{ik delta} - 1 in powers of delta for a real wavenumber k (the EFIE
+// potential's series, see efie_em1).
+struct EfieSeries {
+    float c[5];  // (-1)^n k^2n / (2n)!, n = 1..5
+    float s[4];  // (-1)^n k^(2n+1) / (2n+1)!, n = 1..4
+    float k;
+};
+
+static EfieSeries efie_series(double k) {
+    EfieSeries e;
+    double cn = 1.0, sn = k;
+    for (int n = 1; n <= 5; ++n) {
+        cn *= -k * k / ((2.0 * n - 1.0) * (2.0 * n));
+        e.c[n - 1] = (float)cn;
+        if (n <= 4) {
+            sn *= -k * k / ((2.0 * n) * (2.0 * n + 1.0));
+            e.s[n - 1] = (float)sn;
+        }
+    }
+    e.k = (float)k;
+    return e;
+}
 
 struct PotArgs {
     SpaceArgs sp;
@@ -41,6 +65,7 @@ struct PotArgs {
     double *partial;  // [nchunk][kdim][npts][2]
     int nchunk, chunk_len;
     void *out;
+    EfieSeries ser;  // series of e^{i Re(k) delta} - 1 (EFIE potential)
 };
 
 template <int OP>
@@ -174,189 +199,263 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_fp64(const PotArgs
     }
 }
 
-// MIXED evaluation: FP32 per source, FP64 per (target, group of sources). EFIE uses groups of
-// 32 sources (a few neighbouring elements) as references, the scalar kernels one group per tile.
 // sin and cos of a half angle x (radians) with relative accuracy, for e^{i 2x} - 1 = -2 sin^2 x +
 // 2i sin x cos x, a small value that must not inherit the absolute error of the hardware sine:
 // Taylor series for |x| <= 0.5 (remainders x^11 / 11! and x^10 / 10! < 3e-10), sincosf beyond.
 __device__ __forceinline__ void half_angle_sincos(float x, float &s, float &c) {
     if (fabsf(x) <= 0.5f) {
         const float x2 = x * x;
-        s = x * (1.f - x2 * (1.f / 6.f) * (1.f - x2 * (1.f / 20.f) * (1.f - x2 * (1.f / 42.f) * (1.f - x2 * (1.f / 72.f)))));
-        c = 1.f - x2 * 0.5f * (1.f - x2 * (1.f / 12.f) * (1.f - x2 * (1.f / 30.f) * (1.f - x2 * (1.f / 56.f))));
+        float p = fmaf(x2, 1.f / 362880.f, -1.f / 5040.f);
+        p = fmaf(x2, p, 1.f / 120.f);
+        p = fmaf(x2, p, -1.f / 6.f);
+        s = fmaf(x * x2, p, x);
+        float q = fmaf(x2, 1.f / 40320.f, -1.f / 720.f);
+        q = fmaf(x2, q, 1.f / 24.f);
+        q = fmaf(x2, q, -0.5f);
+        c = fmaf(x2, q, 1.f);
     } else {
         sincosf(x, &s, &c);
     }
 }
 
-template <int OP>
-__global__ __launch_bounds__(kPotThreads) void potential_eval_mixed(const PotArgs a) {
-    constexpr int KD = OP == kMaxwellEFIE ? 3 : 1;
-    constexpr int NF = OP == kMaxwellEFIE ? kSrcFields : 8;
-    constexpr int GS = OP == kMaxwellEFIE ? 32 : kPotTile;  // sources per reference group
+// e^{ik delta} - 1 = (cos y - 1) + i sin y, y = k delta (k real), with relative accuracy from the
+// Taylor series in delta with the powers of k folded into the coefficients:
+//   cos y - 1 = d2 (c2 + d2 (c4 + ...)),  sin y = delta s,  s = k + d2 (s3 + d2 (s5 + ...)),
+// d2 = delta^2. MODE 0 (|y| <= 0.25): through y^6 and y^5; MODE 1 (|y| <= 0.5): through y^8 and
+// y^7; MODE 2 (|y| <= 1): through y^10 and y^9 (remainders below 5e-8 of the leading terms
+// y^2 / 2 and y). MODE 3, any y: the series of MODE 2 in y where |y| <= 1, half-angle sines beyond
+// (si is then not used).
+template <int MODE>
+__device__ __forceinline__ void efie_em1(float delta, const EfieSeries &e, float &em1r, float &em1i, float &si) {
+    const float d2 = delta * delta;
+    if constexpr (MODE < 3) {
+        constexpr int NC = MODE + 3, NS = MODE + 2;  // terms of the cosine and sine series
+        float p = e.c[NC - 1];
+#pragma unroll
+        for (int n = NC - 2; n >= 0; --n) p = fmaf(d2, p, e.c[n]);
+        em1r = d2 * p;
+        float q = e.s[NS - 1];
+#pragma unroll
+        for (int n = NS - 2; n >= 0; --n) q = fmaf(d2, q, e.s[n]);
+        si = fmaf(d2, q, e.k);
+        em1i = delta * si;
+    } else {
+        const float y = e.k * delta;
+        if (fabsf(y) <= 1.f) {
+            const float y2 = y * y;
+            float p = fmaf(y2, -1.f / 3628800.f, 1.f / 40320.f);
+            p = fmaf(y2, p, -1.f / 720.f);
+            p = fmaf(y2, p, 1.f / 24.f);
+            p = fmaf(y2, p, -0.5f);
+            em1r = y2 * p;
+            float q = fmaf(y2, 1.f / 362880.f, -1.f / 5040.f);
+            q = fmaf(y2, q, 1.f / 120.f);
+            q = fmaf(y2, q, -1.f / 6.f);
+            em1i = fmaf(y * y2, q, y);
+        } else {
+            float hs, hc;
+            sincosf(0.5f * y, &hs, &hc);
+            em1r = -2.f * hs * hs;
+            em1i = 2.f * hs * hc;
+        }
+        si = 0.f;
+    }
+}
+
+// MIXED EFIE potential: FP32 per source, FP64 per (target, group of 32 sources, a few
+// neighbouring elements). With c the centre of the group's bounding box, v = x - c, u = c - y
+// (so x - y = v + u), r0 = |v|, delta = r - r0, E = e^{ik delta}, h(r) = 1/r^2 - alpha/r^3
+// (alpha = 1/(ik)), one source contributes e^{ik r0} / (4 pi) [ ik (E / r) t1 - E h(r) (v + u) t2 ].
+// It is split into
+//   the group-centre term  ik t1 / r0 - h(r0) v t2   (group sums of t1, t2 in FP64), and
+//   the remainder          f t1' - (b v + c u) t2'   (FP32, t1' = ik t1 / 4 pi, t2' = t2 / 4 pi),
+// f = E/r - 1/r0, b = E h(r) - h(r0), c = E h(r). f and b are small differences formed without
+// cancellation from E - 1 (series in delta) and q = 1/r0 - 1/r = delta / (r r0):
+//   f = (E - 1) / r - q,   dh = h(r) - h(r0) = -q (1/r + 1/r0) + alpha q (1/r^2 + 1/(r r0) + 1/r0^2),
+//   b = dh + (E - 1) h(r),  c = h(r) + (E - 1) h(r).
+// b v t2' is summed per group as v B, B = sum b t2'. Sources are staged per tile of 256 as three
+// FP32 records (u, |u|^2), (t2', t1'_x), (t1'_y, t1'_z), read as LDS broadcasts. The group radius
+// max |u| bounds |delta| and selects the shortest series for e^{ik delta} - 1 (wave-uniform).
+// DAMP: complex wavenumber, E = e^{-Im(k) delta} e^{i Re(k) delta}.
+template <bool DAMP>
+__global__ __launch_bounds__(kPotThreads) void potential_eval_mixed_efie(const PotArgs a) {
+    constexpr int GS = 32;  // sources per reference group
     constexpr int NG = kPotTile / GS;
-    __shared__ float sh[NF][kPotTile];
+    __shared__ float4 shu[kPotTile];  // u, |u|^2
+    __shared__ float4 shq[kPotTile];  // t2', t1'_x
+    __shared__ float4 shp[kPotTile];  // t1'_y, t1'_z
     __shared__ double sc[NG][3];
-    __shared__ double wsum[NG][OP == kMaxwellEFIE ? 8 : 1];
+    __shared__ double wsum[NG][8];    // group sums of t2 and t1 (FP64)
+    __shared__ float srad[NG];        // group radius max |u| (bounds |delta|)
     const int m = blockIdx.x * kPotThreads + threadIdx.x;
     const int chunk = blockIdx.y;
     const int s0 = chunk * a.chunk_len;
     const int s1 = min(s0 + a.chunk_len, a.nsrc);
+    const int P = a.nsrc_pad;
     double x0 = 0, x1 = 0, x2 = 0;
     if (m < a.npts) {
         x0 = a.pts[3 * (int64_t)m];
         x1 = a.pts[3 * (int64_t)m + 1];
         x2 = a.pts[3 * (int64_t)m + 2];
     }
-    double accr[KD], acci[KD];
-    for (int d = 0; d < KD; ++d) accr[d] = acci[d] = 0.0;
-    const float kr = (float)a.kr, ki = (float)a.ki;
-    // phases in revolutions for the hardware sin / cos (|k (r - r0)| stays within a few radians)
-    const float kr_rev = (float)(a.kr * 0.15915494309189533577);
+    double accr[3] = {0.0, 0.0, 0.0}, acci[3] = {0.0, 0.0, 0.0};
+    const float ki = (float)a.ki;
+    const EfieSeries &ser = a.ser;
+    // alpha = 1/(ik) (0 for k = 0, where the t2 term has no 1/r^3 part)
+    const double kkd = a.kr * a.kr + a.ki * a.ki;
+    const double alr_d = kkd > 0.0 ? -a.ki / kkd : 0.0, ali_d = kkd > 0.0 ? -a.kr / kkd : 0.0;
+    const float al_re = (float)alr_d, al_im = (float)ali_d;
+    const float kabs = fabsf((float)a.kr);
     for (int t0 = s0; t0 < s1; t0 += kPotTile) {
         const int nt = min(kPotTile, s1 - t0);
         __syncthreads();
         {
-            // reference point of this thread's group: its middle source (a real source)
-            const int g = threadIdx.x / GS;
-            const int cidx = t0 + min(g * GS + GS / 2, nt - 1);
-            double c[3];
-            for (int d = 0; d < 3; ++d) c[d] = a.src[(int64_t)d * a.nsrc_pad + cidx];
-            if (threadIdx.x % GS == 0)
-                for (int d = 0; d < 3; ++d) sc[g][d] = c[d];
-            int s = t0 + threadIdx.x;
-            bool ok = threadIdx.x < nt;
-            for (int d = 0; d < 3; ++d) sh[d][threadIdx.x] = ok ? (float)(c[d] - a.src[(int64_t)d * a.nsrc_pad + s]) : 0.f;
-            for (int k = 3; k < NF; ++k) sh[k][threadIdx.x] = ok ? (float)a.src[(int64_t)k * a.nsrc_pad + s] : 0.f;
-            if constexpr (OP == kMaxwellEFIE) {
-                // FP64 group sums of the source weights t2 (fields 6, 7) and t1 (fields 8..13)
-                double v[8];
+            const int j = threadIdx.x, g = j / GS, s = t0 + j;
+            const bool ok = j < nt;
+            // bounding box of the group's sources (lanes g*GS .. g*GS+GS-1 of one wave)
+            double y[3], lo[3], hi[3];
 #pragma unroll
-                for (int k = 0; k < 8; ++k) v[k] = ok ? a.src[(int64_t)(6 + k) * a.nsrc_pad + s] : 0.0;
+            for (int d = 0; d < 3; ++d) {
+                y[d] = ok ? a.src[(int64_t)d * P + s] : 0.0;
+                lo[d] = ok ? y[d] : INFINITY;
+                hi[d] = ok ? y[d] : -INFINITY;
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-#pragma unroll
-                    for (int off = GS / 2; off >= 1; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+                for (int off = GS / 2; off >= 1; off >>= 1) {
+                    lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, 64));
+                    hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
                 }
-                if (threadIdx.x % GS == 0)
+            }
+            const double c[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
+            if (j % GS == 0)
+                for (int d = 0; d < 3; ++d) sc[g][d] = c[d];
+            double w[8];
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) wsum[g][k] = v[k];
+            for (int k = 0; k < 8; ++k) w[k] = ok ? a.src[(int64_t)(6 + k) * P + s] : 0.0;
+            float4 U = make_float4(0.f, 0.f, 0.f, 0.f), Q = U, S = U;
+            double uu = 0.0;
+            if (ok) {
+                const double u0 = c[0] - y[0], u1 = c[1] - y[1], u2 = c[2] - y[2];
+                uu = u0 * u0 + u1 * u1 + u2 * u2;
+                U = make_float4((float)u0, (float)u1, (float)u2, (float)uu);
+                const double ikr = -a.ki, iki = a.kr;
+                float t1[6];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double tr = w[2 + 2 * d], ti = w[3 + 2 * d];
+                    t1[2 * d] = (float)(kInv4Pi * (ikr * tr - iki * ti));
+                    t1[2 * d + 1] = (float)(kInv4Pi * (ikr * ti + iki * tr));
+                }
+                Q = make_float4((float)(kInv4Pi * w[0]), (float)(kInv4Pi * w[1]), t1[0], t1[1]);
+                S = make_float4(t1[2], t1[3], t1[4], t1[5]);
+            }
+            shu[j] = U;
+            shq[j] = Q;
+            shp[j] = S;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+#pragma unroll
+                for (int off = GS / 2; off >= 1; off >>= 1) w[k] += __shfl_xor(w[k], off, 64);
+            }
+#pragma unroll
+            for (int off = GS / 2; off >= 1; off >>= 1) uu = fmax(uu, __shfl_xor(uu, off, 64));
+            if (j % GS == 0) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) wsum[g][k] = w[k];
+                srad[g] = (float)(sqrt(uu) * (1.0 + 1e-6));
             }
         }
         __syncthreads();
         for (int g = 0; g < NG && g * GS < nt; ++g) {
-        const int j0 = g * GS, j1 = min(j0 + GS, nt);
-        // per (target, group) FP64 reference distance and phase
-        double v0d = x0 - sc[g][0], v1d = x1 - sc[g][1], v2d = x2 - sc[g][2];
-        double r0sq_d = v0d * v0d + v1d * v1d + v2d * v2d;
-        double r0d = sqrt(r0sq_d);
-        double ph_s, ph_c;
-        sincos_fast(a.kr * r0d, &ph_s, &ph_c);
-        double damp0 = a.ki != 0.0 ? exp(-a.ki * r0d) : 1.0;
-        const float v0 = (float)v0d, v1 = (float)v1d, v2 = (float)v2d;
-        const float r0 = (float)r0d, r0sq = (float)r0sq_d;
-        // EFIE: ik, alpha = 1/(ik), 1/r0 powers and h0 = h(r0) = 1/r0^2 - alpha/r0^3
-        const float ik_re = -ki, ik_im = kr;
-        const float kk = kr * kr + ki * ki;
-        const float al_re = kk > 0.f ? -ki / kk : 0.f, al_im = kk > 0.f ? -kr / kk : 0.f;
-        const float r0inv = (float)(1.0 / r0d), r0inv2 = r0inv * r0inv, r0inv3 = r0inv2 * r0inv;
-        const float h0r = r0inv2 - al_re * r0inv3, h0i = -al_im * r0inv3;
-        (void)h0r;
-        (void)h0i;
-        float Ar[KD], Ai[KD];
-        for (int d = 0; d < KD; ++d) Ar[d] = Ai[d] = 0.f;
-        for (int j = j0; j < j1; ++j) {
-            // x - y = v + u, u = c - y
-            float u0 = sh[0][j], u1 = sh[1][j], u2 = sh[2][j];
-            float w0 = v0 + u0, w1 = v1 + u1, w2 = v2 + u2;
-            float tt = u0 * (2.f * v0 + u0) + u1 * (2.f * v1 + u1) + u2 * (2.f * v2 + u2);  // r^2 - r0^2
-            float r2 = r0sq + tt;
-            float rinv = rsqrt_t(r2);
-            float r = r2 * rinv;
-            float delta = tt * __builtin_amdgcn_rcpf(r + r0);
-            const float s = __builtin_amdgcn_sinf(kr_rev * delta), c = __builtin_amdgcn_cosf(kr_rev * delta);
-            float damp = ki != 0.f ? __expf(-ki * delta) : 1.f;
-            // e^{ik(r - r0)} / (4 pi r) ...
-            float g = float(kInv4Pi) * rinv * damp;
-            float er = c * g, ei = s * g;
-            if constexpr (OP == kLaplaceSL || OP == kHelmholtzSL) {
-                float cr = sh[6][j], ci = sh[7][j];
-                Ar[0] += er * cr - ei * ci;
-                Ai[0] += er * ci + ei * cr;
-            } else if constexpr (OP == kLaplaceDL || OP == kHelmholtzDL) {
-                // (ikr - 1) (y - x).n_y / r^2 factor; y - x = -w
-                float dot = -(w0 * sh[3][j] + w1 * sh[4][j] + w2 * sh[5][j]);
-                float f = dot * rinv * rinv;
-                float ar = -1.f - ki * r, ai = kr * r;
-                float fr = (ar * er - ai * ei) * f, fi = (ar * ei + ai * er) * f;
-                float cr = sh[6][j], ci = sh[7][j];
-                Ar[0] += fr * cr - fi * ci;
-                Ai[0] += fr * ci + fi * cr;
-            } else {  // EFIE: the tile-centre kernel value is subtracted here and added back in FP64
-                (void)er;
-                (void)ei;
-                // E - 1 = e^{ik delta} - 1 from half-angle sin/cos (no cancellation for small delta)
-                float sh2, ch2;
-                half_angle_sincos(0.5f * kr * delta, sh2, ch2);
-                float em1r = -2.f * sh2 * sh2, em1i = 2.f * sh2 * ch2;
-                if (ki != 0.f) {
-                    const float dmp = __expf(-ki * delta), dm1 = expm1f(-ki * delta);
-                    em1r = dm1 + dmp * em1r;
-                    em1i = dmp * em1i;
-                }
-                const float Er = 1.f + em1r, Ei = em1i;
-                // t1: ik (E / r - 1 / r0) t1,  E / r - 1 / r0 = (E - 1) / r - delta / (r r0)
-                const float fr = em1r * rinv - delta * rinv * r0inv, fi = em1i * rinv;
-                const float cr1 = ik_re * fr - ik_im * fi, ci1 = ik_re * fi + ik_im * fr;
-                // t2: -((E dh + (E - 1) h0) v + E h u) t2, h = 1/r^2 - alpha/r^3
-                const float rinv2 = rinv * rinv, rinv3 = rinv2 * rinv;
-                const float dh1 = -tt * rinv2 * r0inv2, dh3 = delta * (r2 + r * r0 + r0sq) * rinv3 * r0inv3;
-                const float dhr = dh1 + al_re * dh3, dhi = al_im * dh3;  // h(r) - h(r0)
-                const float hr = rinv2 - al_re * rinv3, hi = -al_im * rinv3;
-                const float pvr = (Er * dhr - Ei * dhi) + (em1r * h0r - em1i * h0i);
-                const float pvi = (Er * dhi + Ei * dhr) + (em1r * h0i + em1i * h0r);
-                const float pur = Er * hr - Ei * hi, pui = Er * hi + Ei * hr;
-                const float t2r = sh[6][j], t2i = sh[7][j];
-                const float vv[3] = {v0, v1, v2}, uu[3] = {u0, u1, u2};
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    const float t1r = sh[8 + 2 * d][j], t1i = sh[9 + 2 * d][j];
-                    const float gr = pvr * vv[d] + pur * uu[d], gi = pvi * vv[d] + pui * uu[d];
-                    Ar[d] += float(kInv4Pi) * ((cr1 * t1r - ci1 * t1i) - (gr * t2r - gi * t2i));
-                    Ai[d] += float(kInv4Pi) * ((cr1 * t1i + ci1 * t1r) - (gr * t2i + gi * t2r));
-                }
-            }
-        }
-        // multiply the tile sum by e^{ik r0} (Laplace: kr = ki = 0 gives 1)
-        double pr = ph_c * damp0, pi = ph_s * damp0;
-        if constexpr (OP == kMaxwellEFIE) {
-            // tile-centre term in FP64: ik T1 / r0 - h0 v T2 (times 1/(4 pi), e^{ik r0} below)
-            double T[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) T[k] = wsum[g][k];
-            const double ikr_d = -a.ki, iki_d = a.kr, kkd = a.kr * a.kr + a.ki * a.ki;
-            const double alr = kkd > 0.0 ? -a.ki / kkd : 0.0, ali = kkd > 0.0 ? -a.kr / kkd : 0.0;
+            const int j0 = g * GS, j1 = min(j0 + GS, nt);
+            // |Re(k) delta| <= |Re(k)| max |u| (wave-uniform)
+            const float ymax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(kabs * srad[g])));
+            // per (target, group) FP64 reference distance and phase
+            const double v0d = x0 - sc[g][0], v1d = x1 - sc[g][1], v2d = x2 - sc[g][2];
+            const double r0sq_d = v0d * v0d + v1d * v1d + v2d * v2d;
+            const double r0d = sqrt(r0sq_d);
             const double i0 = 1.0 / r0d, i2 = i0 * i0, i3 = i2 * i0;
-            const double hr0 = i2 - alr * i3, hi0 = -ali * i3;
+            const float tv0 = (float)(2.0 * v0d), tv1 = (float)(2.0 * v1d), tv2 = (float)(2.0 * v2d);
+            const float r0 = (float)r0d, r0sq = (float)r0sq_d;
+            const float r0inv = (float)i0, r0inv2 = (float)i2;
+            const float h0r = (float)(i2 - alr_d * i3), h0i = (float)(-ali_d * i3);
+            float Ar[3] = {0.f, 0.f, 0.f}, Ai[3] = {0.f, 0.f, 0.f}, Br = 0.f, Bi = 0.f;
+            auto sources = [&](auto mode) {
+                constexpr int MODE = decltype(mode)::value;
+#pragma unroll 2
+                for (int j = j0; j < j1; ++j) {
+                    const float4 U = shu[j];
+                    const float tt = fmaf(tv0, U.x, fmaf(tv1, U.y, fmaf(tv2, U.z, U.w)));  // r^2 - r0^2
+                    const float r2 = r0sq + tt;
+                    const float rinv = rsqrt_t(r2);
+                    const float delta = tt * __builtin_amdgcn_rcpf(fmaf(r2, rinv, r0));
+                    float em1r, em1i, si;  // E - 1, and (E - 1) / delta's imaginary part (series)
+                    efie_em1<MODE>(delta, ser, em1r, em1i, si);
+                    const float D = delta * rinv;
+                    const float q = D * r0inv;  // 1/r0 - 1/r
+                    float fi;
+                    if constexpr (DAMP) {
+                        const float dmp = __expf(-ki * delta), dm1 = expm1f(-ki * delta);
+                        em1r = fmaf(dmp, em1r, dm1);
+                        em1i = dmp * em1i;
+                        fi = em1i * rinv;
+                    } else {
+                        fi = MODE < 3 ? D * si : em1i * rinv;
+                    }
+                    const float fr = fmaf(em1r, rinv, -q);
+                    const float s1 = rinv + r0inv;
+                    const float q1 = q * s1, q3 = q * fmaf(s1, rinv, r0inv2);
+                    // h(r) - h(r0); alpha is imaginary for a real wavenumber
+                    const float dhr = DAMP ? fmaf(al_re, q3, -q1) : -q1, dhi = al_im * q3;
+                    const float hr = h0r + dhr, hi = h0i + dhi;
+                    const float pr = em1r * hr - em1i * hi, pi = em1r * hi + em1i * hr;  // (E - 1) h(r)
+                    const float br = dhr + pr, bi = dhi + pi, cr = hr + pr, ci = hi + pi;
+                    const float4 Q = shq[j], S = shp[j];
+                    Br = fmaf(br, Q.x, fmaf(-bi, Q.y, Br));
+                    Bi = fmaf(br, Q.y, fmaf(bi, Q.x, Bi));
+                    const float ctr = cr * Q.x - ci * Q.y, cti = cr * Q.y + ci * Q.x;
+                    Ar[0] = fmaf(fr, Q.z, fmaf(-fi, Q.w, fmaf(-ctr, U.x, Ar[0])));
+                    Ai[0] = fmaf(fr, Q.w, fmaf(fi, Q.z, fmaf(-cti, U.x, Ai[0])));
+                    Ar[1] = fmaf(fr, S.x, fmaf(-fi, S.y, fmaf(-ctr, U.y, Ar[1])));
+                    Ai[1] = fmaf(fr, S.y, fmaf(fi, S.x, fmaf(-cti, U.y, Ai[1])));
+                    Ar[2] = fmaf(fr, S.z, fmaf(-fi, S.w, fmaf(-ctr, U.z, Ar[2])));
+                    Ai[2] = fmaf(fr, S.w, fmaf(fi, S.z, fmaf(-cti, U.z, Ai[2])));
+                }
+            };
+            // the folded series need k^10 / 10! in FP32 range (|k| < 1000)
+            if (ymax <= 0.24f && kabs < 1000.f)
+                sources(std::integral_constant<int, 0>{});
+            else if (ymax <= 0.49f && kabs < 1000.f)
+                sources(std::integral_constant<int, 1>{});
+            else if (ymax <= 0.98f && kabs < 1000.f)
+                sources(std::integral_constant<int, 2>{});
+            else
+                sources(std::integral_constant<int, 3>{});
+            // group-centre term in FP64, ik T1 / r0 - h(r0) v T2 (times 1/(4 pi)), plus the
+            // FP32 remainder with its v B part, all times e^{ik r0}
+            double ph_s, ph_c;
+            sincos_fast(a.kr * r0d, &ph_s, &ph_c);
+            const double damp0 = DAMP ? exp(-a.ki * r0d) : 1.0;
+            const double pr = ph_c * damp0, pi = ph_s * damp0;
+            const double ikr_d = -a.ki, iki_d = a.kr;
+            const double hr0 = i2 - alr_d * i3, hi0 = -ali_d * i3;
             const double vd[3] = {v0d, v1d, v2d};
-            for (int d = 0; d < KD; ++d) {
-                const double t1r = T[2 + 2 * d], t1i = T[3 + 2 * d];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double t1r = wsum[g][2 + 2 * d], t1i = wsum[g][3 + 2 * d];
                 const double gr = hr0 * vd[d], gi = hi0 * vd[d];
-                const double cr = kInv4Pi * ((ikr_d * t1r - iki_d * t1i) * i0 - (gr * T[0] - gi * T[1]));
-                const double ci = kInv4Pi * ((ikr_d * t1i + iki_d * t1r) * i0 - (gr * T[1] + gi * T[0]));
+                const float vf = (float)vd[d];
+                const double cr = kInv4Pi * ((ikr_d * t1r - iki_d * t1i) * i0 - (gr * wsum[g][0] - gi * wsum[g][1])) +
+                                  (double)(Ar[d] - vf * Br);
+                const double ci = kInv4Pi * ((ikr_d * t1i + iki_d * t1r) * i0 - (gr * wsum[g][1] + gi * wsum[g][0])) +
+                                  (double)(Ai[d] - vf * Bi);
                 accr[d] += pr * cr - pi * ci;
                 acci[d] += pr * ci + pi * cr;
             }
-        }
-        for (int d = 0; d < KD; ++d) {
-            accr[d] += pr * Ar[d] - pi * Ai[d];
-            acci[d] += pr * Ai[d] + pi * Ar[d];
-        }
         }  // groups
     }
     if (m < a.npts) {
-        for (int d = 0; d < KD; ++d) {
-            double *p = a.partial + 2 * (((int64_t)chunk * KD + d) * a.npts + m);
+        for (int d = 0; d < 3; ++d) {
+            double *p = a.partial + 2 * (((int64_t)chunk * 3 + d) * a.npts + m);
             p[0] = accr[d];
             p[1] = acci[d];
         }
@@ -628,7 +727,7 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_fp64(const PotArgs
 // and free of cancellation (e^{i theta} - 1 from half-angle sines), is FP32. The weights cancel
 // over the surface (RWG divergences sum to zero per basis function, sum n c dS over a closed
 // surface vanishes for constant c), so the leading term must not carry FP32 rounding.
-// Magnetic potential: the reference-distance split of potential_eval_mixed, r0 = |x - c| and
+// Magnetic potential: the reference-distance split of potential_eval_mixed_efie, r0 = |x - c| and
 // e^{ik r0} in FP64 per (target, group), r - r0 in FP32 from the small offsets.
 template <int FF>
 __global__ __launch_bounds__(kPotThreads) void far_field_eval_mixed(const PotArgs a) {
@@ -908,8 +1007,12 @@ static int run_potential(PotArgs &a, int precision, int out_dtype, hipStream_t s
         } else if constexpr (OP != kMaxwellEFIE) {
             hipLaunchKernelGGL((potential_eval_mixed_scalar<OP, false>), grid, dim3(kPotThreads), 0, s, a);
         }
-    } else if (precision == BEM_PRECISION_MIXED)
-        hipLaunchKernelGGL((potential_eval_mixed<OP>), grid, dim3(kPotThreads), 0, s, a);
+    } else if (precision == BEM_PRECISION_MIXED) {
+        if (a.ki != 0.0)
+            hipLaunchKernelGGL((potential_eval_mixed_efie<true>), grid, dim3(kPotThreads), 0, s, a);
+        else
+            hipLaunchKernelGGL((potential_eval_mixed_efie<false>), grid, dim3(kPotThreads), 0, s, a);
+    }
     else
         hipLaunchKernelGGL((potential_eval_fp64<OP>), grid, dim3(kPotThreads), 0, s, a);
     if ((rc = check_launch("potential_eval"))) return rc;
@@ -982,6 +1085,7 @@ static int potential_evaluate(int op, const bem_space_t *space, const void *coef
     a.src = (double *)base;
     a.partial = a.src + (size_t)kSrcFields * a.nsrc_pad;
     a.out = out;
+    a.ser = efie_series(a.kr);
     switch (op) {
     case BEM_LAPLACE_SINGLE_LAYER: return run_potential<kLaplaceSL>(a, precision, out_dtype, s);
     case BEM_LAPLACE_DOUBLE_LAYER: return run_potential<kLaplaceDL>(a, precision, out_dtype, s);

@@ -365,7 +365,7 @@ def efie_bench(level, k, reps, with_cpu=False, cpu_seconds=4.0):
             "ms": 1e3 * t_pot,
             "tflops": n_pot * 91 / t_pot / 1e12,
             "rel_err_vs_fp64": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
-            "roofline": roofline("potential_eval_mixed<maxwell_electric_field> (+ source prep and chunk reduce)",
+            "roofline": roofline("potential_eval_mixed_efie (+ source prep and chunk reduce)",
                                  "valu", n_pot * 91 / (pot_ms * 1e-3) / 1e12, PEAK_FP32_TFLOPS, "TFLOP/s",
                                  traffic_key="potential_eval_mixed_efie_65536", evals_per_launch=n_pot,
                                  flop_per_eval=91, launch_ms=pot_ms),
@@ -1075,7 +1075,7 @@ def main():
             "ms": 1e3 * t_pot,
             "tflops": n_pot * FLOP_PER_POTENTIAL_EVAL / t_pot / 1e12,
             "rel_err_vs_fp64": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
-            "roofline": roofline("potential_eval_mixed<helmholtz_single_layer> (+ source prep and chunk reduce)",
+            "roofline": roofline("potential_eval_mixed_scalar<helmholtz_single_layer> (+ source prep and chunk reduce)",
                                  "valu", n_pot * FLOP_PER_POTENTIAL_EVAL / (pot_ms * 1e-3) / 1e12, PEAK_FP32_TFLOPS,
                                  "TFLOP/s", traffic_key="potential_eval_mixed_helmholtz_sl_p1_16384",
                                  evals_per_launch=n_pot, flop_per_eval=FLOP_PER_POTENTIAL_EVAL, launch_ms=pot_ms),
