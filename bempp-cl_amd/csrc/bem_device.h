@@ -94,17 +94,13 @@ __device__ __forceinline__ void sincos_small(double x, double *s, double *c) {
     *c = 1.0 - x2 * pc;
 }
 
-// sin and cos of x (radians): x = n pi / 2 + y with n = rint(2 x / pi) and |y| <= pi / 4 from a
-// three-part Cody-Waite subtraction (pi / 2 split into three doubles; FMA keeps each step to one
-// rounding), the Taylor polynomials of sin y through y^15 and of cos y through y^16 (remainders below
-// 3e-17 at |y| = pi / 4), then the quadrant's swap and signs. Absolute error about 1.4e-16 for
+// sin and cos of x (radians), |x| < 2^26: x = n pi / 2 + y with n = rint(2 x / pi) and |y| <= pi / 4
+// from a three-part Cody-Waite subtraction (pi / 2 split into three doubles; FMA keeps each step to
+// one rounding), the Taylor polynomials of sin y through y^15 and of cos y through y^16 (remainders
+// below 3e-17 at |y| = pi / 4), then the quadrant's swap and signs. Absolute error about 1.4e-16 for
 // |x| <= 1e7 (checked against long double on the host), with half the instructions of the library
-// sincos; |x| >= 2^26 takes the library routine.
-__device__ __forceinline__ void sincos_fast(double x, double *sp, double *cp) {
-    if (!(fabs(x) < 67108864.0)) {
-        sincos(x, sp, cp);
-        return;
-    }
+// sincos and no branch.
+__device__ __forceinline__ void sincos_reduced(double x, double *sp, double *cp) {
     const double n = rint(x * 0.63661977236758134308);
     double y = fma(-n, 1.5707963267948966, x);
     y = fma(-n, 6.123233995736766e-17, y);
@@ -133,21 +129,45 @@ __device__ __forceinline__ void sincos_fast(double x, double *sp, double *cp) {
     *cp = ((q + 1) & 2) ? -C : C;
 }
 
-// SMALL: the caller guarantees |x| <= 1 (FP64 only)
-template <bool SMALL = false>
+// sincos_reduced, with the library routine for |x| >= 2^26
+__device__ __forceinline__ void sincos_fast(double x, double *sp, double *cp) {
+    if (!(fabs(x) < 67108864.0)) {
+        sincos(x, sp, cp);
+        return;
+    }
+    sincos_reduced(x, sp, cp);
+}
+
+// Phase modes of the FP64 Green's functions (template argument PH): kPhSmall, the caller
+// guarantees |k r| <= 1 (Taylor series, sincos_small); kPhReduced, the caller guarantees
+// |k r| < 2^26 (sincos_reduced, no large-argument branch); otherwise sincos_fast. kPhReal may be
+// or-ed in when the wavenumber is real: the e^{-Im(k) r} factor and its branch are left out.
+// kPhShift: the Green's functions return e^{-i ph0} times their value, with the phase k r - ph0
+// (|k r - ph0| <= 1 guaranteed by the caller) from sincos_small; the caller multiplies by e^{i ph0}.
+constexpr int kPhSmall = 1, kPhReduced = 2, kPhReal = 4, kPhShift = 8;
+
+template <int PH = 0>
 __device__ __forceinline__ void sincos_t(double x, double *s, double *c) {
-    if constexpr (SMALL) sincos_small(x, s, c);
+    if constexpr ((PH & 3) == kPhSmall) sincos_small(x, s, c);
+    else if constexpr ((PH & 3) == kPhReduced) sincos_reduced(x, s, c);
     else sincos_fast(x, s, c);
 }
-template <bool SMALL = false>
+template <int PH = 0>
 __device__ __forceinline__ void sincos_t(float x, float *s, float *c) { sincosf(x, s, c); }
+// sin and cos of the phase k r (k r - ph0 with kPhShift)
+template <int PH, typename T>
+__device__ __forceinline__ void phase_t(T kr, T r, T ph0, T *s, T *c) {
+    if constexpr ((PH & kPhShift) && sizeof(T) == 8) sincos_small(fma(kr, r, -ph0), s, c);
+    else sincos_t<PH>(kr * r, s, c);
+}
 __device__ __forceinline__ double exp_t(double x) { return exp(x); }
 __device__ __forceinline__ float exp_t(float x) { return __expf(x); }
 
 // Green's function of operator OP for d = y - x, r2 = |d|^2.
 // nx: test normal (ADL), ny: trial normal (DL). Returns (re, im).
-template <int OP, typename T, bool SMALL = false>
-__device__ __forceinline__ void green(T dx, T dy, T dz, T r2, const T *nx, const T *ny, T kr, T ki, T &gre, T &gim) {
+template <int OP, typename T, int PH = 0>
+__device__ __forceinline__ void green(T dx, T dy, T dz, T r2, const T *nx, const T *ny, T kr, T ki, T &gre, T &gim,
+                                      T ph0 = T(0)) {
     T rinv = rsqrt_t(r2);
     if constexpr (OP == kLaplaceSL) {
         gre = T(kInv4Pi) * rinv;
@@ -163,9 +183,9 @@ __device__ __forceinline__ void green(T dx, T dy, T dz, T r2, const T *nx, const
     } else if constexpr (OP == kHelmholtzSL || OP == kMaxwellEFIE) {
         T r = r2 * rinv;
         T s, c;
-        sincos_t<SMALL>(kr * r, &s, &c);
+        phase_t<PH>(kr, r, ph0, &s, &c);
         T g = T(kInv4Pi) * rinv;
-        if (ki != T(0)) g *= exp_t(-ki * r);
+        if constexpr (!(PH & kPhReal)) { if (ki != T(0)) g *= exp_t(-ki * r); }
         gre = c * g;
         gim = s * g;
     } else {  // Helmholtz DL / ADL
@@ -174,9 +194,9 @@ __device__ __forceinline__ void green(T dx, T dy, T dz, T r2, const T *nx, const
         else dot = -(dx * nx[0] + dy * nx[1] + dz * nx[2]);
         T r = r2 * rinv;
         T s, c;
-        sincos_t<SMALL>(kr * r, &s, &c);
+        phase_t<PH>(kr, r, ph0, &s, &c);
         T g = T(kInv4Pi) * dot * rinv * rinv * rinv;
-        if (ki != T(0)) g *= exp_t(-ki * r);
+        if constexpr (!(PH & kPhReal)) { if (ki != T(0)) g *= exp_t(-ki * r); }
         T fr = c * g, fi = s * g;
         T a = T(-1) - ki * r, b = kr * r;  // i k r - 1 = a + i b
         gre = a * fr - b * fi;
@@ -186,9 +206,9 @@ __device__ __forceinline__ void green(T dx, T dy, T dz, T r2, const T *nx, const
 
 // Single and double layer of one family at the same point pair, sharing r, 1/r and e^{ikr}:
 // (g1re, g1im) = SL, (g2re, g2im) = DL with the trial normal ny.
-template <bool HELM, typename T, bool SMALL = false>
+template <bool HELM, typename T, int PH = 0>
 __device__ __forceinline__ void green_sl_dl(T dx, T dy, T dz, T r2, const T *ny, T kr, T ki, T &g1re, T &g1im,
-                                            T &g2re, T &g2im) {
+                                            T &g2re, T &g2im, T ph0 = T(0)) {
     const T rinv = rsqrt_t(r2);
     const T dot = dx * ny[0] + dy * ny[1] + dz * ny[2];
     if constexpr (!HELM) {
@@ -199,14 +219,14 @@ __device__ __forceinline__ void green_sl_dl(T dx, T dy, T dz, T r2, const T *ny,
     } else {
         const T r = r2 * rinv;
         T s, c;
-        sincos_t<SMALL>(kr * r, &s, &c);
+        phase_t<PH>(kr, r, ph0, &s, &c);
         T g = T(kInv4Pi) * rinv;
-        if (ki != T(0)) g *= exp_t(-ki * r);
+        if constexpr (!(PH & kPhReal)) { if (ki != T(0)) g *= exp_t(-ki * r); }
         g1re = c * g;
         g1im = s * g;
         const T gd = g * dot * rinv * rinv;
         const T fr = c * gd, fi = s * gd;
-        const T a = T(-1) - ki * r, b = kr * r;
+        const T a = (PH & kPhReal) ? T(-1) : T(-1) - ki * r, b = kr * r;
         g2re = a * fr - b * fi;
         g2im = b * fr + a * fi;
     }
@@ -216,9 +236,9 @@ __device__ __forceinline__ void green_sl_dl(T dx, T dy, T dz, T r2, const T *ny,
 // trial side, y on the test side: kernel G'(r) (x - y).n_x / r), all from one distance and one
 // e^{ikr}; (dx, dy, dz) = y - x. The third value is the (f, e) double-layer entry that the
 // symmetric pass computes together with the (e, f) entries.
-template <bool HELM, typename T, bool SMALL = false>
+template <bool HELM, typename T, int PH = 0>
 __device__ __forceinline__ void green_sl_dl_rev(T dx, T dy, T dz, T r2, const T *nx, const T *ny, T kr, T ki, T &g1re,
-                                                T &g1im, T &g2re, T &g2im, T &g3re, T &g3im) {
+                                                T &g1im, T &g2re, T &g2im, T &g3re, T &g3im, T ph0 = T(0)) {
     const T rinv = rsqrt_t(r2);
     const T dot2 = dx * ny[0] + dy * ny[1] + dz * ny[2];
     const T dot3 = -(dx * nx[0] + dy * nx[1] + dz * nx[2]);
@@ -232,14 +252,14 @@ __device__ __forceinline__ void green_sl_dl_rev(T dx, T dy, T dz, T r2, const T 
     } else {
         const T r = r2 * rinv;
         T s, c;
-        sincos_t<SMALL>(kr * r, &s, &c);
+        phase_t<PH>(kr, r, ph0, &s, &c);
         T g = T(kInv4Pi) * rinv;
-        if (ki != T(0)) g *= exp_t(-ki * r);
+        if constexpr (!(PH & kPhReal)) { if (ki != T(0)) g *= exp_t(-ki * r); }
         g1re = c * g;
         g1im = s * g;
         const T gd = g * rinv * rinv;
         const T fr = c * gd, fi = s * gd;
-        const T a = T(-1) - ki * r, b = kr * r;
+        const T a = (PH & kPhReal) ? T(-1) : T(-1) - ki * r, b = kr * r;
         const T hr = a * fr - b * fi, hi = b * fr + a * fi;
         g2re = hr * dot2;
         g2im = hi * dot2;
@@ -250,13 +270,13 @@ __device__ __forceinline__ void green_sl_dl_rev(T dx, T dy, T dz, T r2, const T 
 
 // Green's function of OP and, for DUAL (OP a single layer), also the double layer of the same
 // family at the same point pair (shared r and e^{ikr}).
-template <int OP, bool DUAL, typename T, bool SMALL = false>
+template <int OP, bool DUAL, typename T, int PH = 0>
 __device__ __forceinline__ void green2(T dx, T dy, T dz, T r2, const T *nx, const T *ny, T kr, T ki, T &g1re, T &g1im,
-                                       T &g2re, T &g2im) {
+                                       T &g2re, T &g2im, T ph0 = T(0)) {
     if constexpr (DUAL) {
-        green_sl_dl<OpTraits<OP>::is_complex, T, SMALL>(dx, dy, dz, r2, ny, kr, ki, g1re, g1im, g2re, g2im);
+        green_sl_dl<OpTraits<OP>::is_complex, T, PH>(dx, dy, dz, r2, ny, kr, ki, g1re, g1im, g2re, g2im, ph0);
     } else {
-        green<OP, T, SMALL>(dx, dy, dz, r2, nx, ny, kr, ki, g1re, g1im);
+        green<OP, T, PH>(dx, dy, dz, r2, nx, ny, kr, ki, g1re, g1im, ph0);
         g2re = g2im = T(0);
     }
 }

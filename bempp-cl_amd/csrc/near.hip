@@ -46,7 +46,11 @@ __global__ __launch_bounds__(256) void near_kernel(const NearArgs a) {
     double tnm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
     double snm = a.trial.nmult ? (double)a.trial.nmult[f] : 1.0;
     C2d S[NTS][NSS], S2[NTS][NSS], S3[NTS][NSS];
-    pair_fp64_geom<green_op<OP>(), NTS, NSS, QT, DUAL, SYM>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2, S3);
+    if (CPLX && fp64_phase_reduced(a.kr, a.ki, sqrt(dist2), gt[13] + gf[13]))
+        pair_fp64_geom<green_op<OP>(), NTS, NSS, QT, DUAL, SYM, kPhReduced | kPhReal>(gt, gf, tnm, snm, a.rule, a.kr,
+                                                                                      a.ki, S, S2, S3);
+    else
+        pair_fp64_geom<green_op<OP>(), NTS, NSS, QT, DUAL, SYM>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2, S3);
     // to: 0 adds block (i, j) at (e_i, f_j), 1 at (f_j, e_i), 2 at both
     auto scatter = [&](const C2d(&Sm)[NTS][NSS], auto op_tag, void *out, int to) {
         constexpr int OPX = decltype(op_tag)::value;
@@ -113,8 +117,9 @@ __global__ __launch_bounds__(256) void near_group_kernel(const NearArgs a) {
         centroid_rn(gf, cf);
         if (take && !pair_is_near(ct, cf, gt[13], gf[13], a.near_factor, &dist2)) take = false;
     }
-    const bool small_ok = !take || fabs(a.kr) * (sqrt(dist2) + gt[13] + gf[13]) <= 1.0;
-    const bool small = __all(small_ok);  // wave-uniform
+    const double rmax = sqrt(dist2) + gt[13] + gf[13];  // bounds every point distance of the pair
+    const bool small = __all(!take || fabs(a.kr) * rmax <= 1.0);  // wave-uniform
+    const bool reduced = a.ki == 0.0 && __all(!take || fabs(a.kr) * rmax < 0x1p25);
     if (!take) return;  // whole groups leave together
 
     const double tnm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
@@ -131,8 +136,8 @@ __global__ __launch_bounds__(256) void near_group_kernel(const NearArgs a) {
         const double x0 = gt[0] + gt[3] * xi + gt[6] * eta;
         const double x1 = gt[1] + gt[4] * xi + gt[7] * eta;
         const double x2 = gt[2] + gt[5] * xi + gt[8] * eta;
-        auto points = [&](auto small_tag) {
-            constexpr bool SM = decltype(small_tag)::value;
+        auto points = [&](auto phase_tag) {
+            constexpr int SM = decltype(phase_tag)::value;
             C2d T[NOPS][NSS];
 #pragma unroll
             for (int o = 0; o < NOPS; ++o)
@@ -159,8 +164,10 @@ __global__ __launch_bounds__(256) void near_group_kernel(const NearArgs a) {
 #pragma unroll
             for (int o = 0; o < NOPS; ++o) add_test_moments<NTS, NSS>(S[o], T[o], a.rule.w[p] * gt[12], xi, eta);
         };
-        if (small) points(std::true_type{});
-        else points(std::false_type{});
+        if (small && a.ki == 0.0) points(std::integral_constant<int, kPhSmall | kPhReal>{});
+        else if (small) points(std::integral_constant<int, kPhSmall>{});
+        else if (reduced) points(std::integral_constant<int, kPhReduced | kPhReal>{});
+        else points(std::integral_constant<int, 0>{});
     }
     // sum over the group
 #pragma unroll

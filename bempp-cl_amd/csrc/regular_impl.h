@@ -139,22 +139,30 @@ __device__ __forceinline__ void zero_moments(C2d (&S)[NTS][NSS]) {
 
 // Green's function values of one point pair: OP (g), for DUAL the double layer (g2) and for
 // SYM also the double layer of the reversed pair (g3, test normal nx).
-template <int OP, bool DUAL, bool SYM>
+template <int OP, bool DUAL, bool SYM, int PH = 0>
 __device__ __forceinline__ void green3(double dx, double dy, double dz, double r2, const double *nx, const double *ny,
                                        double kr, double ki, double &gre, double &gim, double &g2re, double &g2im,
-                                       double &g3re, double &g3im) {
+                                       double &g3re, double &g3im, double ph0 = 0.0) {
     if constexpr (SYM) {
-        green_sl_dl_rev<OpTraits<OP>::is_complex, double>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im, g3re,
-                                                          g3im);
+        green_sl_dl_rev<OpTraits<OP>::is_complex, double, PH>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im,
+                                                              g3re, g3im, ph0);
     } else {
-        green2<OP, DUAL, double>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im);
+        green2<OP, DUAL, double, PH>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im, ph0);
         g3re = g3im = 0.0;
     }
 }
 
+// Wave-uniform: the wavenumber is real and |k r| < 2^25 on every point pair of every active lane's
+// element pair (r <= centroid distance + both diameters), so the FP64 Green's functions may take
+// the branch-free phase (kPhReduced | kPhReal), which lets the compiler interleave the point pairs.
+__device__ __forceinline__ bool fp64_phase_reduced(double kr, double ki, double centroid_dist, double diam_sum) {
+    return ki == 0.0 && __all(fabs(kr) * (centroid_dist + diam_sum) < 0x1p25);
+}
+
 // FP64 pair from the geometry table (any rule). DUAL: S2 receives the double layer; SYM: S3
-// receives the double layer of the reversed pair (indexed like S: test side first).
-template <int OP, int NTS, int NSS, int QT, bool DUAL = false, bool SYM = false, typename R = RuleArgs>
+// receives the double layer of the reversed pair (indexed like S: test side first). PH: phase
+// mode of the Green's functions.
+template <int OP, int NTS, int NSS, int QT, bool DUAL = false, bool SYM = false, int PH = 0, typename R = RuleArgs>
 __device__ __forceinline__ void pair_fp64_geom(const double *__restrict__ gt, const double *__restrict__ gf, double tnm,
                                                double snm, const R &rule, double kr, double ki,
                                                C2d (&S)[NTS][NSS], C2d (&S2)[NTS][NSS], C2d (&S3)[NTS][NSS]) {
@@ -182,7 +190,7 @@ __device__ __forceinline__ void pair_fp64_geom(const double *__restrict__ gt, co
             double dz = gf[2] + gf[5] * s0 + gf[8] * s1 - x2;
             double r2 = dx * dx + dy * dy + dz * dz;
             double gre, gim, g2re, g2im, g3re, g3im;
-            green3<OP, DUAL, SYM>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im, g3re, g3im);
+            green3<OP, DUAL, SYM, PH>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im, g3re, g3im);
             const double wq = rule.w[q] * jf;
             add_trial_moments<NSS>(T, gre * wq, gim * wq, s0, s1);
             if constexpr (DUAL) add_trial_moments<NSS>(T2, g2re * wq, g2im * wq, s0, s1);
@@ -195,12 +203,13 @@ __device__ __forceinline__ void pair_fp64_geom(const double *__restrict__ gt, co
 }
 
 // FP64 pair with test points from LDS and trial points from registers. DUAL: S2 = double
-// layer; SYM: S3 = double layer of the reversed pair.
-template <int OP, int NTS, int NSS, int QT, bool DUAL = false, bool SYM = false, typename TT>
+// layer; SYM: S3 = double layer of the reversed pair. PH: phase mode of the Green's functions.
+template <int OP, int NTS, int NSS, int QT, bool DUAL = false, bool SYM = false, int PH = 0, typename TT>
 __device__ __forceinline__ void pair_fp64_regs(const TT &T_, int t, const double *gt, double tnm,
                                                const double (&yq)[QT > 0 ? QT : 1][3], const double (&wq)[QT > 0 ? QT : 1],
                                                const double *ny, const RuleArgs &rule, double kr, double ki,
-                                               C2d (&S)[NTS][NSS], C2d (&S2)[NTS][NSS], C2d (&S3)[NTS][NSS]) {
+                                               C2d (&S)[NTS][NSS], C2d (&S2)[NTS][NSS], C2d (&S3)[NTS][NSS],
+                                               double ph0 = 0.0) {
     double nx[3] = {gt[9] * tnm, gt[10] * tnm, gt[11] * tnm};
     double jt = gt[12];
     zero_moments(S);
@@ -217,7 +226,7 @@ __device__ __forceinline__ void pair_fp64_regs(const TT &T_, int t, const double
             double dx = yq[q][0] - x0, dy = yq[q][1] - x1, dz = yq[q][2] - x2;
             double r2 = dx * dx + dy * dy + dz * dz;
             double gre, gim, g2re, g2im, g3re, g3im;
-            green3<OP, DUAL, SYM>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im, g3re, g3im);
+            green3<OP, DUAL, SYM, PH>(dx, dy, dz, r2, nx, ny, kr, ki, gre, gim, g2re, g2im, g3re, g3im, ph0);
             add_trial_moments<NSS>(T, gre * wq[q], gim * wq[q], rule.x[q], rule.y[q]);
             if constexpr (DUAL) add_trial_moments<NSS>(T2, g2re * wq[q], g2im * wq[q], rule.x[q], rule.y[q]);
             if constexpr (SYM) add_trial_moments<NSS>(T3, g3re * wq[q], g3im * wq[q], rule.x[q], rule.y[q]);
@@ -761,10 +770,11 @@ __device__ __forceinline__ void store_direct(void *out, int64_t ld, int row, int
 // at (e, f) and the double layer of the reversed pair (test normal) to a.out2 at (f, e) (REV).
 // waves per SIMD the launch bound asks for: the FP32 fused pass and the symmetric single layer
 // fit 3; the symmetric single + double layer pass and the mixed EFIE pass keep more state (2);
-// the others leave it to the compiler
+// the single-operator FP64 passes, ordered or symmetric, 2 (their few spills lie outside the
+// point-pair loops); the others leave it to the compiler
 template <int OP, bool MIXED, bool DUAL, bool SYM>
 constexpr int regular_waves() {
-    return !MIXED ? 1 : SYM ? (DUAL ? 2 : 3) : DUAL ? 3 : is_rwg_op<OP>() ? 2 : 1;
+    return !MIXED ? (DUAL ? 1 : 2) : SYM ? (DUAL ? 2 : 3) : DUAL ? 3 : is_rwg_op<OP>() ? 2 : 1;
 }
 
 // UPPER (direct DP0 stores, a symmetric operator on one space): only the pairs whose trial element
@@ -1077,11 +1087,57 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
             if (valid && !adj) {
                 if constexpr (!MIXED) {
                     double tnm = a.test.nmult ? (double)a.test.nmult[e] : 1.0;
-                    if constexpr (QT > 0)
-                        pair_fp64_regs<GOP, NTS, NSS, QT, DUAL, REV>(T_, te, gt, tnm, yq, wq64, ny64,
-                                                                                      a.rule, a.kr, a.ki, S, S2, S3);
-                    else
-                        pair_fp64_geom<GOP, NTS, NSS, QT, DUAL, REV>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2, S3);
+                    bool fast = false, shift = false;
+                    double R0 = 0.0;
+                    if constexpr (OpTraits<GOP>::is_complex) {
+                        const double c0 = cf0 - T_.c[te][0], c1 = cf1 - T_.c[te][1], c2 = cf2 - T_.c[te][2];
+                        R0 = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+                        fast = fp64_phase_reduced(a.kr, a.ki, R0, T_.diam[te] + diam_f);
+                        // |k (r - R0)| <= k (diam_e + diam_f) <= 1 on every lane: e^{ikr} = e^{ik R0} e^{ik (r - R0)}
+                        // with the second factor from its Taylor series, the first once per pair
+                        shift = fast && __all(fabs(a.kr) * (T_.diam[te] + diam_f) <= 1.0);
+                    }
+                    if constexpr (QT > 0 && OpTraits<GOP>::is_complex) {
+                        if (shift) {
+                            const double ph0 = a.kr * R0;
+                            pair_fp64_regs<GOP, NTS, NSS, QT, DUAL, REV, kPhShift | kPhReal>(
+                                T_, te, gt, tnm, yq, wq64, ny64, a.rule, a.kr, a.ki, S, S2, S3, ph0);
+                            double ps, pc;
+                            sincos_reduced(ph0, &ps, &pc);
+                            auto rotate = [&](C2d(&Sm)[NTS][NSS]) {
+#pragma unroll
+                                for (int i = 0; i < NTS; ++i)
+#pragma unroll
+                                    for (int j = 0; j < NSS; ++j) {
+                                        const double re = Sm[i][j].re, im = Sm[i][j].im;
+                                        Sm[i][j].re = re * pc - im * ps;
+                                        Sm[i][j].im = re * ps + im * pc;
+                                    }
+                            };
+                            rotate(S);
+                            if constexpr (DUAL) rotate(S2);
+                            if constexpr (REV) rotate(S3);
+                        } else if (fast)
+                            pair_fp64_regs<GOP, NTS, NSS, QT, DUAL, REV, kPhReduced | kPhReal>(
+                                T_, te, gt, tnm, yq, wq64, ny64, a.rule, a.kr, a.ki, S, S2, S3);
+                        else
+                            pair_fp64_regs<GOP, NTS, NSS, QT, DUAL, REV>(T_, te, gt, tnm, yq, wq64, ny64, a.rule, a.kr,
+                                                                         a.ki, S, S2, S3);
+                    } else if constexpr (QT > 0) {
+                        if (fast)
+                            pair_fp64_regs<GOP, NTS, NSS, QT, DUAL, REV, kPhReduced | kPhReal>(
+                                T_, te, gt, tnm, yq, wq64, ny64, a.rule, a.kr, a.ki, S, S2, S3);
+                        else
+                            pair_fp64_regs<GOP, NTS, NSS, QT, DUAL, REV>(T_, te, gt, tnm, yq, wq64, ny64, a.rule, a.kr,
+                                                                         a.ki, S, S2, S3);
+                    } else {
+                        if (fast)
+                            pair_fp64_geom<GOP, NTS, NSS, QT, DUAL, REV, kPhReduced | kPhReal>(
+                                gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2, S3);
+                        else
+                            pair_fp64_geom<GOP, NTS, NSS, QT, DUAL, REV>(gt, gf, tnm, snm, a.rule, a.kr, a.ki, S, S2,
+                                                                         S3);
+                    }
                 } else {
                     const double cfv[3] = {cf0, cf1, cf2};
                     double dist2;

@@ -65,9 +65,10 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
         }
 
     // every distance within the pair is at most diam_t + diam_f (triangles sharing a vertex), so a
-    // wave-uniform bound selects the series sin/cos without range reduction when k r <= 1
-    auto points = [&](auto small_tag) {
-        constexpr bool SMALL = decltype(small_tag)::value;
+    // wave-uniform bound selects the phase mode: the series sin/cos without range reduction when
+    // k r <= 1, else (real wavenumbers) the branch-free reduction
+    auto points = [&](auto phase_tag) {
+        constexpr int PH = decltype(phase_tag)::value;
         for (int k = lane; k < np; k += kSingGroup) {
             double xi0 = a.tpts[2 * (int64_t)(toff + k)], xi1 = a.tpts[2 * (int64_t)(toff + k) + 1];
             double et0 = a.spts[2 * (int64_t)(soff + k)], et1 = a.spts[2 * (int64_t)(soff + k) + 1];
@@ -77,7 +78,7 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
             double dz = (gf[2] + gf[5] * et0 + gf[8] * et1) - (gt[2] + gt[5] * xi0 + gt[8] * xi1);
             double r2 = dx * dx + dy * dy + dz * dz;
             double gre, gim, g2re, g2im;
-            green2<green_op<OP>(), DUAL, double, SMALL>(dx, dy, dz, r2, nx, ny, a.kr, a.ki, gre, gim, g2re, g2im);
+            green2<green_op<OP>(), DUAL, double, PH>(dx, dy, dz, r2, nx, ny, a.kr, a.ki, gre, gim, g2re, g2im);
             gre *= w;
             gim *= w;
             g2re *= w;
@@ -97,8 +98,11 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
                 }
         }
     };
-    if (fabs(a.kr) * (gt[13] + gf[13]) <= 1.0) points(std::true_type{});
-    else points(std::false_type{});
+    const double kd = fabs(a.kr) * (gt[13] + gf[13]);
+    if (kd <= 1.0 && a.ki == 0.0) points(std::integral_constant<int, kPhSmall | kPhReal>{});
+    else if (kd <= 1.0) points(std::integral_constant<int, kPhSmall>{});
+    else if (kd < 0x1p25 && a.ki == 0.0) points(std::integral_constant<int, kPhReduced | kPhReal>{});
+    else points(std::integral_constant<int, 0>{});
     const double jj = gt[12] * gf[12];
     C2d S[NTS][NSS], S2[NTS][NSS];
 #pragma unroll
