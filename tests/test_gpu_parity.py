@@ -315,11 +315,13 @@ def test_small_kr_singular_series_vs_oracle(k, default_parameters, hip_available
         assert rel_fro(A, ref) < 1e-12
 
 
-@pytest.mark.parametrize("k", [2.0, 0.5])
+@pytest.mark.parametrize("k", [2.0, 0.5, 8.0])
 @pytest.mark.parametrize("density", ["random", "smooth"])
 def test_efie_potential_far_mixed(k, density, default_parameters, hip_available):
     """E-field potential at radius 10: the divergence term of RWG densities cancels in pairs
-    (each basis function has zero total divergence), which the mixed kernel must not amplify."""
+    (each basis function has zero total divergence), which the mixed kernel must not amplify.
+    k = 0.5, 2 and 8 on level 3 take the short, long and per-source (half-angle sine beyond
+    |k delta| = 1) series for e^{ik delta} - 1."""
     import bempp_amd.api as bempp
     from bempp_amd.api.operators.potential import maxwell as mpot
 

@@ -64,13 +64,15 @@ def test_symmetric_equals_ordered(case, level, default_parameters, hip_available
     assert rel(m2, o2) < 1e-6
 
 
-def test_symmetric_fp64_vs_oracle(default_parameters, hip_available):
+# k = 4: k (diam_e + diam_f) > 1 on most far pairs (the branch-free reduced phase);
+# k = 1.5: below 1 on every pair (e^{ik R0} per pair times the Taylor series of e^{ik (r - R0)})
+@pytest.mark.parametrize("k", [4.0, 1.5])
+def test_symmetric_fp64_vs_oracle(k, default_parameters, hip_available):
     from bempp_amd.api import function_space
     from oracle import oracle as O
 
     grid = perturbed_sphere(3)
     p1 = function_space(grid, "P", 1)
-    k = 4.0
     a1, a2 = fused(p1, "helmholtz", k, "double", True, default_parameters)
     for name, a in (("helmholtz_single_layer", a1), ("helmholtz_double_layer", a2)):
         ref = O.assemble(name, grid.vertices, grid.elements, p1, p1, k)
