@@ -146,6 +146,13 @@ __device__ __forceinline__ void sincos_fast(double x, double *sp, double *cp) {
 // (|k r - ph0| <= 1 guaranteed by the caller) from sincos_small; the caller multiplies by e^{i ph0}.
 constexpr int kPhSmall = 1, kPhReduced = 2, kPhReal = 4, kPhShift = 8;
 
+// Wave-uniform: the wavenumber is real and |k r| < 2^25 on every point pair of every active lane's
+// element pair (r <= centroid distance + both diameters), so the FP64 Green's functions may take
+// the branch-free phase (kPhReduced | kPhReal), which lets the compiler interleave the point pairs.
+__device__ __forceinline__ bool fp64_phase_reduced(double kr, double ki, double centroid_dist, double diam_sum) {
+    return ki == 0.0 && __all(fabs(kr) * (centroid_dist + diam_sum) < 0x1p25);
+}
+
 template <int PH = 0>
 __device__ __forceinline__ void sincos_t(double x, double *s, double *c) {
     if constexpr ((PH & 3) == kPhSmall) sincos_small(x, s, c);

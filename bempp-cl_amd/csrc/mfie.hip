@@ -175,9 +175,14 @@ __global__ __launch_bounds__(kThreads) void mfie_regular_kernel(const Args a) {
         }
         const double fd[3] = {cf[0] - T_.c[t][0], cf[1] - T_.c[t][1], cf[2] - T_.c[t][2]};
         double ar[3][3], ai[3][3];
-        pair_block_fp64([&](int p, double (&ap)[3]) {
+        auto ap_of = [&](int p, double (&ap)[3]) {
             for (int d = 0; d < 3; ++d) ap[d] = T_.a[t][p][d];
-        }, T_.e1[t], T_.e2[t], b, kap, fd, a.rule, a.kr, a.ki, ar, ai);
+        };
+        const double fdn = sqrt(fd[0] * fd[0] + fd[1] * fd[1] + fd[2] * fd[2]);
+        if (fp64_phase_reduced(a.kr, a.ki, fdn, T_.diam[t] + gf[13]))
+            pair_block_fp64<kPhReduced | kPhReal>(ap_of, T_.e1[t], T_.e2[t], b, kap, fd, a.rule, a.kr, a.ki, ar, ai);
+        else
+            pair_block_fp64(ap_of, T_.e1[t], T_.e2[t], b, kap, fd, a.rule, a.kr, a.ki, ar, ai);
         scatter_block<OutT>(a, T_.dof[t], T_.ml[t], fdof, fml, ar, ai);
     }
 }
@@ -207,9 +212,13 @@ __global__ __launch_bounds__(kThreads) void mfie_near_pairs_kernel(const Args a)
     rwg_offsets(gf + 3, gf + 6, kap);
     const double fd[3] = {cf[0] - ct[0], cf[1] - ct[1], cf[2] - ct[2]};
     double ar[3][3], ai[3][3];
-    pair_block_fp64([&](int q, double (&ap)[3]) {
+    auto ap_of = [&](int q, double (&ap)[3]) {
         for (int d = 0; d < 3; ++d) ap[d] = at[q][d];
-    }, gt + 3, gt + 6, b, kap, fd, a.rule, a.kr, a.ki, ar, ai);
+    };
+    if (fp64_phase_reduced(a.kr, a.ki, sqrt(dist2), gt[13] + gf[13]))
+        pair_block_fp64<kPhReduced | kPhReal>(ap_of, gt + 3, gt + 6, b, kap, fd, a.rule, a.kr, a.ki, ar, ai);
+    else
+        pair_block_fp64(ap_of, gt + 3, gt + 6, b, kap, fd, a.rule, a.kr, a.ki, ar, ai);
     int tdof[3], fdof[3];
     double tml[3], fml[3];
     dof_factors(a.test, e, tdof, tml);

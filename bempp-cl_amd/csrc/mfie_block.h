@@ -122,7 +122,9 @@ __device__ __forceinline__ void edge_lengths(const double *g, double (&l)[3]) {
 // Jacobians cancel the Piola maps): ap(p) the offsets of the test points from the test centroid,
 // e1 / e2 the test edge vectors, b the trial offsets, kap c_f - Q_j of the trial element,
 // fd = c_f - c_e.
-template <int Q, typename AP>
+// PH: phase mode of e^{ikr} (kPhReduced | kPhReal when the caller has bounded |k r| on the wave's
+// pairs and the wavenumber is real).
+template <int PH = 0, int Q, typename AP>
 __device__ __forceinline__ void pair_block_fp64(AP ap_of, const double *e1, const double *e2, const double (&b)[Q][3],
                                                 const double (&kap)[3][3], const double (&fd)[3], const RuleArgs &rule,
                                                 double kr, double ki, double (&ar)[3][3], double (&ai)[3][3]) {
@@ -138,12 +140,12 @@ __device__ __forceinline__ void pair_block_fp64(AP ap_of, const double *e1, cons
         for (int q = 0; q < Q; ++q) {
             const double u0 = d0 - b[q][0], u1 = d1 - b[q][1], u2 = d2 - b[q][2];  // x_p - y_q
             const double r2 = u0 * u0 + u1 * u1 + u2 * u2;
-            const double r = sqrt(r2);
+            const double rinv = rsqrt_t(r2), r = r2 * rinv;
             double sn, cs;
-            sincos_fast(kr * r, &sn, &cs);
-            const double damp = ki != 0.0 ? exp(-ki * r) : 1.0;
-            const double g = rule.w[p] * rule.w[q] * kInv4Pi * damp / (r * r2);
-            const double fr = -ki * r - 1.0, fi = kr * r;
+            sincos_t<PH>(kr * r, &sn, &cs);
+            const double damp = (PH & kPhReal) ? 1.0 : ki != 0.0 ? exp(-ki * r) : 1.0;
+            const double g = rule.w[p] * rule.w[q] * kInv4Pi * damp * (rinv * rinv * rinv);
+            const double fr = (PH & kPhReal) ? -1.0 : -ki * r - 1.0, fi = kr * r;
             const double vr = g * (cs * fr - sn * fi), vi = g * (cs * fi + sn * fr);
             Rp[0] += vr;
             Rp[1] += vi;

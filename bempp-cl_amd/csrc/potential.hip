@@ -130,65 +130,93 @@ __global__ __launch_bounds__(256) void potential_prep_kernel(const PotArgs a) {
     }
 }
 
-// FP64 evaluation
+// Largest |y|^2 over the staged sources of a tile (0 for lanes without a source): per-wave
+// maximum by shuffles into wmax[wave]; the caller's barrier publishes it.
+__device__ __forceinline__ void tile_extent_stage(double y2, double *wmax) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) y2 = fmax(y2, __shfl_xor(y2, off, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = y2;
+}
+
+// Wave-uniform: a real wavenumber and |k| (|x| + max |y|) < 2^25 for every active lane's target x
+// over the tile's sources, so every phase |k r| (r <= |x| + |y|) or |k xh.y| takes the branch-free
+// reduction (kPhReduced | kPhReal).
+__device__ __forceinline__ bool tile_phase_reduced(double kr, double ki, double xnorm, const double *wmax) {
+    double y2 = wmax[0];
+#pragma unroll
+    for (int w = 1; w < kPotThreads / 64; ++w) y2 = fmax(y2, wmax[w]);
+    return ki == 0.0 && __all(fabs(kr) * (xnorm + sqrt(y2)) < 0x1p25);
+}
+
+// FP64 evaluation. EFIE: term_d = ik t1_d - (x - y)_d (ik r - 1) t2 / (ik r^2) times G, with
+// (ik r - 1) / (ik r^2) = 1/r - alpha / r^2, alpha = 1 / (ik).
 template <int OP>
 __global__ __launch_bounds__(kPotThreads) void potential_eval_fp64(const PotArgs a) {
     constexpr int KD = OP == kMaxwellEFIE ? 3 : 1;
     __shared__ double sh[kSrcFields][kPotTile];
+    __shared__ double wmax[kPotThreads / 64];
     const int m = blockIdx.x * kPotThreads + threadIdx.x;
     const int chunk = blockIdx.y;
     const int s0 = chunk * a.chunk_len;
-    const int s1 = min(s0 + a.chunk_len, a.nsrc_pad);
+    const int s1 = min(s0 + a.chunk_len, a.nsrc);
     double x0 = 0, x1 = 0, x2 = 0;
     if (m < a.npts) {
         x0 = a.pts[3 * (int64_t)m];
         x1 = a.pts[3 * (int64_t)m + 1];
         x2 = a.pts[3 * (int64_t)m + 2];
     }
+    const double xnorm = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
     double accr[KD], acci[KD];
     for (int d = 0; d < KD; ++d) accr[d] = acci[d] = 0.0;
     const double kr = a.kr, ki = a.ki;
+    const double kk = kr * kr + ki * ki;
+    const double alr = kk > 0.0 ? -ki / kk : 0.0, ali = kk > 0.0 ? -kr / kk : 0.0;  // alpha = 1 / (ik)
     const double zero3[3] = {0.0, 0.0, 0.0};
     for (int t0 = s0; t0 < s1; t0 += kPotTile) {
         __syncthreads();
         {
-            int s = t0 + threadIdx.x;
+            const int s = t0 + threadIdx.x;
             const int nf = OP == kMaxwellEFIE ? kSrcFields : 8;
-            for (int k = 0; k < nf; ++k) sh[k][threadIdx.x] = s < s1 ? a.src[(int64_t)k * a.nsrc_pad + s] : (k == 0 ? 1e30 : 0.0);
+            for (int k = 0; k < nf; ++k) sh[k][threadIdx.x] = s < s1 ? a.src[(int64_t)k * a.nsrc_pad + s] : 0.0;
+            const double y0 = sh[0][threadIdx.x], y1 = sh[1][threadIdx.x], y2 = sh[2][threadIdx.x];
+            tile_extent_stage(y0 * y0 + y1 * y1 + y2 * y2, wmax);
         }
         __syncthreads();
         const int nt = min(kPotTile, s1 - t0);
-        for (int j = 0; j < nt; ++j) {
-            double dx = sh[0][j] - x0, dy = sh[1][j] - x1, dz = sh[2][j] - x2;
-            double r2 = dx * dx + dy * dy + dz * dz;
-            double ny[3] = {sh[3][j], sh[4][j], sh[5][j]};
-            double gre, gim;
-            green<OP == kMaxwellEFIE ? kHelmholtzSL : OP, double>(dx, dy, dz, r2, zero3, ny, kr, ki, gre, gim);
-            if constexpr (OP == kMaxwellEFIE) {
-                // term_d = ik t1_d - (x - y)_d (ik r - 1) t2 / (ik r^2), x - y = -d
-                double r = sqrt(r2);
-                double ikr_re = -ki, ikr_im = kr;             // ik
-                double nr = ikr_re * r - 1.0, ni = ikr_im * r;  // ik r - 1
-                double dr = ikr_re * r2, di = ikr_im * r2;      // ik r^2
-                double den = dr * dr + di * di;
-                double qr = (nr * dr + ni * di) / den, qi = (ni * dr - nr * di) / den;
-                double t2r = sh[6][j], t2i = sh[7][j];
-                double qtr = qr * t2r - qi * t2i, qti = qr * t2i + qi * t2r;
-                double diffv[3] = {-dx, -dy, -dz};
+        auto sources = [&](auto ph_tag) {
+            constexpr int PH = decltype(ph_tag)::value;
+            for (int j = 0; j < nt; ++j) {
+                double dx = sh[0][j] - x0, dy = sh[1][j] - x1, dz = sh[2][j] - x2;
+                double r2 = dx * dx + dy * dy + dz * dz;
+                double ny[3] = {sh[3][j], sh[4][j], sh[5][j]};
+                double gre, gim;
+                green<OP == kMaxwellEFIE ? kHelmholtzSL : OP, double, PH>(dx, dy, dz, r2, zero3, ny, kr, ki, gre, gim);
+                if constexpr (OP == kMaxwellEFIE) {
+                    const double rinv = rsqrt_t(r2), rinv2 = rinv * rinv;
+                    const double qr = rinv - alr * rinv2, qi = -ali * rinv2;  // (ik r - 1) / (ik r^2)
+                    const double t2r = sh[6][j], t2i = sh[7][j];
+                    const double qtr = qr * t2r - qi * t2i, qti = qr * t2i + qi * t2r;
+                    const double diffv[3] = {-dx, -dy, -dz};
+                    const double ikr_re = -ki, ikr_im = kr;  // ik
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    double t1r = sh[8 + 2 * d][j], t1i = sh[9 + 2 * d][j];
-                    double tr = (ikr_re * t1r - ikr_im * t1i) - diffv[d] * qtr;
-                    double ti = (ikr_re * t1i + ikr_im * t1r) - diffv[d] * qti;
-                    accr[d] += gre * tr - gim * ti;
-                    acci[d] += gre * ti + gim * tr;
+                    for (int d = 0; d < 3; ++d) {
+                        const double t1r = sh[8 + 2 * d][j], t1i = sh[9 + 2 * d][j];
+                        const double tr = (ikr_re * t1r - ikr_im * t1i) - diffv[d] * qtr;
+                        const double ti = (ikr_re * t1i + ikr_im * t1r) - diffv[d] * qti;
+                        accr[d] += gre * tr - gim * ti;
+                        acci[d] += gre * ti + gim * tr;
+                    }
+                } else {
+                    double cr = sh[6][j], ci = sh[7][j];
+                    accr[0] += gre * cr - gim * ci;
+                    acci[0] += gre * ci + gim * cr;
                 }
-            } else {
-                double cr = sh[6][j], ci = sh[7][j];
-                accr[0] += gre * cr - gim * ci;
-                acci[0] += gre * ci + gim * cr;
             }
-        }
+        };
+        if (OpTraits<OP>::is_complex && tile_phase_reduced(kr, ki, xnorm, wmax))
+            sources(std::integral_constant<int, kPhReduced | kPhReal>{});
+        else
+            sources(std::integral_constant<int, 0>{});
     }
     if (m < a.npts) {
         for (int d = 0; d < KD; ++d) {
@@ -619,10 +647,11 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_fp64(const PotArgs
     constexpr int KD = MAXWELL ? 3 : 1;
     constexpr int NF = MAXWELL ? kSrcFields : 8;
     __shared__ double sh[NF][kPotTile];
+    __shared__ double wmax[kPotThreads / 64];
     const int m = blockIdx.x * kPotThreads + threadIdx.x;
     const int chunk = blockIdx.y;
     const int s0 = chunk * a.chunk_len;
-    const int s1 = min(s0 + a.chunk_len, a.nsrc_pad);
+    const int s1 = min(s0 + a.chunk_len, a.nsrc);
     double x0 = 0, x1 = 0, x2 = 0;
     if (m < a.npts) {
         x0 = a.pts[3 * (int64_t)m];
@@ -630,6 +659,7 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_fp64(const PotArgs
         x2 = a.pts[3 * (int64_t)m + 2];
     }
     const double xh[3] = {x0, x1, x2};
+    const double xnorm = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
     double accr[KD], acci[KD];
     for (int d = 0; d < KD; ++d) accr[d] = acci[d] = 0.0;
     const double kr = a.kr, ki = a.ki;
@@ -637,19 +667,21 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_fp64(const PotArgs
         __syncthreads();
         {
             const int s = t0 + threadIdx.x;
-            // sources past the chunk: far away with zero weights (r > 0 for every target)
-            for (int k = 0; k < NF; ++k)
-                sh[k][threadIdx.x] = s < s1 ? a.src[(int64_t)k * a.nsrc_pad + s] : (k == 0 ? 1e30 : 0.0);
+            for (int k = 0; k < NF; ++k) sh[k][threadIdx.x] = s < s1 ? a.src[(int64_t)k * a.nsrc_pad + s] : 0.0;
+            const double y0 = sh[0][threadIdx.x], y1 = sh[1][threadIdx.x], y2 = sh[2][threadIdx.x];
+            tile_extent_stage(y0 * y0 + y1 * y1 + y2 * y2, wmax);
         }
         __syncthreads();
         const int nt = min(kPotTile, s1 - t0);
+        auto sources = [&](auto ph_tag) {
+        constexpr int PH = decltype(ph_tag)::value;
         for (int j = 0; j < nt; ++j) {
             if constexpr (FF == kMagneticPotential) {
                 const double d0 = x0 - sh[0][j], d1 = x1 - sh[1][j], d2 = x2 - sh[2][j];
                 const double r2 = d0 * d0 + d1 * d1 + d2 * d2, r = sqrt(r2);
                 double sn, cs;
-                sincos_fast(kr * r, &sn, &cs);
-                const double damp = ki != 0.0 ? exp(-ki * r) : 1.0;
+                sincos_t<PH>(kr * r, &sn, &cs);
+                const double damp = (PH & kPhReal) ? 1.0 : ki != 0.0 ? exp(-ki * r) : 1.0;
                 const double g = kInv4Pi * damp / (r * r2);
                 const double fr = -ki * r - 1.0, fi = kr * r;  // ik r - 1
                 const double vr = g * (cs * fr - sn * fi), vi = g * (cs * fi + sn * fr);
@@ -668,10 +700,9 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_fp64(const PotArgs
                 acci[2] += d0 * wi[1] - d1 * wi[0];
                 continue;
             }
-            // padding sources sit at 1e30 with zero weights: the phase stays finite
             const double dot = x0 * sh[0][j] + x1 * sh[1][j] + x2 * sh[2][j];
             double sn, cs;
-            sincos_fast(-kr * dot, &sn, &cs);
+            sincos_t<PH>(-kr * dot, &sn, &cs);
             const double gre = kInv4Pi * cs, gim = kInv4Pi * sn;
             if constexpr (FF == kFarSL || FF == kFarDL) {
                 double vr = gre, vi = gim;
@@ -699,6 +730,11 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_fp64(const PotArgs
                 }
             }
         }
+        };
+        if (tile_phase_reduced(kr, ki, xnorm, wmax))
+            sources(std::integral_constant<int, kPhReduced | kPhReal>{});
+        else
+            sources(std::integral_constant<int, 0>{});
     }
     if (m < a.npts) {
         if constexpr (FF == kFarH) {

@@ -152,13 +152,6 @@ __device__ __forceinline__ void green3(double dx, double dy, double dz, double r
     }
 }
 
-// Wave-uniform: the wavenumber is real and |k r| < 2^25 on every point pair of every active lane's
-// element pair (r <= centroid distance + both diameters), so the FP64 Green's functions may take
-// the branch-free phase (kPhReduced | kPhReal), which lets the compiler interleave the point pairs.
-__device__ __forceinline__ bool fp64_phase_reduced(double kr, double ki, double centroid_dist, double diam_sum) {
-    return ki == 0.0 && __all(fabs(kr) * (centroid_dist + diam_sum) < 0x1p25);
-}
-
 // FP64 pair from the geometry table (any rule). DUAL: S2 receives the double layer; SYM: S3
 // receives the double layer of the reversed pair (indexed like S: test side first). PH: phase
 // mode of the Green's functions.
