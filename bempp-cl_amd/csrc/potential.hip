@@ -291,7 +291,7 @@ __device__ __forceinline__ void efie_em1(float delta, const EfieSeries &e, float
 }
 
 // MIXED EFIE potential: FP32 per source, FP64 per (target, group of 32 sources, a few
-// neighbouring elements). With c the centre of the group's bounding box, v = x - c, u = c - y
+// neighbouring elements). With c the group's middle source, v = x - c, u = c - y
 // (so x - y = v + u), r0 = |v|, delta = r - r0, E = e^{ik delta}, h(r) = 1/r^2 - alpha/r^3
 // (alpha = 1/(ik)), one source contributes e^{ik r0} / (4 pi) [ ik (E / r) t1 - E h(r) (v + u) t2 ].
 // It is split into
@@ -340,20 +340,15 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed_efie(const P
         {
             const int j = threadIdx.x, g = j / GS, s = t0 + j;
             const bool ok = j < nt;
-            // bounding box of the group's sources (lanes g*GS .. g*GS+GS-1 of one wave)
-            double y[3], lo[3], hi[3];
+            // reference point of the group: its middle source (a real source, so r0 > 0 for every
+            // target off the sources)
+            const int cidx = t0 + min(g * GS + GS / 2, nt - 1);
+            double y[3], c[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 y[d] = ok ? a.src[(int64_t)d * P + s] : 0.0;
-                lo[d] = ok ? y[d] : INFINITY;
-                hi[d] = ok ? y[d] : -INFINITY;
-#pragma unroll
-                for (int off = GS / 2; off >= 1; off >>= 1) {
-                    lo[d] = fmin(lo[d], __shfl_xor(lo[d], off, 64));
-                    hi[d] = fmax(hi[d], __shfl_xor(hi[d], off, 64));
-                }
+                c[d] = a.src[(int64_t)d * P + cidx];
             }
-            const double c[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
             if (j % GS == 0)
                 for (int d = 0; d < 3; ++d) sc[g][d] = c[d];
             double w[8];
