@@ -129,10 +129,14 @@ __device__ __forceinline__ void sincos_reduced(double x, double *sp, double *cp)
     *cp = ((q + 1) & 2) ? -C : C;
 }
 
+// the library sincos out of line: its large-argument reduction needs many registers, which an
+// inlined copy would claim in every kernel that merely may take it
+__device__ __noinline__ void sincos_large(double x, double *sp, double *cp) { sincos(x, sp, cp); }
+
 // sincos_reduced, with the library routine for |x| >= 2^26
 __device__ __forceinline__ void sincos_fast(double x, double *sp, double *cp) {
     if (!(fabs(x) < 67108864.0)) {
-        sincos(x, sp, cp);
+        sincos_large(x, sp, cp);
         return;
     }
     sincos_reduced(x, sp, cp);

@@ -148,8 +148,66 @@ __device__ __forceinline__ bool tile_phase_reduced(double kr, double ki, double 
     return ki == 0.0 && __all(fabs(kr) * (xnorm + sqrt(y2)) < 0x1p25);
 }
 
-// FP64 evaluation. EFIE: term_d = ik t1_d - (x - y)_d (ik r - 1) t2 / (ik r^2) times G, with
-// (ik r - 1) / (ik r^2) = 1/r - alpha / r^2, alpha = 1 / (ik).
+// One source's FP64 contribution to the potential at x (the body of potential_eval_fp64, also the
+// mixed kernels' path for targets close to a tile's sources). field(k) returns field k of the
+// source record (px py pz nx ny nz, c or t2, EFIE t1). EFIE: term_d = ik t1_d - (x - y)_d
+// (ik r - 1) t2 / (ik r^2) times G, with (ik r - 1) / (ik r^2) = 1/r - alpha / r^2, alpha = 1 / (ik).
+template <int OP, int PH, typename F, int KD>
+__device__ __forceinline__ void fp64_source(F field, double x0, double x1, double x2, double kr, double ki, double alr,
+                                            double ali, double (&accr)[KD], double (&acci)[KD]) {
+    const double zero3[3] = {0.0, 0.0, 0.0};
+    const double dx = field(0) - x0, dy = field(1) - x1, dz = field(2) - x2;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double ny[3] = {field(3), field(4), field(5)};
+    double gre, gim;
+    green<OP == kMaxwellEFIE ? kHelmholtzSL : OP, double, PH>(dx, dy, dz, r2, zero3, ny, kr, ki, gre, gim);
+    if constexpr (OP == kMaxwellEFIE) {
+        const double rinv = rsqrt_t(r2), rinv2 = rinv * rinv;
+        const double qr = rinv - alr * rinv2, qi = -ali * rinv2;  // (ik r - 1) / (ik r^2)
+        const double t2r = field(6), t2i = field(7);
+        const double qtr = qr * t2r - qi * t2i, qti = qr * t2i + qi * t2r;
+        const double diffv[3] = {-dx, -dy, -dz};
+        const double ikr_re = -ki, ikr_im = kr;  // ik
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double t1r = field(8 + 2 * d), t1i = field(9 + 2 * d);
+            const double tr = (ikr_re * t1r - ikr_im * t1i) - diffv[d] * qtr;
+            const double ti = (ikr_re * t1i + ikr_im * t1r) - diffv[d] * qti;
+            accr[d] += gre * tr - gim * ti;
+            acci[d] += gre * ti + gim * tr;
+        }
+    } else {
+        const double cr = field(6), ci = field(7);
+        accr[0] += gre * cr - gim * ci;
+        acci[0] += gre * ci + gim * cr;
+    }
+}
+
+// Mixed kernels: a (target, tile or group) pair whose reference distance r0 is below
+// kNearTileFactor times the radius R of the tile's sources around the reference takes the FP64
+// path: the FP32 offsets lose relative accuracy once the target comes within a fraction of R of
+// the sources (targets just off the surface), while targets beyond r0 >= 1.5 R stay within 1e-6
+// of FP64 (measured for the single, double layer and EFIE potentials on icosphere levels 2 and 4).
+constexpr double kNearTileFactor = 1.5;
+
+// The FP64 path of the mixed kernels for the sources [s_begin, s_end) of the workspace.
+template <int OP, int KD>
+__device__ __forceinline__ void sources_fp64_near(const PotArgs &a, int s_begin, int s_end, double x0, double x1,
+                                               double x2, double alr, double ali, double *accr, double *acci) {
+    double ar[KD], ai[KD];
+#pragma unroll
+    for (int d = 0; d < KD; ++d) ar[d] = ai[d] = 0.0;
+    const int64_t P = a.nsrc_pad;
+    for (int s = s_begin; s < s_end; ++s)
+        fp64_source<OP, 0>([&](int k) { return a.src[k * P + s]; }, x0, x1, x2, a.kr, a.ki, alr, ali, ar, ai);
+#pragma unroll
+    for (int d = 0; d < KD; ++d) {
+        accr[d] += ar[d];
+        acci[d] += ai[d];
+    }
+}
+
+// FP64 evaluation (fp64_source per source).
 template <int OP>
 __global__ __launch_bounds__(kPotThreads) void potential_eval_fp64(const PotArgs a) {
     constexpr int KD = OP == kMaxwellEFIE ? 3 : 1;
@@ -171,7 +229,6 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_fp64(const PotArgs
     const double kr = a.kr, ki = a.ki;
     const double kk = kr * kr + ki * ki;
     const double alr = kk > 0.0 ? -ki / kk : 0.0, ali = kk > 0.0 ? -kr / kk : 0.0;  // alpha = 1 / (ik)
-    const double zero3[3] = {0.0, 0.0, 0.0};
     for (int t0 = s0; t0 < s1; t0 += kPotTile) {
         __syncthreads();
         {
@@ -185,33 +242,8 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_fp64(const PotArgs
         const int nt = min(kPotTile, s1 - t0);
         auto sources = [&](auto ph_tag) {
             constexpr int PH = decltype(ph_tag)::value;
-            for (int j = 0; j < nt; ++j) {
-                double dx = sh[0][j] - x0, dy = sh[1][j] - x1, dz = sh[2][j] - x2;
-                double r2 = dx * dx + dy * dy + dz * dz;
-                double ny[3] = {sh[3][j], sh[4][j], sh[5][j]};
-                double gre, gim;
-                green<OP == kMaxwellEFIE ? kHelmholtzSL : OP, double, PH>(dx, dy, dz, r2, zero3, ny, kr, ki, gre, gim);
-                if constexpr (OP == kMaxwellEFIE) {
-                    const double rinv = rsqrt_t(r2), rinv2 = rinv * rinv;
-                    const double qr = rinv - alr * rinv2, qi = -ali * rinv2;  // (ik r - 1) / (ik r^2)
-                    const double t2r = sh[6][j], t2i = sh[7][j];
-                    const double qtr = qr * t2r - qi * t2i, qti = qr * t2i + qi * t2r;
-                    const double diffv[3] = {-dx, -dy, -dz};
-                    const double ikr_re = -ki, ikr_im = kr;  // ik
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        const double t1r = sh[8 + 2 * d][j], t1i = sh[9 + 2 * d][j];
-                        const double tr = (ikr_re * t1r - ikr_im * t1i) - diffv[d] * qtr;
-                        const double ti = (ikr_re * t1i + ikr_im * t1r) - diffv[d] * qti;
-                        accr[d] += gre * tr - gim * ti;
-                        acci[d] += gre * ti + gim * tr;
-                    }
-                } else {
-                    double cr = sh[6][j], ci = sh[7][j];
-                    accr[0] += gre * cr - gim * ci;
-                    acci[0] += gre * ci + gim * cr;
-                }
-            }
+            for (int j = 0; j < nt; ++j)
+                fp64_source<OP, PH>([&](int k) { return sh[k][j]; }, x0, x1, x2, kr, ki, alr, ali, accr, acci);
         };
         if (OpTraits<OP>::is_complex && tile_phase_reduced(kr, ki, xnorm, wmax))
             sources(std::integral_constant<int, kPhReduced | kPhReal>{});
@@ -395,6 +427,12 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed_efie(const P
             // per (target, group) FP64 reference distance and phase
             const double v0d = x0 - sc[g][0], v1d = x1 - sc[g][1], v2d = x2 - sc[g][2];
             const double r0sq_d = v0d * v0d + v1d * v1d + v2d * v2d;
+            // targets close to the group's sources: the group in FP64 for the whole wave
+            const double rn = kNearTileFactor * (double)srad[g];
+            if (__any(m < a.npts && r0sq_d < rn * rn)) {
+                sources_fp64_near<kMaxwellEFIE, 3>(a, t0 + j0, t0 + j1, x0, x1, x2, alr_d, ali_d, accr, acci);
+                continue;
+            }
             const double r0d = sqrt(r0sq_d);
             const double i0 = 1.0 / r0d, i2 = i0 * i0, i3 = i2 * i0;
             const float tv0 = (float)(2.0 * v0d), tv1 = (float)(2.0 * v1d), tv2 = (float)(2.0 * v2d);
@@ -502,6 +540,7 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed_scalar(const
     __shared__ float4 shw[kPotTile];           // Re c / 4 pi, Im c / 4 pi, u.n_y, -
     __shared__ float4 shn[DL ? kPotTile : 1];  // n_y, -
     __shared__ double sc[3];
+    __shared__ float srad2[kPotThreads / 64];  // per wave: max |u|^2 of its staged sources
     const int m = blockIdx.x * kPotThreads + threadIdx.x;
     const int chunk = blockIdx.y;
     const int s0 = chunk * a.chunk_len;
@@ -545,11 +584,25 @@ __global__ __launch_bounds__(kPotThreads) void potential_eval_mixed_scalar(const
             shu[j] = U;
             shw[j] = W;
             if constexpr (DL) shn[j] = N;
+            float w2 = U.w;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) w2 = fmaxf(w2, __shfl_xor(w2, off, 64));
+            if ((j & 63) == 0) srad2[j >> 6] = w2;
         }
         __syncthreads();
         // per (target, tile): FP64 reference distance and phase
         const double v0d = x0 - sc[0], v1d = x1 - sc[1], v2d = x2 - sc[2];
         const double r0sq_d = v0d * v0d + v1d * v1d + v2d * v2d;
+        {
+            // targets close to the tile's sources: this tile in FP64 for the whole wave
+            float R2 = srad2[0];
+#pragma unroll
+            for (int w = 1; w < kPotThreads / 64; ++w) R2 = fmaxf(R2, srad2[w]);
+            if (__any(m < a.npts && r0sq_d < kNearTileFactor * kNearTileFactor * (double)R2 * (1.0 + 1e-6))) {
+                sources_fp64_near<OP, 1>(a, t0, t0 + nt, x0, x1, x2, 0.0, 0.0, &accr, &acci);
+                continue;
+            }
+        }
         const double r0d = sqrt(r0sq_d);
         const float tv0 = (float)(2.0 * v0d), tv1 = (float)(2.0 * v1d), tv2 = (float)(2.0 * v2d);
         const float v0 = (float)v0d, v1 = (float)v1d, v2 = (float)v2d;
@@ -987,12 +1040,12 @@ __global__ void potential_reduce_kernel(const PotArgs a, int kd) {
 
 static void plan(int npts, int nsrc, int *nchunk, int *chunk_len, int *nsrc_pad) {
     *nsrc_pad = ((nsrc + kPotTile - 1) / kPotTile) * kPotTile;
-    int tblocks = (npts + kPotThreads - 1) / kPotThreads;
-    int ntiles = *nsrc_pad / kPotTile;
+    const int tblocks = max(1, (npts + kPotThreads - 1) / kPotThreads);  // no points: one block's plan
+    const int ntiles = max(1, *nsrc_pad / kPotTile);                     // no sources: one empty chunk
     int want = (4096 + tblocks - 1) / tblocks;  // aim at >= 4096 workgroups
-    if (want < 1) want = 1;
     if (want > ntiles) want = ntiles;
     if (want > 65535) want = 65535;
+    if (want < 1) want = 1;
     int tiles_per_chunk = (ntiles + want - 1) / want;
     *chunk_len = tiles_per_chunk * kPotTile;
     *nchunk = (*nsrc_pad + *chunk_len - 1) / *chunk_len;
@@ -1019,34 +1072,39 @@ static bool is_vector_potential_op(int op) {
 template <int OP, int FF = -1>
 static int run_potential(PotArgs &a, int precision, int out_dtype, hipStream_t s) {
     int kd = OP == kMaxwellEFIE ? 3 : 1;
-    hipLaunchKernelGGL((potential_prep_kernel<OP>), dim3((a.nsrc_pad + 255) / 256), dim3(256), 0, s, a);
-    int rc = check_launch("potential_prep_kernel");
-    if (rc) return rc;
-    dim3 grid((a.npts + kPotThreads - 1) / kPotThreads, a.nchunk);
-    if constexpr (FF >= 0) {
-        if (precision == BEM_PRECISION_MIXED)
-            hipLaunchKernelGGL((far_field_eval_mixed<FF>), grid, dim3(kPotThreads), 0, s, a);
-        else
-            hipLaunchKernelGGL((far_field_eval_fp64<FF>), grid, dim3(kPotThreads), 0, s, a);
-    }
-    else if (precision == BEM_PRECISION_MIXED && OP != kMaxwellEFIE) {
-        if constexpr (OP == kHelmholtzSL || OP == kHelmholtzDL) {
-            if (a.ki != 0.0)
-                hipLaunchKernelGGL((potential_eval_mixed_scalar<OP, true>), grid, dim3(kPotThreads), 0, s, a);
+    int rc;
+    // an empty space has no sources: only the reduction runs (it writes zeros)
+    if (a.nsrc > 0) {
+        hipLaunchKernelGGL((potential_prep_kernel<OP>), dim3((a.nsrc_pad + 255) / 256), dim3(256), 0, s, a);
+        if ((rc = check_launch("potential_prep_kernel"))) return rc;
+        dim3 grid((a.npts + kPotThreads - 1) / kPotThreads, a.nchunk);
+        if constexpr (FF >= 0) {
+            if (precision == BEM_PRECISION_MIXED)
+                hipLaunchKernelGGL((far_field_eval_mixed<FF>), grid, dim3(kPotThreads), 0, s, a);
             else
-                hipLaunchKernelGGL((potential_eval_mixed_scalar<OP, false>), grid, dim3(kPotThreads), 0, s, a);
-        } else if constexpr (OP != kMaxwellEFIE) {
-            hipLaunchKernelGGL((potential_eval_mixed_scalar<OP, false>), grid, dim3(kPotThreads), 0, s, a);
+                hipLaunchKernelGGL((far_field_eval_fp64<FF>), grid, dim3(kPotThreads), 0, s, a);
         }
-    } else if (precision == BEM_PRECISION_MIXED) {
-        if (a.ki != 0.0)
-            hipLaunchKernelGGL((potential_eval_mixed_efie<true>), grid, dim3(kPotThreads), 0, s, a);
+        else if (precision == BEM_PRECISION_MIXED && OP != kMaxwellEFIE) {
+            if constexpr (OP == kHelmholtzSL || OP == kHelmholtzDL) {
+                if (a.ki != 0.0)
+                    hipLaunchKernelGGL((potential_eval_mixed_scalar<OP, true>), grid, dim3(kPotThreads), 0, s, a);
+                else
+                    hipLaunchKernelGGL((potential_eval_mixed_scalar<OP, false>), grid, dim3(kPotThreads), 0, s, a);
+            } else if constexpr (OP != kMaxwellEFIE) {
+                hipLaunchKernelGGL((potential_eval_mixed_scalar<OP, false>), grid, dim3(kPotThreads), 0, s, a);
+            }
+        } else if (precision == BEM_PRECISION_MIXED) {
+            if (a.ki != 0.0)
+                hipLaunchKernelGGL((potential_eval_mixed_efie<true>), grid, dim3(kPotThreads), 0, s, a);
+            else
+                hipLaunchKernelGGL((potential_eval_mixed_efie<false>), grid, dim3(kPotThreads), 0, s, a);
+        }
         else
-            hipLaunchKernelGGL((potential_eval_mixed_efie<false>), grid, dim3(kPotThreads), 0, s, a);
+            hipLaunchKernelGGL((potential_eval_fp64<OP>), grid, dim3(kPotThreads), 0, s, a);
+        if ((rc = check_launch("potential_eval"))) return rc;
+    } else {
+        a.nchunk = 0;
     }
-    else
-        hipLaunchKernelGGL((potential_eval_fp64<OP>), grid, dim3(kPotThreads), 0, s, a);
-    if ((rc = check_launch("potential_eval"))) return rc;
     int n = kd * a.npts;
     if (out_dtype == BEM_C128)
         hipLaunchKernelGGL((potential_reduce_kernel<double>), dim3((n + 255) / 256), dim3(256), 0, s, a, kd);
@@ -1074,7 +1132,8 @@ static int potential_evaluate(int op, const bem_space_t *space, const void *coef
                               const double *points, const bem_quad_rule_t *rule, double k_re, double k_im,
                               int precision, void *out, int out_dtype, void *workspace, size_t workspace_bytes,
                               hipStream_t s) {
-    if (!space || !rule || !out || !coefficients || (n_points > 0 && !points) || !workspace) {
+    // no points: points and out may be null (an empty tensor has no storage)
+    if (!space || !rule || !coefficients || (n_points > 0 && (!points || !out)) || !workspace) {
         set_error("null argument");
         return BEM_ERR_INVALID_ARGUMENT;
     }

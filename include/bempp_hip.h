@@ -273,7 +273,8 @@ int bem_singular_assemble_sl_dl(int op, const bem_space_t *test, const bem_space
  * coefficients: complex128 [n_grid_elements * nshape] per (element, shape function), already
  *   multiplied by the local multipliers (space.map_to_full_grid @ x).
  * points: float64 [3 * n_points] (x0 y0 z0 x1 ...). out: [kdim * n_points] of out_dtype
- *   (BEM_C128 or BEM_C64), row d holds component d (kdim = 3 for Maxwell, else 1).
+ *   (BEM_C128 or BEM_C64), row d holds component d (kdim = 3 for Maxwell, else 1). With
+ *   n_points = 0, points and out may be null; a space without elements gives zeros.
  * Only BEM_LAPLACE_SINGLE_LAYER, BEM_LAPLACE_DOUBLE_LAYER, BEM_HELMHOLTZ_SINGLE_LAYER,
  * BEM_HELMHOLTZ_DOUBLE_LAYER and BEM_MAXWELL_ELECTRIC_FIELD are potentials.
  * workspace: device memory of at least bem_potential_workspace_size(...) bytes.

@@ -406,3 +406,36 @@ def test_high_order_potentials_vs_oracle(order, hip_available):
         u = factory(space, pts, k, parameters=params, precision="double").evaluate(GridFunction(space, coefficients=c))
         ref = O.potential(name, grid.vertices, grid.elements, space, c, pts, k, order=order)
         assert rel_fro(u, ref) < 1e-12, name
+
+
+@pytest.mark.parametrize("op,sk,k", [("helmholtz.single_layer", ("P", 1), 3.0), ("helmholtz.double_layer", ("P", 1), 3.0),
+                                     ("maxwell.electric_field", ("RWG", 0), 3.0)])
+def test_potentials_edge_point_sets(op, sk, k, ico2, default_parameters, hip_available):
+    """No points, one point, and points just off the surface (radius 1.05 on the unit sphere, a
+    quarter of an element size away): shapes, FP64 against the oracle and mixed against FP64."""
+    import bempp_amd.api.operators.potential as P
+    from bempp_amd.api import GridFunction, function_space
+    from oracle import oracle as O
+
+    space = function_space(ico2, *sk)
+    rng = np.random.default_rng(5)
+    c = rng.standard_normal(space.global_dof_count) + 1j * rng.standard_normal(space.global_dof_count)
+    fun = GridFunction(space, coefficients=c)
+    mod, fn = op.split(".")
+    factory = getattr(getattr(P, mod), fn)
+    kdim = 3 if mod == "maxwell" else 1
+    name = "maxwell_electric_field" if mod == "maxwell" else f"{mod}_{fn}"
+    g = rng.standard_normal((3, 200))
+    near = 1.05 * g / np.linalg.norm(g, axis=0)
+    for pts in (np.zeros((3, 0)), near[:, :1], near):
+        u64 = factory(space, pts, k, parameters=default_parameters, precision="double").evaluate(fun)
+        u32 = factory(space, pts, k, parameters=default_parameters, precision="single").evaluate(fun)
+        assert u64.shape == (kdim, pts.shape[1]) and u32.shape == (kdim, pts.shape[1])
+        if pts.shape[1] == 0:
+            continue
+        ref = O.potential(name, ico2.vertices, ico2.elements, space, c, pts, k)
+        err64, err32 = rel_fro(u64, ref), rel_fro(u32, u64)
+        print(f"{op} {pts.shape[1]} points at radius 1.05: FP64 vs oracle {err64:.1e}, mixed vs FP64 {err32:.1e}",
+              flush=True)
+        assert err64 < 1e-12
+        assert err32 < 1e-6

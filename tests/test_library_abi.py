@@ -93,3 +93,19 @@ def test_noalloc_potential_refuses_oversize_without_gpu():
                                           0.0, 1, dummy, 2, None)
     assert rc == 4
     assert b"scratch" in L.bem_last_error()
+
+
+@pytest.mark.parametrize("n_elements,n_points", [(20, 0), (0, 100), (0, 0), (1, 1)])
+def test_potential_workspace_size_empty_sets(n_elements, n_points):
+    """The workspace plan of bem_potential_evaluate is computed on the host for empty point sets
+    and empty spaces as well (a plan that divided by the point-block count once crashed here)."""
+    from bempp_amd.core import hip_backend
+
+    L = hip_backend.lib()
+    rule = hip_backend.BemQuadRule()
+    rule.n_points = 6
+    sp = hip_backend.BemSpace()
+    sp.n_elements = n_elements
+    for op in (0, 3, 6):
+        need = L.bem_potential_workspace_size(op, ctypes.byref(sp), ctypes.byref(rule), n_points, 1)
+        assert need >= 256
