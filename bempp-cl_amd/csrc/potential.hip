@@ -183,6 +183,35 @@ __device__ __forceinline__ void fp64_source(F field, double x0, double x1, doubl
     }
 }
 
+// One source's FP64 contribution to the magnetic field potential at x: (x - y) x (G'(r) / r) t1 with
+// G'(r) / r = e^{ikr} (ik r - 1) / (4 pi r^3) (the body of far_field_eval_fp64<kMagneticPotential>,
+// also the mixed kernel's path for targets close to a group's sources).
+template <int PH, typename F>
+__device__ __forceinline__ void fp64_magnetic_source(F field, double x0, double x1, double x2, double kr, double ki,
+                                                     double (&accr)[3], double (&acci)[3]) {
+    const double d0 = x0 - field(0), d1 = x1 - field(1), d2 = x2 - field(2);
+    const double r2 = d0 * d0 + d1 * d1 + d2 * d2, rinv = rsqrt_t(r2), r = r2 * rinv;
+    double sn, cs;
+    sincos_t<PH>(kr * r, &sn, &cs);
+    const double damp = (PH & kPhReal) ? 1.0 : ki != 0.0 ? exp(-ki * r) : 1.0;
+    const double g = kInv4Pi * damp * (rinv * rinv * rinv);
+    const double fr = (PH & kPhReal) ? -1.0 : -ki * r - 1.0, fi = kr * r;  // ik r - 1
+    const double vr = g * (cs * fr - sn * fi), vi = g * (cs * fi + sn * fr);
+    double wr[3], wi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double t1r = field(8 + 2 * d), t1i = field(9 + 2 * d);
+        wr[d] = vr * t1r - vi * t1i;
+        wi[d] = vr * t1i + vi * t1r;
+    }
+    accr[0] += d1 * wr[2] - d2 * wr[1];
+    acci[0] += d1 * wi[2] - d2 * wi[1];
+    accr[1] += d2 * wr[0] - d0 * wr[2];
+    acci[1] += d2 * wi[0] - d0 * wi[2];
+    accr[2] += d0 * wr[1] - d1 * wr[0];
+    acci[2] += d0 * wi[1] - d1 * wi[0];
+}
+
 // Mixed kernels: a (target, tile or group) pair whose reference distance r0 is below
 // kNearTileFactor times the radius R of the tile's sources around the reference takes the FP64
 // path: the FP32 offsets lose relative accuracy once the target comes within a fraction of R of
@@ -725,27 +754,7 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_fp64(const PotArgs
         constexpr int PH = decltype(ph_tag)::value;
         for (int j = 0; j < nt; ++j) {
             if constexpr (FF == kMagneticPotential) {
-                const double d0 = x0 - sh[0][j], d1 = x1 - sh[1][j], d2 = x2 - sh[2][j];
-                const double r2 = d0 * d0 + d1 * d1 + d2 * d2, r = sqrt(r2);
-                double sn, cs;
-                sincos_t<PH>(kr * r, &sn, &cs);
-                const double damp = (PH & kPhReal) ? 1.0 : ki != 0.0 ? exp(-ki * r) : 1.0;
-                const double g = kInv4Pi * damp / (r * r2);
-                const double fr = -ki * r - 1.0, fi = kr * r;  // ik r - 1
-                const double vr = g * (cs * fr - sn * fi), vi = g * (cs * fi + sn * fr);
-                double wr[3], wi[3];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    const double t1r = sh[8 + 2 * d][j], t1i = sh[9 + 2 * d][j];
-                    wr[d] = vr * t1r - vi * t1i;
-                    wi[d] = vr * t1i + vi * t1r;
-                }
-                accr[0] += d1 * wr[2] - d2 * wr[1];
-                acci[0] += d1 * wi[2] - d2 * wi[1];
-                accr[1] += d2 * wr[0] - d0 * wr[2];
-                acci[1] += d2 * wi[0] - d0 * wi[2];
-                accr[2] += d0 * wr[1] - d1 * wr[0];
-                acci[2] += d0 * wi[1] - d1 * wi[0];
+                fp64_magnetic_source<PH>([&](int k) { return sh[k][j]; }, x0, x1, x2, kr, ki, accr, acci);
                 continue;
             }
             const double dot = x0 * sh[0][j] + x1 * sh[1][j] + x2 * sh[2][j];
@@ -826,6 +835,7 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_mixed(const PotArg
     __shared__ float sh[NF][kPotTile];
     __shared__ double sc[NG][3];
     __shared__ double wsum[NG][POT ? 1 : NW];
+    __shared__ float grad2[POT ? NG : 1];  // magnetic potential: the group's max |u|^2
     const int m = blockIdx.x * kPotThreads + threadIdx.x;
     const int chunk = blockIdx.y;
     const int s0 = chunk * a.chunk_len;
@@ -857,6 +867,13 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_mixed(const PotArg
             const bool ok = threadIdx.x < nt;
             for (int d = 0; d < 3; ++d) sh[d][threadIdx.x] = ok ? (float)(a.src[(int64_t)d * a.nsrc_pad + s] - c[d]) : 0.f;
             for (int k = 3; k < NF; ++k) sh[k][threadIdx.x] = ok ? (float)a.src[(int64_t)k * a.nsrc_pad + s] : 0.f;
+            if constexpr (POT) {
+                float u2 = sh[0][threadIdx.x] * sh[0][threadIdx.x] + sh[1][threadIdx.x] * sh[1][threadIdx.x] +
+                           sh[2][threadIdx.x] * sh[2][threadIdx.x];
+#pragma unroll
+                for (int off = GS / 2; off >= 1; off >>= 1) u2 = fmaxf(u2, __shfl_xor(u2, off, 64));
+                if (threadIdx.x % GS == 0) grad2[g] = u2;
+            }
             if constexpr (!POT) {
                 double v[NW];
                 if constexpr (FF == kFarDL) {
@@ -892,6 +909,14 @@ __global__ __launch_bounds__(kPotThreads) void far_field_eval_mixed(const PotArg
             if constexpr (POT) {
                 const double v0d = x0 - sc[g][0], v1d = x1 - sc[g][1], v2d = x2 - sc[g][2];
                 const double r0sq_d = v0d * v0d + v1d * v1d + v2d * v2d;
+                // targets close to the group's sources: the group in FP64 for the whole wave
+                if (__any(m < a.npts && r0sq_d < kNearTileFactor * kNearTileFactor * (double)grad2[g] * (1.0 + 1e-6))) {
+                    const int64_t P = a.nsrc_pad;
+                    for (int jj = t0 + j0; jj < t0 + j1; ++jj)
+                        fp64_magnetic_source<0>([&](int k) { return a.src[k * P + jj]; }, x0, x1, x2, a.kr, a.ki,
+                                                accr, acci);
+                    continue;
+                }
                 const double r0d = sqrt(r0sq_d);
                 double sn, cs;
                 sincos_fast(a.kr * r0d, &sn, &cs);

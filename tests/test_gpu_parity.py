@@ -409,7 +409,8 @@ def test_high_order_potentials_vs_oracle(order, hip_available):
 
 
 @pytest.mark.parametrize("op,sk,k", [("helmholtz.single_layer", ("P", 1), 3.0), ("helmholtz.double_layer", ("P", 1), 3.0),
-                                     ("maxwell.electric_field", ("RWG", 0), 3.0)])
+                                     ("maxwell.electric_field", ("RWG", 0), 3.0),
+                                     ("maxwell.magnetic_field", ("RWG", 0), 3.0)])
 def test_potentials_edge_point_sets(op, sk, k, ico2, default_parameters, hip_available):
     """No points, one point, and points just off the surface (radius 1.05 on the unit sphere, a
     quarter of an element size away): shapes, FP64 against the oracle and mixed against FP64."""
@@ -424,7 +425,7 @@ def test_potentials_edge_point_sets(op, sk, k, ico2, default_parameters, hip_ava
     mod, fn = op.split(".")
     factory = getattr(getattr(P, mod), fn)
     kdim = 3 if mod == "maxwell" else 1
-    name = "maxwell_electric_field" if mod == "maxwell" else f"{mod}_{fn}"
+    name = f"{mod}_{fn}"
     g = rng.standard_normal((3, 200))
     near = 1.05 * g / np.linalg.norm(g, axis=0)
     for pts in (np.zeros((3, 0)), near[:, :1], near):
