@@ -440,3 +440,35 @@ def test_potentials_edge_point_sets(op, sk, k, ico2, default_parameters, hip_ava
               flush=True)
         assert err64 < 1e-12
         assert err32 < 1e-6
+
+
+@pytest.mark.parametrize("radius", [1.02, 1.2])
+def test_mixed_two_grids_close(radius, hip_available):
+    """Test space on a sphere of the given radius, trial space on the unit sphere (level 3): no
+    singular pairs, many near pairs across the two grids; mixed within 1e-6 of FP64."""
+    import bempp_amd.api as bempp
+    from bempp_amd.api.operators.boundary import helmholtz
+
+    g1 = bempp.shapes.icosphere(3)
+    g2 = g1.with_vertices(g1.vertices * radius)
+    for fam, sk in (("single_layer", ("P", 1)), ("double_layer", ("P", 1)), ("adjoint_double_layer", ("DP", 0))):
+        s1, s2 = bempp.function_space(g1, *sk), bempp.function_space(g2, *sk)
+        f = getattr(helmholtz, fam)
+        a64 = f(s2, s1, s1, 3.0, precision="double").weak_form().A
+        a32 = f(s2, s1, s1, 3.0, precision="single").weak_form().A
+        err = rel_fro(a32.astype(a64.dtype), a64)
+        print(f"{fam} across grids at radius {radius}: mixed vs FP64 {err:.1e}", flush=True)
+        assert err < 1e-6
+
+
+@pytest.mark.parametrize("k", [20.0, 40.0])
+def test_mixed_large_wavenumber(k, hip_available):
+    """k = 20 and 40 on level 4 (about 3 and 6 elements per wavelength): phases of many turns per
+    pair; mixed within 1e-6 of FP64."""
+    import bempp_amd.api as bempp
+    from bempp_amd.api.operators.boundary import helmholtz
+
+    sp = bempp.function_space(bempp.shapes.icosphere(4), "P", 1)
+    a64 = helmholtz.single_layer(sp, sp, sp, k, precision="double").weak_form().A
+    a32 = helmholtz.single_layer(sp, sp, sp, k, precision="single").weak_form().A
+    assert rel_fro(a32.astype(a64.dtype), a64) < 1e-6
