@@ -472,3 +472,32 @@ def test_mixed_large_wavenumber(k, hip_available):
     a64 = helmholtz.single_layer(sp, sp, sp, k, precision="double").weak_form().A
     a32 = helmholtz.single_layer(sp, sp, sp, k, precision="single").weak_form().A
     assert rel_fro(a32.astype(a64.dtype), a64) < 1e-6
+
+
+@pytest.mark.parametrize("k", [30.0, 30.0 + 2.0j])
+def test_mixed_potentials_large_wavenumber(k, hip_available):
+    """k = 30 on level 3: many phase turns inside one source group; EFIE, magnetic and Helmholtz
+    potentials (radius 2; double layer at 1.05) and, for real k, the four far fields: mixed within
+    1e-6 of FP64."""
+    import bempp_amd.api as bempp
+    import bempp_amd.api.operators.far_field as FF
+    import bempp_amd.api.operators.potential as P
+
+    g = bempp.shapes.icosphere(3)
+    rwg, p1 = bempp.function_space(g, "RWG", 0), bempp.function_space(g, "P", 1)
+    rng = np.random.default_rng(1)
+    fr = bempp.GridFunction(rwg, coefficients=rng.standard_normal(rwg.global_dof_count) + 1j * rng.standard_normal(
+        rwg.global_dof_count))
+    fp = bempp.GridFunction(p1, coefficients=rng.standard_normal(p1.global_dof_count) + 1j * rng.standard_normal(
+        p1.global_dof_count))
+    x = rng.standard_normal((3, 400))
+    xh = x / np.linalg.norm(x, axis=0)
+    cases = [(P.maxwell.electric_field, rwg, fr, 2 * xh), (P.maxwell.magnetic_field, rwg, fr, 2 * xh),
+             (P.helmholtz.single_layer, p1, fp, 2 * xh), (P.helmholtz.double_layer, p1, fp, 1.05 * xh)]
+    if np.imag(k) == 0:
+        cases += [(FF.maxwell.electric_field, rwg, fr, xh), (FF.maxwell.magnetic_field, rwg, fr, xh),
+                  (FF.helmholtz.single_layer, p1, fp, xh), (FF.helmholtz.double_layer, p1, fp, xh)]
+    for fac, space, fun, pts in cases:
+        u64 = fac(space, pts, k, precision="double").evaluate(fun)
+        u32 = fac(space, pts, k, precision="single").evaluate(fun)
+        assert rel_fro(u32, u64) < 1e-6, fac
