@@ -501,3 +501,34 @@ def test_mixed_potentials_large_wavenumber(k, hip_available):
         u64 = fac(space, pts, k, precision="double").evaluate(fun)
         u32 = fac(space, pts, k, precision="single").evaluate(fun)
         assert rel_fro(u32, u64) < 1e-6, fac
+
+
+def test_open_screen_vs_oracle(hip_available):
+    """A flat square screen (an open surface, 512 triangles; P1 has no dofs on its boundary):
+    single layers on P1 and DP0 and the hypersingular operator, FP64 against the oracle within
+    1e-12 and mixed against FP64 within 1e-6."""
+    import bempp_amd.api as bempp
+    from bempp_amd.api.operators.boundary import helmholtz, laplace
+    from oracle import oracle as O
+
+    n = 16
+    xs = np.linspace(0.0, 1.0, n + 1)
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    vertices = np.vstack([X.ravel(), Y.ravel(), np.zeros(X.size)])
+    tris = []
+    for i in range(n):
+        for j in range(n):
+            a = i * (n + 1) + j
+            tris += [[a, a + n + 1, a + 1], [a + 1, a + n + 1, a + n + 2]]
+    g = bempp.Grid(vertices, np.array(tris).T)
+    cases = [(("P", 1), "helmholtz_single_layer", helmholtz.single_layer, 3.0),
+             (("DP", 0), "laplace_single_layer", laplace.single_layer, None),
+             (("P", 1), "helmholtz_hypersingular", helmholtz.hypersingular, 3.0)]
+    for sk, name, f, k in cases:
+        sp = bempp.function_space(g, *sk)
+        args = (sp, sp, sp) if k is None else (sp, sp, sp, k)
+        a64 = f(*args, precision="double").weak_form().A
+        a32 = f(*args, precision="single").weak_form().A
+        ref = O.assemble(name, g.vertices, g.elements, sp, sp, 0.0 if k is None else k)
+        assert rel_fro(a64, ref) < 1e-12, name
+        assert rel_fro(a32.astype(a64.dtype), a64) < 1e-6, name
