@@ -136,6 +136,10 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     # library zeroes the output and adds the transpose itself
     sym_sl = (not accumulate and not wide and sym_opt and op in _SYMMETRIC_P1_OPERATORS
               and use_symmetric_pass(domain, dual_to_range))
+    # the electric field operator on one RWG dof map (RWG trial, SNC or RWG test) is symmetric too
+    sym_efie = (not accumulate and not wide and sym_opt and op == "maxwell_electric_field"
+                and use_symmetric_efie(domain, dual_to_range))
+    sym_sl = sym_sl or sym_efie
     if not accumulate and not full_cover and not sym_sl:
         hb.fill_zero(out)
     near = getattr(hip_opts, "mixed_near_field_factor", DEFAULT_NEAR_FIELD_FACTOR) if hip_opts is not None else DEFAULT_NEAR_FIELD_FACTOR
@@ -166,8 +170,10 @@ def hip_dense_assemble(domain, dual_to_range, parameters, operator_descriptor, o
     elif sym_sl:
         from bempp_amd.core.blocking import device_symmetric_blocks
 
-        hb.regular_assemble_sl_dl_symmetric(op, device_symmetric_blocks(domain), rule, _wavenumber(operator_descriptor),
-                                            mode, out, None, near_factor=near, kahan=kahan)
+        role = "trial_rwg_mixed" if sym_efie and mode == hb.PRECISION_MIXED else "trial"
+        hb.regular_assemble_sl_dl_symmetric(op, device_symmetric_blocks(domain, role), rule,
+                                            _wavenumber(operator_descriptor), mode, out, None, near_factor=near,
+                                            kahan=kahan)
     elif direct:
         hb.regular_assemble(
             op, test, trial, grids_identical, rule, _wavenumber(operator_descriptor), mode, out,
@@ -246,6 +252,24 @@ def use_symmetric_pass(domain, dual_to_range):
         and not _is_localised_injective(domain)
         and domain.codomain_dimension == 1
         and domain.number_of_shape_functions == 3
+    )
+
+
+def use_symmetric_efie(domain, dual_to_range):
+    """True if the electric field operator can run the symmetric pass: both spaces carry RWG
+    or SNC functions (the kernel evaluates the SNC test side through the RWG functions) over the
+    same support, dof map, multipliers and normal orientation, so entry (i, j) equals (j, i) up
+    to the singular quadrature."""
+    kinds = ("rwg0", "snc0")
+    if domain.shapeset.identifier not in kinds or dual_to_range.shapeset.identifier not in kinds:
+        return False
+    if domain.grid != dual_to_range.grid or domain.global_dof_count != dual_to_range.global_dof_count:
+        return False
+    return bool(
+        _np.array_equal(domain.support_elements, dual_to_range.support_elements)
+        and _np.array_equal(domain.local2global, dual_to_range.local2global)
+        and _np.array_equal(domain.local_multipliers, dual_to_range.local_multipliers)
+        and _np.array_equal(domain.normal_multipliers, dual_to_range.normal_multipliers)
     )
 
 

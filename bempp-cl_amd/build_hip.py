@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "bempp_amd", "lib", "libbempp_hip.so")
-SOURCES = ["common.hip", "regular.hip", "near.hip", "singular.hip", "potential.hip", "kbatch.hip", "adjoint.hip", "matvec.hip", "mfie.hip"] + [f"regular_op{k}.hip" for k in range(9)] + ["regular_dual0.hip", "regular_dual3.hip", "regular_sym0.hip", "regular_sym3.hip", "regular_upper.hip", "regular_mfie.hip", "regular_symsl0.hip", "regular_symsl3.hip", "regular_symsl7.hip", "regular_symsl8.hip", "regular_wide.hip"]
+SOURCES = ["common.hip", "regular.hip", "near.hip", "singular.hip", "potential.hip", "kbatch.hip", "adjoint.hip", "matvec.hip", "mfie.hip"] + [f"regular_op{k}.hip" for k in range(9)] + ["regular_dual0.hip", "regular_dual3.hip", "regular_sym0.hip", "regular_sym3.hip", "regular_upper.hip", "regular_mfie.hip", "regular_symsl0.hip", "regular_symsl3.hip", "regular_symsl7.hip", "regular_symsl8.hip", "regular_symsl6.hip", "regular_wide.hip"]
 EXTRA_SOURCES = []
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

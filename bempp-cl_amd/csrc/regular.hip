@@ -115,7 +115,9 @@ static int regular_entry(int op, const bem_space_t *test, const bem_space_t *tri
                         : regular_launch_symsl_3(a, precision, kahan, out_dtype, s);
         if (op == BEM_LAPLACE_HYPERSINGULAR && !out2) return regular_launch_symsl_7(a, precision, kahan, out_dtype, s);
         if (op == BEM_HELMHOLTZ_HYPERSINGULAR && !out2) return regular_launch_symsl_8(a, precision, kahan, out_dtype, s);
-        set_error("the symmetric pass needs a single-layer operator (alone also a hypersingular one), got %d", op);
+        if (op == BEM_MAXWELL_ELECTRIC_FIELD && !out2) return regular_launch_symsl_6(a, precision, kahan, out_dtype, s);
+        set_error("the symmetric pass needs a single-layer operator (alone also a hypersingular or the electric "
+                  "field operator), got %d", op);
         return BEM_ERR_UNSUPPORTED;
     }
     if (out2) {

@@ -762,12 +762,13 @@ __device__ __forceinline__ void store_direct(void *out, int64_t ld, int row, int
 // adds the transpose afterwards: S_fe = S_ef^T); with DUAL also its double-layer block to a.out2
 // at (e, f) and the double layer of the reversed pair (test normal) to a.out2 at (f, e) (REV).
 // waves per SIMD the launch bound asks for: the FP32 fused pass and the symmetric single layer
-// fit 3; the symmetric single + double layer pass and the mixed EFIE pass keep more state (2);
+// fit 3; the symmetric single + double layer pass and the mixed EFIE passes (ordered and
+// symmetric) keep more state (2);
 // the single-operator FP64 passes, ordered or symmetric, 2 (their few spills lie outside the
 // point-pair loops); the others leave it to the compiler
 template <int OP, bool MIXED, bool DUAL, bool SYM>
 constexpr int regular_waves() {
-    return !MIXED ? (DUAL ? 1 : 2) : SYM ? (DUAL ? 2 : 3) : DUAL ? 3 : is_rwg_op<OP>() ? 2 : 1;
+    return !MIXED ? (DUAL ? 1 : 2) : SYM ? (DUAL || is_rwg_op<OP>() ? 2 : 3) : DUAL ? 3 : is_rwg_op<OP>() ? 2 : 1;
 }
 
 // UPPER (direct DP0 stores, a symmetric operator on one space): only the pairs whose trial element
@@ -1403,7 +1404,7 @@ static int launch2(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
 template <int OP, bool MIXED, typename OutT, bool DUAL = false, bool SYM = false, bool UPPER = false>
 static int launch1(const RegularArgs &a, int kahan, bool tile, hipStream_t stream) {
     if constexpr (OP == kMaxwellEFIE) {
-        return launch2<OP, 3, 3, MIXED, OutT>(a, kahan, tile, stream);
+        return launch2<OP, 3, 3, MIXED, OutT, false, SYM>(a, kahan, tile, stream);
     } else if constexpr (is_hypersingular<OP>()) {
         if (nshape_of(a.test.shape) != 3 || nshape_of(a.trial.shape) != 3) {
             set_error("hypersingular operators need P1 shape functions on both sides");
@@ -1482,6 +1483,7 @@ int regular_launch_symsl_0(const RegularArgs &a, int precision, int kahan, int o
 int regular_launch_symsl_3(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
 int regular_launch_symsl_7(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
 int regular_launch_symsl_8(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
+int regular_launch_symsl_6(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
 // symmetric DP0 single layer, upper triangle (regular_upper.hip)
 int regular_launch_upper_0(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);
 int regular_launch_upper_3(const RegularArgs &a, int precision, int kahan, int out_dtype, hipStream_t s);

@@ -313,6 +313,20 @@ def efie_bench(level, k, reps, with_cpu=False, cpu_seconds=4.0):
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / reps
     reg = float(np.mean([e["regular_start"].elapsed_time(e["regular_end"]) for e in evs]))
+    # the same assembly over ordered element pairs (parameters.hip.symmetric = False)
+    params.hip.symmetric = False
+    try:
+        hip_dense_assemble(rwg, snc, params, d32, out=out, accumulate=False)
+        torch.cuda.synchronize()
+        oev = {key: _ev() for key in ("regular_start", "regular_end", "singular_end")}
+        t = time.perf_counter()
+        hip_dense_assemble(rwg, snc, params, d32, out=out, accumulate=False, events=oev)
+        torch.cuda.synchronize()
+        ordered_ms = 1e3 * (time.perf_counter() - t)
+        ordered_reg = oev["regular_start"].elapsed_time(oev["regular_end"])
+    finally:
+        params.hip.symmetric = True
+    hip_dense_assemble(rwg, snc, params, d32, out=out, accumulate=False)  # the symmetric result for the check
     ref = hip_dense_assemble(rwg, snc, params, d64)  # warm: blocking, singular rule, allocation
     torch.cuda.synchronize()
     t = time.perf_counter()
@@ -352,10 +366,15 @@ def efie_bench(level, k, reps, with_cpu=False, cpu_seconds=4.0):
         "regular_kernel_ms": reg,
         "regular_tflops": n_evals * 73 / (reg * 1e-3) / 1e12,
         "flop_per_eval": 73,
+        "traversal": "symmetric (each unordered element pair of the regular part once, then the transpose; "
+                     "the flop count is that of the ordered pairs)",
+        "ordered_ms": ordered_ms,
+        "ordered_regular_ms": ordered_reg,
         "rel_err_vs_fp64": err,
         "fp64_ms": 1e3 * t64,
         "speedup_vs_fp64": t64 / wall,
-        "roofline": roofline("regular_kernel<maxwell_electric_field, RWGxSNC, mixed, tiled>", "valu",
+        "roofline": roofline("regular_kernel<maxwell_electric_field, RWGxSNC, mixed, symmetric> (+ zero fill and "
+                             "transpose add)", "valu",
                              n_evals * 73 / (reg * 1e-3) / 1e12, PEAK_FP32_TFLOPS, "TFLOP/s",
                              traffic_key="regular_kernel_maxwell_efie_rwg_level4_mixed", evals_per_launch=n_evals,
                              flop_per_eval=73),

@@ -5,15 +5,30 @@ import numpy as np
 import pytest
 
 
-@pytest.mark.parametrize("level,run", [(1, None), (2, None), (2, 3), (3, None), (3, 7)])
-def test_work_list_covers_each_unordered_pair_once(level, run):
+# (level, tiles per item, space); RWG with the mixed electric field pass's 416-dof trial blocks
+CASES = [(1, None, "P1"), (2, None, "P1"), (2, 3, "P1"), (3, None, "P1"), (3, 7, "P1"), (2, None, "RWG"),
+         (3, None, "RWG"), (3, 5, "RWG")]
+
+
+@pytest.mark.parametrize("level,run,kind", CASES)
+def test_work_list_covers_each_unordered_pair_once(level, run, kind):
     from bempp_amd.api import function_space
     from bempp_amd.api.grid.shapes import icosphere
-    from bempp_amd.core.blocking import SYM_TEST_TILE_DOFS, SYM_TEST_TILE_ELEMENTS, make_symmetric_blocks
+    from bempp_amd.core.blocking import (RWG_MIXED_TRIAL_BLOCK_DOFS, SYM_RUN_ELEMENTS, SYM_TEST_TILE_DOFS,
+                                         SYM_TEST_TILE_ELEMENTS, TRIAL_BLOCK_DOFS, make_symmetric_blocks)
 
     grid = icosphere(level)
-    p1 = function_space(grid, "P", 1)
-    trial, tiles, owner, work = make_symmetric_blocks(p1, tiles_per_item=run)
+    if kind == "RWG":
+        p1 = function_space(grid, "RWG", 0)
+        trial, tiles, owner, work = make_symmetric_blocks(p1, tiles_per_item=run,
+                                                          trial_dofs=RWG_MIXED_TRIAL_BLOCK_DOFS, min_tiles_per_item=2)
+        assert trial.ndofs.max() <= RWG_MIXED_TRIAL_BLOCK_DOFS
+    else:
+        p1 = function_space(grid, "P", 1)
+        trial, tiles, owner, work = make_symmetric_blocks(p1, tiles_per_item=run)
+        assert trial.ndofs.max() <= TRIAL_BLOCK_DOFS
+    # a work item's run of test tiles fits the kernel's LDS copy of the run
+    assert (tiles.start[work[:, 2]] - tiles.start[work[:, 1]]).max() <= SYM_RUN_ELEMENTS
     n = len(trial.order)
     assert np.array_equal(tiles.order, trial.order)
     assert np.diff(tiles.start).max() <= SYM_TEST_TILE_ELEMENTS and tiles.ndofs.max() <= SYM_TEST_TILE_DOFS
@@ -43,6 +58,21 @@ def test_work_list_covers_each_unordered_pair_once(level, run):
         for pos in range(tiles.start[t], tiles.start[t + 1]):
             for i in range(3):
                 assert tiles.dofs[t, tiles.slots[pos, i]] == l2g[trial.order[pos], i]
+
+
+def test_symmetric_efie_predicate():
+    from bempp_amd.api import function_space
+    from bempp_amd.api.grid.shapes import icosphere
+    from bempp_amd.core.dense_assembler import use_symmetric_efie
+
+    g = icosphere(1)
+    rwg, snc = function_space(g, "RWG", 0), function_space(g, "SNC", 0)
+    assert use_symmetric_efie(rwg, snc)
+    assert use_symmetric_efie(rwg, rwg)
+    assert not use_symmetric_efie(rwg, function_space(icosphere(2), "SNC", 0))
+    assert not use_symmetric_efie(function_space(g, "P", 1), function_space(g, "P", 1))
+    part = function_space(g, "SNC", 0, support_elements=np.arange(40))
+    assert not use_symmetric_efie(rwg, part)
 
 
 def test_symmetric_pass_predicate():
