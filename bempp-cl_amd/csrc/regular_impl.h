@@ -100,6 +100,8 @@ struct TestTile {
 
 // test elements staged per SYM work item (a run of test tiles)
 constexpr int kSymRunElements = BEM_TILE_SYM_RUN_ELEMENTS;
+// below this many workgroups (8 per CU) the DP0 upper-triangle pass takes 4-element test tiles
+constexpr int64_t kUpperSmallGrid = 2048;
 
 // ------------------------------------------------------------------ FP64 pair
 // T[b] += g u_b(eta_q) with u = (1, eta0, eta1)
@@ -1386,6 +1388,25 @@ static int launch3(const RegularArgs &a, int kahan, bool tile, hipStream_t strea
         } else {
             if (MIXED && kahan) BEM_LAUNCH(true, true);
             else BEM_LAUNCH(false, true);
+        }
+    } else if constexpr (UPPER) {
+        // a small DP0 space (1,280 elements: 400 workgroups of 16-element test tiles, half of them
+        // below the diagonal) leaves most SIMDs without a wave: 4-element test tiles give four
+        // times the workgroups
+        constexpr int TMS = 4;
+        if ((int64_t)grid.x * grid.y < kUpperSmallGrid) {
+            const dim3 gs(grid.x, (a.test.n + TMS - 1) / TMS);
+            if (MIXED && kahan)
+                hipLaunchKernelGGL((regular_kernel<OP, NTS, NSS, QT, TMS, MIXED, true, false, 1, 1, OutT, false, false,
+                                                   true>),
+                                   gs, dim3(kRegThreads), 0, stream, a);
+            else
+                hipLaunchKernelGGL((regular_kernel<OP, NTS, NSS, QT, TMS, MIXED, false, false, 1, 1, OutT, false, false,
+                                                   true>),
+                                   gs, dim3(kRegThreads), 0, stream, a);
+        } else {
+            if (MIXED && kahan) BEM_LAUNCH(true, false);
+            else BEM_LAUNCH(false, false);
         }
     } else if constexpr (!SYM && OP != kMaxwellMFIE) {
         if (MIXED && kahan) BEM_LAUNCH(true, false);
