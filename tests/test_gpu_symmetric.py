@@ -251,3 +251,21 @@ def test_symmetric_efie_open_screen_vs_oracle(default_parameters, hip_available)
     print(f"EFIE screen ({rwg.global_dof_count} dofs): symmetric FP64 {e64:.2e}, mixed {e32:.2e}", flush=True)
     assert e64 < 1e-12
     assert e32 < 1e-6
+
+
+@pytest.mark.parametrize("order", [2, 6])
+def test_symmetric_efie_other_rules(order, default_parameters, hip_available):
+    """Regular rules other than the 6-point one (3 and 12 points: the generic point loop) in the
+    symmetric EFIE pass, against the ordered pass."""
+    from bempp_amd.api import function_space
+
+    grid = perturbed_sphere(2)
+    rwg = function_space(grid, "RWG", 0)
+    snc = function_space(grid, "SNC", 0)
+    default_parameters.quadrature.regular = order
+    o = efie(rwg, snc, 2.0, "double", False, default_parameters)
+    s = efie(rwg, snc, 2.0, "double", True, default_parameters)
+    m = efie(rwg, snc, 2.0, "single", True, default_parameters)
+    print(f"EFIE regular order {order}: FP64 symmetric vs ordered {rel(s, o):.2e}, mixed {rel(m, o):.2e}", flush=True)
+    assert rel(s, o) < 1e-13
+    assert rel(m, o) < 1e-6
