@@ -231,9 +231,8 @@ extern "C" int bem_regular_assemble_symmetric(int op, const bem_space_t *space, 
         set_error("leading dimension smaller than the number of dofs");
         return BEM_ERR_INVALID_ARGUMENT;
     }
-    // upper triangle (overwritten), then its mirror image below a zero diagonal
-    int rc = regular_entry(op, space, space, nullptr, nullptr, 1, rule, k_re, k_im, precision, mixed_near_factor,
-                           kahan, out, nullptr, ld, out_dtype, 0, stream, nullptr, 0, true);
-    if (rc) return rc;
-    return square_matrix_op(2, out, out_dtype, space->n_elements, ld, (hipStream_t)stream);
+    // the upper triangle and, from the same pairs, its mirror image below a zero diagonal: every
+    // entry is overwritten
+    return regular_entry(op, space, space, nullptr, nullptr, 1, rule, k_re, k_im, precision, mixed_near_factor,
+                         kahan, out, nullptr, ld, out_dtype, 0, stream, nullptr, 0, true);
 }

@@ -774,8 +774,8 @@ constexpr int regular_waves() {
 }
 
 // UPPER (direct DP0 stores, a symmetric operator on one space): only the pairs whose trial element
-// comes later in the list than the test element are evaluated and stored; the caller mirrors the
-// upper triangle into the lower one (bem_regular_assemble_symmetric).
+// comes later in the list than the test element are evaluated; each is stored at (i, j) and at its
+// mirror image (j, i), and the diagonal gets zeros (bem_regular_assemble_symmetric).
 // ORB: a.rule is a 6-point rule in orbit order (orbit_rule); the FP32 pairs use the orbit moments.
 template <int OP, int NTS, int NSS, int QT, int TM, bool MIXED, bool KAHAN, bool TILE, int MAXU_T, int MAXU_S,
           typename OutT, bool DUAL = false, bool SYM = false, bool UPPER = false, bool ORB = false>
@@ -823,6 +823,12 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
     __shared__ int32_t sh_sdofs[TILE ? MAXU_S : 1];
     __shared__ int32_t sh_tdofs[TILE ? 2 : 1][TILE ? MAXU_T : 1];
     __shared__ int32_t sh_tmeta[TILE ? 2 : 1][3];
+    // UPPER: the tile's mirror images, [trial lane][test element] (padded row), written out after
+    // the tile as whole row segments; 16-byte entries (complex128) are stored directly instead
+    using MirT = typename std::conditional<
+        CPLX, typename std::conditional<sizeof(OutT) == 8, double2, float2>::type, OutT>::type;
+    constexpr bool MSTAGE = UPPER && sizeof(MirT) <= 8;
+    __shared__ MirT mir[MSTAGE ? kRegThreads * (TM + 1) : 1];
 
     const int tid = threadIdx.x;
     int block_id = blockIdx.x, tile_begin = blockIdx.y, tile_end = blockIdx.y + 1;
@@ -1061,7 +1067,7 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                       v2 == fv1 || v2 == fv2;
             }
             if constexpr (SYM) adj = adj || (diag && jf <= t0 + t);  // the pair belongs to the other orientation
-            if constexpr (UPPER) adj = adj || jf <= t0 + t;            // lower triangle: mirrored afterwards
+            if constexpr (UPPER) adj = adj || jf <= t0 + t;            // lower triangle: other pairs' mirror images
             C2d S[NTS][NSS], S2[NTS][NSS], S3[NTS][NSS];     // FP64 moments (FP64 mode; MIXED paths that need the FP64 epilogue)
             C2f Sf[NTS][NSS], Sf2[NTS][NSS], Sf3[NTS][NSS];  // FP32 moments (MIXED)
     #pragma unroll
@@ -1263,7 +1269,27 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                             v.im *= m;
                             int row = a.test.l2g[(int64_t)e * NT + i];
                             int col = a.trial.l2g[(int64_t)f * NS + j];
-                            store_direct<OutT, CPLX>(a.out, a.ld, row, col, v, a.accumulate);
+                            if constexpr (UPPER) {
+                                // the entry and its mirror image (dof = list position): a lane
+                                // whose trial element comes before the test element stores
+                                // nothing (its entries are other pairs' mirror images), the
+                                // diagonal stores the zero of its coincident pair
+                                if (jf >= t0 + t) store_direct<OutT, CPLX>(a.out, a.ld, row, col, v, 0);
+                                if constexpr (MSTAGE) {
+                                    MirT mv;
+                                    if constexpr (CPLX) {
+                                        mv.x = (OutT)v.re;
+                                        mv.y = (OutT)v.im;
+                                    } else {
+                                        mv = (OutT)v.re;
+                                    }
+                                    mir[tid * (TM + 1) + t] = mv;
+                                } else {
+                                    if (jf > t0 + t) store_direct<OutT, CPLX>(a.out, a.ld, col, row, v, 0);
+                                }
+                            } else {
+                                store_direct<OutT, CPLX>(a.out, a.ld, row, col, v, a.accumulate);
+                            }
                             if constexpr (DUAL) {
                                 C2d v2 = local_value<OP2, NTS, NSS>(S2, i, j, gt, gf, a.kr, a.ki);
                                 v2.re *= m;
@@ -1272,6 +1298,29 @@ __global__ __launch_bounds__(kRegThreads, (regular_waves<OP, MIXED, DUAL, SYM>()
                             }
                         }
                     }
+                }
+            }
+        }
+
+        if constexpr (MSTAGE) {
+            // the mirror images of the tile: row jr (a trial position) gets columns t0 .. t0 + ntile,
+            // TM consecutive lanes per row, so a wave stores whole row segments (a 128-byte line for
+            // 16 complex64 entries); entries left of the diagonal are other tiles', the diagonal is zero
+            __syncthreads();
+            MirT *o = reinterpret_cast<MirT *>(a.out);
+            const int c = tid % TM, ic = t0 + c;
+            for (int r = tid / TM; r < s1 - s0; r += kRegThreads / TM) {
+                const int jr = s0 + r;
+                if (c < ntile && jr >= ic) {
+                    MirT mv;
+                    if (jr > ic) {
+                        mv = mir[r * (TM + 1) + c];
+                    } else if constexpr (CPLX) {
+                        mv.x = mv.y = (OutT)0;
+                    } else {
+                        mv = (OutT)0;
+                    }
+                    o[(int64_t)jr * a.ld + ic] = mv;
                 }
             }
         }

@@ -392,8 +392,9 @@ int bem_dense_matvec(int trans, const void *A, int a_dtype, int64_t rows, int64_
  * Regular part of a symmetric single layer (BEM_LAPLACE_SINGLE_LAYER, BEM_HELMHOLTZ_SINGLE_LAYER)
  * on one piecewise constant space whose dof of list position p is p (no reference counterpart:
  * the reference computes every ordered pair, bempp_cl/core/numba_kernels.py:980-1074). Only the
- * pairs with the trial element later in the list are evaluated; the upper triangle is then copied
- * into the lower one. out (n x n, n = space->n_elements, leading dimension ld) is OVERWRITTEN,
+ * pairs with the trial element later in the list are evaluated; each is stored in the upper
+ * triangle and at its mirror image in the lower one by the same launch. out (n x n,
+ * n = space->n_elements, leading dimension ld) is OVERWRITTEN,
  * with zeros on the diagonal and at the pairs that share a vertex (and, in MIXED precision, at
  * the near pairs): add bem_near_assemble and bem_singular_assemble afterwards.
  */
@@ -410,7 +411,7 @@ int bem_dense_add_transpose(void *A, int dtype, int64_t n, int64_t ld, void *str
 int bem_dense_zero(void *A, int dtype, int64_t n, int64_t ld, void *stream);
 
 /* Copy the upper triangle of a square n x n matrix into its lower triangle and zero the diagonal
- * (the second step of bem_regular_assemble_symmetric; no reference counterpart). */
+ * (no reference counterpart). */
 int bem_dense_mirror_upper(void *A, int dtype, int64_t n, int64_t ld, void *stream);
 
 /* Set `bytes` bytes at `ptr` to zero with a fill kernel (safe inside graph capture). */
