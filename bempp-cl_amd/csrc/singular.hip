@@ -1,13 +1,13 @@
 // Singular (Sauter-Schwab) pairs, FP64, for gfx950.
 //
-// A group of kSingGroup lanes per (test element, trial element) pair that shares a vertex,
-// an edge or the whole triangle (bempp_cl/core/numba_kernels.py:1471-1526 scalar,
-// 1973-2061 EFIE); a wavefront takes 8 consecutive pairs, which are mostly of one kind (the
+// A group of G lanes (8, or 32 for small launches) per (test element, trial element) pair that
+// shares a vertex, an edge or the whole triangle (bempp_cl/core/numba_kernels.py:1471-1526 scalar,
+// 1973-2061 EFIE); a wavefront takes 64 / G consecutive pairs, which are mostly of one kind (the
 // pair table lists coincident, then edge, then vertex pairs). The pair's 512..1536 Duffy
 // points are strided over the group's lanes; each lane accumulates the 3x3 moment block
 // S[a][b] = sum_k G_k w_k u_a(xi_k) u_b(eta_k) with u = (1, xi_0, xi_1), the block is summed
-// over the group with 3 shuffle steps (a reduction over the whole wavefront would cost 6 steps
-// for one pair), scaled by J_test J_trial and turned into the local matrix block exactly as
+// over the group with log2(G) shuffle steps (a reduction over the whole wavefront would cost 6
+// steps for one pair), scaled by J_test J_trial and turned into the local matrix block exactly as
 // for regular pairs. The group's lanes add the multiplied entries to the dense output with
 // hardware atomics, which replaces the reference's host-side np.add.at scatter
 // (bempp_cl/core/dense_assembler.py:78-98).
@@ -27,22 +27,28 @@ struct SingularArgs {
     int64_t ld;
 };
 
-constexpr int kSingGroup = 8;  // lanes per pair
+// lanes per pair: 8, or 32 for launches of fewer than kSingSmallPairs pairs, whose 8-lane groups
+// would give fewer than four waves per SIMD on 256 CUs (1,280 triangles: 16,580 pairs, 2,073 waves;
+// there 32 lanes per pair take 0.033 instead of 0.048 ms, while at 66,500 EFIE pairs they are 25 %
+// slower than 8)
+constexpr int kSingGroup = 8;
+constexpr int kSingGroupSmall = 32;
+constexpr int kSingSmallPairs = 32768;
 
-template <typename T>
+template <int G, typename T>
 __device__ __forceinline__ T group_sum(T v) {
 #pragma unroll
-    for (int off = kSingGroup / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kSingGroup);
+    for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, G);
     return v;
 }
 
 // DUAL: OP is a single layer; the double layer of the same family (sharing r and e^{ikr} at
 // every point) goes to a.out2.
-template <int OP, int NTS, int NSS, typename OutT, bool DUAL = false>
+template <int OP, int NTS, int NSS, typename OutT, bool DUAL = false, int G = kSingGroup>
 __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
     constexpr bool CPLX = OpTraits<OP>::is_complex;
-    const int lane = threadIdx.x & (kSingGroup - 1);  // lane within the pair's group
-    const int pair = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kSingGroup);
+    const int lane = threadIdx.x & (G - 1);  // lane within the pair's group
+    const int pair = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G);
     // the group's lanes share the pair, so whole groups leave together and the group shuffles
     // below only read lanes of groups that stay
     if (pair >= a.n_pairs) return;
@@ -69,7 +75,7 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
     // k r <= 1, else (real wavenumbers) the branch-free reduction
     auto points = [&](auto phase_tag) {
         constexpr int PH = decltype(phase_tag)::value;
-        for (int k = lane; k < np; k += kSingGroup) {
+        for (int k = lane; k < np; k += G) {
             double xi0 = a.tpts[2 * (int64_t)(toff + k)], xi1 = a.tpts[2 * (int64_t)(toff + k) + 1];
             double et0 = a.spts[2 * (int64_t)(soff + k)], et1 = a.spts[2 * (int64_t)(soff + k) + 1];
             double w = a.wts[woff + k];
@@ -109,17 +115,17 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
     for (int i = 0; i < NTS; ++i)
 #pragma unroll
         for (int j = 0; j < NSS; ++j) {
-            S[i][j].re = group_sum(Sr[i][j]) * jj;
-            S[i][j].im = CPLX ? group_sum(Si[i][j]) * jj : 0.0;
+            S[i][j].re = group_sum<G>(Sr[i][j]) * jj;
+            S[i][j].im = CPLX ? group_sum<G>(Si[i][j]) * jj : 0.0;
             if constexpr (DUAL) {
-                S2[i][j].re = group_sum(Sr2[i][j]) * jj;
-                S2[i][j].im = CPLX ? group_sum(Si2[i][j]) * jj : 0.0;
+                S2[i][j].re = group_sum<G>(Sr2[i][j]) * jj;
+                S2[i][j].im = CPLX ? group_sum<G>(Si2[i][j]) * jj : 0.0;
             }
         }
-    // the group's lanes add entries l = lane, lane + kSingGroup, ... (DUAL: l >= NTS*NSS are the
-    // double layer's)
+    // the group's lanes add entries l = lane, lane + G, ... (DUAL: l >= NTS*NSS are the double
+    // layer's)
     const int nb = NTS * NSS;
-    for (int ll = lane; ll < (DUAL ? 2 * nb : nb); ll += kSingGroup) {
+    for (int ll = lane; ll < (DUAL ? 2 * nb : nb); ll += G) {
         const bool second = DUAL && ll >= nb;
         const int l = second ? ll - nb : ll;
         int i = l / NSS, j = l % NSS;
@@ -146,26 +152,36 @@ __global__ __launch_bounds__(256) void singular_kernel(const SingularArgs a) {
     }
 }
 
-template <int OP, typename OutT, bool DUAL>
-static int launch_sing(const SingularArgs &a, hipStream_t s) {
-    dim3 grid((unsigned)(((int64_t)a.n_pairs * kSingGroup + 255) / 256));
-    if (a.n_pairs == 0) return BEM_OK;
+template <int OP, typename OutT, bool DUAL, int G>
+static int launch_sing_g(const SingularArgs &a, hipStream_t s) {
+    dim3 grid((unsigned)(((int64_t)a.n_pairs * G + 255) / 256));
     if constexpr (OP == kMaxwellEFIE) {
-        hipLaunchKernelGGL((singular_kernel<OP, 3, 3, OutT>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((singular_kernel<OP, 3, 3, OutT, false, G>), grid, dim3(256), 0, s, a);
     } else if constexpr (is_hypersingular<OP>()) {
         if (nshape_of(a.test.shape) != 3 || nshape_of(a.trial.shape) != 3) {
             set_error("hypersingular operators need P1 shape functions on both sides");
             return BEM_ERR_INVALID_ARGUMENT;
         }
-        hipLaunchKernelGGL((singular_kernel<OP, 3, 3, OutT>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((singular_kernel<OP, 3, 3, OutT, false, G>), grid, dim3(256), 0, s, a);
     } else {
         int nt = nshape_of(a.test.shape), ns = nshape_of(a.trial.shape);
-        if (nt == 1 && ns == 1) hipLaunchKernelGGL((singular_kernel<OP, 1, 1, OutT, DUAL>), grid, dim3(256), 0, s, a);
-        else if (nt == 1 && ns == 3) hipLaunchKernelGGL((singular_kernel<OP, 1, 3, OutT, DUAL>), grid, dim3(256), 0, s, a);
-        else if (nt == 3 && ns == 1) hipLaunchKernelGGL((singular_kernel<OP, 3, 1, OutT, DUAL>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((singular_kernel<OP, 3, 3, OutT, DUAL>), grid, dim3(256), 0, s, a);
+        if (nt == 1 && ns == 1)
+            hipLaunchKernelGGL((singular_kernel<OP, 1, 1, OutT, DUAL, G>), grid, dim3(256), 0, s, a);
+        else if (nt == 1 && ns == 3)
+            hipLaunchKernelGGL((singular_kernel<OP, 1, 3, OutT, DUAL, G>), grid, dim3(256), 0, s, a);
+        else if (nt == 3 && ns == 1)
+            hipLaunchKernelGGL((singular_kernel<OP, 3, 1, OutT, DUAL, G>), grid, dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((singular_kernel<OP, 3, 3, OutT, DUAL, G>), grid, dim3(256), 0, s, a);
     }
     return check_launch("singular_kernel");
+}
+
+template <int OP, typename OutT, bool DUAL>
+static int launch_sing(const SingularArgs &a, hipStream_t s) {
+    if (a.n_pairs == 0) return BEM_OK;
+    if (a.n_pairs < kSingSmallPairs) return launch_sing_g<OP, OutT, DUAL, kSingGroupSmall>(a, s);
+    return launch_sing_g<OP, OutT, DUAL, kSingGroup>(a, s);
 }
 
 template <int OP, bool DUAL = false>
