@@ -217,7 +217,10 @@ int bem_regular_assemble_sl_dl(int op, const bem_space_t *test, const bem_space_
  * excluded, and in MIXED precision so are the near pairs (bem_near_assemble_sl_dl adds them).
  * out_double may be NULL: the operator alone, at the cost of its own point pairs only (the near
  * pairs then come from bem_near_assemble). op may then also be BEM_LAPLACE_HYPERSINGULAR or
- * BEM_HELMHOLTZ_HYPERSINGULAR, symmetric like the single layers (W_fe = W_ef^T).
+ * BEM_HELMHOLTZ_HYPERSINGULAR, symmetric like the single layers (W_fe = W_ef^T), or
+ * BEM_MAXWELL_ELECTRIC_FIELD on one RWG dof map (space: RWG; the SNC test functions of the same
+ * dofs, multipliers and normal orientation give the same matrix). MIXED precision with the
+ * electric field operator takes trial blocks of up to BEM_TILE_RWG_MIXED_TRIAL_DOFS dofs.
  */
 int bem_regular_assemble_sl_dl_symmetric(int op, const bem_space_t *space, const bem_dof_blocks_t *test_tiles,
                                          const bem_dof_blocks_t *trial_blocks, int n_work, const int32_t *work,
