@@ -269,3 +269,23 @@ def test_symmetric_efie_other_rules(order, default_parameters, hip_available):
     print(f"EFIE regular order {order}: FP64 symmetric vs ordered {rel(s, o):.2e}, mixed {rel(m, o):.2e}", flush=True)
     assert rel(s, o) < 1e-13
     assert rel(m, o) < 1e-6
+
+
+def test_symmetric_efie_partial_support(default_parameters, hip_available):
+    """RWG and SNC restricted to the same part of the sphere (edges on the part's boundary carry
+    no dof): the symmetric pass against the ordered one, and mixed against FP64."""
+    from bempp_amd.api import function_space
+    from bempp_amd.core.dense_assembler import use_symmetric_efie
+
+    grid = perturbed_sphere(2)
+    part = np.nonzero(grid.centroids[:, 2] > -0.3)[0]
+    rwg = function_space(grid, "RWG", 0, support_elements=part)
+    snc = function_space(grid, "SNC", 0, support_elements=part)
+    assert use_symmetric_efie(rwg, snc)
+    o = efie(rwg, snc, 2.5, "double", False, default_parameters)
+    s = efie(rwg, snc, 2.5, "double", True, default_parameters)
+    m = efie(rwg, snc, 2.5, "single", True, default_parameters)
+    print(f"EFIE on {len(part)} of {grid.number_of_elements} triangles ({rwg.global_dof_count} dofs): "
+          f"symmetric vs ordered {rel(s, o):.2e}, mixed {rel(m, o):.2e}", flush=True)
+    assert rel(s, o) < 1e-13
+    assert rel(m, o) < 1e-6
