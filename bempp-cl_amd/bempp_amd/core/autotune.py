@@ -86,26 +86,31 @@ def lookup(key):
     return None
 
 
-def tune(key, run, candidates=None, repeats=2):
+def tune(key, run, candidates=None, repeats=2, passes=2):
     """Time run(config) for every candidate config on the current stream; cache and return the best.
 
     run: callable enqueuing one complete launch with the given (test_tile, workgroups_per_cu).
+    Each candidate keeps the fastest of its launches over `passes` sweeps through the candidate
+    list, so a clock ramp or a stray slow launch early in the first sweep does not decide.
     """
     import torch
 
     if candidates is None:
         candidates = [(t, w) for t in CANDIDATE_TILES for w in CANDIDATE_WG_PER_CU]
-    timings = {}
     for cfg in candidates:
         run(cfg)  # warm-up (code object load, caches)
-        start = torch.cuda.Event(enable_timing=True)
-        end = torch.cuda.Event(enable_timing=True)
-        start.record()
-        for _ in range(repeats):
-            run(cfg)
-        end.record()
-        end.synchronize()
-        timings[cfg] = start.elapsed_time(end) / repeats
+    timings = {}
+    for _ in range(passes):
+        for cfg in candidates:
+            for _ in range(repeats):
+                start = torch.cuda.Event(enable_timing=True)
+                end = torch.cuda.Event(enable_timing=True)
+                start.record()
+                run(cfg)
+                end.record()
+                end.synchronize()
+                t = start.elapsed_time(end)
+                timings[cfg] = min(t, timings.get(cfg, t))
     best = min(timings, key=timings.get)
     _MEMORY[key] = best
     _store(key, {"config": list(best), "ms": timings[best],

@@ -440,6 +440,11 @@ def frequency_sweep_bench(level, reps, with_cpu=False, cpu_seconds=4.0):
     n_geom = (F * F - n_sing) * 36
     out = hip_kbatch_assemble(sp, sp, ks, "single", parameters=params)  # tunes on first use
     torch.cuda.synchronize()
+    from bempp_amd.core import autotune as at
+
+    # the tuner's record for this problem: chosen launch shape and every candidate's best time
+    tuned = [v for k, v in at._load().items() if k.split("|")[3:6] == [str(F), str(F), str(len(ks))]
+             and "single" in k]
     ev = [{key: torch.cuda.Event(enable_timing=True) for key in ("regular_start", "regular_end", "end")}
           for _ in range(reps)]
     t0 = time.perf_counter()
@@ -480,6 +485,7 @@ def frequency_sweep_bench(level, reps, with_cpu=False, cpu_seconds=4.0):
         "regular_algorithmic_tflops": flops / (reg_ms * 1e-3) / 1e12,
         "flop_per_eval": "10 + 9 per wavenumber",
         "regular_write_gbs": entries * 8 / (reg_ms * 1e-3) / 1e9,
+        "autotune": tuned,
         "note": "the mixed kernel expands e^{ik(r-r0)} in moments of r - r0, so the algorithmic "
                 "FLOP rate counts the work of direct evaluation; the kernel issues VALU instructions about 3/4 "
                 "of its time (SQ_ACTIVE_INST_VALU) and writes the complex64 output at under half the HBM rate",
