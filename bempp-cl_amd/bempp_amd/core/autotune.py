@@ -90,8 +90,9 @@ def tune(key, run, candidates=None, repeats=2, passes=2):
     """Time run(config) for every candidate config on the current stream; cache and return the best.
 
     run: callable enqueuing one complete launch with the given (test_tile, workgroups_per_cu).
-    Each candidate keeps the fastest of its launches over `passes` sweeps through the candidate
-    list, so a clock ramp or a stray slow launch early in the first sweep does not decide.
+    Each candidate is timed over `passes` sweeps through the candidate list (so a clock ramp
+    early in the first sweep does not decide) and ranked by the median of its launches (so one
+    fast or slow launch does not either).
     """
     import torch
 
@@ -99,7 +100,7 @@ def tune(key, run, candidates=None, repeats=2, passes=2):
         candidates = [(t, w) for t in CANDIDATE_TILES for w in CANDIDATE_WG_PER_CU]
     for cfg in candidates:
         run(cfg)  # warm-up (code object load, caches)
-    timings = {}
+    samples = {cfg: [] for cfg in candidates}
     for _ in range(passes):
         for cfg in candidates:
             for _ in range(repeats):
@@ -109,8 +110,8 @@ def tune(key, run, candidates=None, repeats=2, passes=2):
                 run(cfg)
                 end.record()
                 end.synchronize()
-                t = start.elapsed_time(end)
-                timings[cfg] = min(t, timings.get(cfg, t))
+                samples[cfg].append(start.elapsed_time(end))
+    timings = {cfg: sorted(v)[len(v) // 2] for cfg, v in samples.items()}
     best = min(timings, key=timings.get)
     _MEMORY[key] = best
     _store(key, {"config": list(best), "ms": timings[best],
