@@ -452,12 +452,26 @@ __global__ __launch_bounds__(256) void adjoint_near_kernel(const Args a, int n_p
 }
 
 // ------------------------------------------------------------------ singular pairs, FP64, both elements
+// kSingLanes lanes per pair (8 pairs per wavefront): the 512..1536 Duffy points are strided over
+// them and the 19 sums reduced in 3 shuffle steps (a wavefront per pair left most lanes of the
+// vertex pairs with 8 points and spent 6 steps per sum)
+constexpr int kSingLanes = 8;
+
+template <typename T>
+__device__ __forceinline__ T sing_group_sum(T v) {
+#pragma unroll
+    for (int off = kSingLanes / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, kSingLanes);
+    return v;
+}
+
 __global__ __launch_bounds__(256) void adjoint_singular_kernel(const Args a, int n_pairs, const int32_t *pd_all,
                                                                const double *tpts, const double *spts,
                                                                const double *wts) {
-    const int lane = threadIdx.x & 63;
-    const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (pair >= n_pairs) return;  // uniform per wavefront
+    const int lane = threadIdx.x & (kSingLanes - 1);
+    const int pair = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kSingLanes);
+    // the group's lanes share the pair, so whole groups leave together and the group shuffles
+    // below only read lanes of groups that stay
+    if (pair >= n_pairs) return;
     const int32_t *pd = pd_all + 6 * (int64_t)pair;
     const int e = pd[0], f = pd[1], np = pd[5];
     const double *tp = tpts + 2 * (int64_t)pd[2], *sp = spts + 2 * (int64_t)pd[3], *w = wts + pd[4];
@@ -470,39 +484,49 @@ __global__ __launch_bounds__(256) void adjoint_singular_kernel(const Args a, int
     Wi *= m;
     // Ay[j]: trial vertex j, Ax[j]: test vertex j (both without the J_e J_f factor); S: sum w Re(W G)
     double Ay[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, Ax[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, S = 0.0;
-    for (int kk = lane; kk < np; kk += 64) {
-        const double xi0 = tp[2 * kk], xi1 = tp[2 * kk + 1], et0 = sp[2 * kk], et1 = sp[2 * kk + 1];
-        double dv[3];
+    auto points = [&](auto phase_tag) {
+        constexpr int PH = decltype(phase_tag)::value;
+        for (int kk = lane; kk < np; kk += kSingLanes) {
+            const double xi0 = tp[2 * kk], xi1 = tp[2 * kk + 1], et0 = sp[2 * kk], et1 = sp[2 * kk + 1];
+            double dv[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-            dv[d] = (gf[d] + gf[3 + d] * et0 + gf[6 + d] * et1) - (gt[d] + gt[3 + d] * xi0 + gt[6 + d] * xi1);
-        const double r = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2]);
-        double s, c;
-        sincos_fast(a.k * r, &s, &c);
-        const double Zr = Wr * c - Wi * s, Zi = Wr * s + Wi * c;
-        const double amp = w[kk] * kInv4Pi / r;
-        S += amp * Zr;
-        const double t = amp * (-Zr - a.k * r * Zi) / (r * r);
-        const double py[3] = {1.0 - et0 - et1, et0, et1}, px[3] = {1.0 - xi0 - xi1, xi0, xi1};
+            for (int d = 0; d < 3; ++d)
+                dv[d] = (gf[d] + gf[3 + d] * et0 + gf[6 + d] * et1) - (gt[d] + gt[3 + d] * xi0 + gt[6 + d] * xi1);
+            const double r2 = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+            const double rinv = rsqrt_t(r2), r = r2 * rinv;
+            double s, c;
+            sincos_t<PH>(a.k * r, &s, &c);
+            const double Zr = Wr * c - Wi * s, Zi = Wr * s + Wi * c;
+            const double amp = w[kk] * kInv4Pi * rinv;
+            S += amp * Zr;
+            const double t = amp * (-Zr - a.k * r * Zi) * (rinv * rinv);
+            const double py[3] = {1.0 - et0 - et1, et0, et1}, px[3] = {1.0 - xi0 - xi1, xi0, xi1};
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
+            for (int j = 0; j < 3; ++j)
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                Ay[j][d] += py[j] * t * dv[d];
-                Ax[j][d] -= px[j] * t * dv[d];
-            }
-    }
+                for (int d = 0; d < 3; ++d) {
+                    Ay[j][d] += py[j] * t * dv[d];
+                    Ax[j][d] -= px[j] * t * dv[d];
+                }
+        }
+    };
+    // every distance within the pair is at most diam_e + diam_f (triangles sharing a vertex): the
+    // series sin/cos without range reduction when |k| r <= 1, else the branch-free reduction
+    const double kd = fabs(a.k) * (gt[13] + gf[13]);
+    if (kd <= 1.0) points(std::integral_constant<int, kPhSmall | kPhReal>{});
+    else if (kd < 0x1p25) points(std::integral_constant<int, kPhReduced | kPhReal>{});
+    else points(std::integral_constant<int, kPhReal>{});
 #pragma unroll
     for (int j = 0; j < 3; ++j)
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            Ay[j][d] = wave_sum64(Ay[j][d]);
-            Ax[j][d] = wave_sum64(Ax[j][d]);
+            Ay[j][d] = sing_group_sum(Ay[j][d]);
+            Ax[j][d] = sing_group_sum(Ax[j][d]);
         }
-    S = wave_sum64(S);
-    if (lane < 18) {
-        const bool trial_side = lane < 9;
-        const int j = (lane % 9) / 3, d = lane % 3;
+    S = sing_group_sum(S);
+    for (int o = lane; o < 18; o += kSingLanes) {
+        const bool trial_side = o < 9;
+        const int j = (o % 9) / 3, d = o % 3;
         const double *g = trial_side ? gf : gt;
         const double other_J = trial_side ? gt[12] : gf[12];
         double dJ[3][3];
@@ -682,8 +706,9 @@ extern "C" int bem_adjoint_single_layer(const bem_space_t *space, const bem_quad
         if ((rc = check_launch("adjoint_near_kernel"))) return rc;
     }
     if (n_singular > 0) {
-        hipLaunchKernelGGL(adj::adjoint_singular_kernel, dim3((n_singular + 3) / 4), dim3(256), 0, s, a, n_singular,
-                           singular_pairs, test_points, trial_points, weights);
+        hipLaunchKernelGGL(adj::adjoint_singular_kernel,
+                           dim3((unsigned)(((int64_t)n_singular * adj::kSingLanes + 255) / 256)), dim3(256), 0, s, a,
+                           n_singular, singular_pairs, test_points, trial_points, weights);
         if ((rc = check_launch("adjoint_singular_kernel"))) return rc;
     }
     return BEM_OK;
