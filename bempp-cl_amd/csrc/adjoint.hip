@@ -563,10 +563,13 @@ struct PotArgs {
 
 constexpr int kPotTile = 256;
 
-__global__ __launch_bounds__(kThreads) void potential_adjoint_kernel(const PotArgs a) {
+__global__ __launch_bounds__(kThreads, 2) void potential_adjoint_kernel(const PotArgs a) {
     __shared__ double P_[kPotTile][3];
     __shared__ double G_[kPotTile][2];
+    // running max of |x| over the staged points (a bound for the current tile)
+    __shared__ unsigned int pmax_bits;
     const int tid = threadIdx.x;
+    if (tid == 0) pmax_bits = 0u;  // published by the first tile's barrier
     const int je = blockIdx.x * kThreads + tid;
     const bool valid = je < a.sp.n;
     const int p0 = blockIdx.y * a.chunk, p1 = min(p0 + a.chunk, a.n_points);
@@ -581,10 +584,12 @@ __global__ __launch_bounds__(kThreads) void potential_adjoint_kernel(const PotAr
         cr = m * a.c[2 * dof];
         ci = m * a.c[2 * dof + 1];
     }
-    double yq[kQ][3];
+    double yq[kQ][3], ynorm = 0.0;  // ynorm: max |y| over the element's quadrature points
 #pragma unroll
-    for (int q = 0; q < kQ; ++q)
+    for (int q = 0; q < kQ; ++q) {
         for (int d = 0; d < 3; ++d) yq[q][d] = valid ? ge[d] + ge[3 + d] * a.rule.x[q] + ge[6 + d] * a.rule.y[q] : 0.0;
+        ynorm = fmax(ynorm, sqrt(yq[q][0] * yq[q][0] + yq[q][1] * yq[q][1] + yq[q][2] * yq[q][2]));
+    }
     double A[kQ][3], S = 0.0, hr = 0.0, hi = 0.0;  // hr + i hi = sum_p conj(P_pe / m_e) g_p
 #pragma unroll
     for (int q = 0; q < kQ; ++q) A[q][0] = A[q][1] = A[q][2] = 0.0;
@@ -592,34 +597,50 @@ __global__ __launch_bounds__(kThreads) void potential_adjoint_kernel(const PotAr
         const int nt = min(kPotTile, p1 - t0);
         __syncthreads();
         if (tid < nt) {
-            for (int d = 0; d < 3; ++d) P_[tid][d] = a.pts[3 * (int64_t)(t0 + tid) + d];
+            double x2 = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                const double x = a.pts[3 * (int64_t)(t0 + tid) + d];
+                P_[tid][d] = x;
+                x2 += x * x;
+            }
             G_[tid][0] = a.g[2 * (int64_t)(t0 + tid)];
             G_[tid][1] = a.g[2 * (int64_t)(t0 + tid) + 1];
+            // non-negative floats order like their bit patterns
+            atomicMax(&pmax_bits, __float_as_uint((float)(sqrt(x2) * (1.0 + 1e-6))));
         }
         __syncthreads();
         if (!valid) continue;
-        for (int t = 0; t < nt; ++t) {
-            const double gr = G_[t][0], gi = G_[t][1];
-            // W = conj(g_p) m_e c_e
-            const double Wr = gr * cr + gi * ci, Wi = gr * ci - gi * cr;
+        auto points = [&](auto phase_tag) {
+            constexpr int PH = decltype(phase_tag)::value;
+            for (int t = 0; t < nt; ++t) {
+                const double gr = G_[t][0], gi = G_[t][1];
+                // W = conj(g_p) m_e c_e
+                const double Wr = gr * cr + gi * ci, Wi = gr * ci - gi * cr;
 #pragma unroll
-            for (int q = 0; q < kQ; ++q) {
-                const double dx = yq[q][0] - P_[t][0], dy = yq[q][1] - P_[t][1], dz = yq[q][2] - P_[t][2];
-                const double r = sqrt(dx * dx + dy * dy + dz * dz);
-                double s, c;
-                sincos_fast(a.k * r, &s, &c);
-                const double amp = a.rule.w[q] * kInv4Pi / r;  // without J_e
-                // grad_c: conj(G) g
-                hr += amp * (c * gr + s * gi);
-                hi += amp * (c * gi - s * gr);
-                const double Zr = Wr * c - Wi * s, Zi = Wr * s + Wi * c;
-                S += amp * Zr;
-                const double tq = amp * (-Zr - a.k * r * Zi) / (r * r);
-                A[q][0] += tq * dx;
-                A[q][1] += tq * dy;
-                A[q][2] += tq * dz;
+                for (int q = 0; q < kQ; ++q) {
+                    const double dx = yq[q][0] - P_[t][0], dy = yq[q][1] - P_[t][1], dz = yq[q][2] - P_[t][2];
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    const double rinv = rsqrt_t(r2), r = r2 * rinv;
+                    double s, c;
+                    sincos_t<PH>(a.k * r, &s, &c);
+                    const double amp = a.rule.w[q] * kInv4Pi * rinv;  // without J_e
+                    // grad_c: conj(G) g
+                    hr += amp * (c * gr + s * gi);
+                    hi += amp * (c * gi - s * gr);
+                    const double Zr = Wr * c - Wi * s, Zi = Wr * s + Wi * c;
+                    S += amp * Zr;
+                    const double tq = amp * (-Zr - a.k * r * Zi) * (rinv * rinv);
+                    A[q][0] += tq * dx;
+                    A[q][1] += tq * dy;
+                    A[q][2] += tq * dz;
+                }
             }
-        }
+        };
+        // |k| r <= |k| (|x| + |y|) < 2^25 on every pair of the wave: the branch-free phase
+        // reduction, which lets the compiler interleave the quadrature points
+        const double pmax = (double)__uint_as_float(pmax_bits);
+        if (__all(fabs(a.k) * (pmax + ynorm) < 0x1p25)) points(std::integral_constant<int, kPhReduced | kPhReal>{});
+        else points(std::integral_constant<int, kPhReal>{});
     }
     if (!valid) return;
     if (a.grad_c) {
