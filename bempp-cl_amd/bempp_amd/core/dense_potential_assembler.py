@@ -64,7 +64,10 @@ def hip_potential_assembler(space, operator_descriptor, points, parameters):
         if torch.is_tensor(x):
             coeffs = x
         elif gather is not None:
-            xd = torch.from_numpy(_np.ascontiguousarray(_np.asarray(x), dtype=_np.complex128)).to(dev)
+            xh = _np.ascontiguousarray(_np.asarray(x), dtype=_np.complex128)
+            if xh.ndim != 1 or xh.shape[0] != full_map.shape[1]:
+                raise ValueError(f"expected {full_map.shape[1]} coefficients, got an array of shape {xh.shape}")
+            xd = torch.from_numpy(xh).to(dev)
             coeffs = gather[1] * xd[gather[0]] if xd.numel() else torch.zeros_like(gather[1])
         else:
             xf = full_map @ _np.asarray(x)

@@ -1,4 +1,5 @@
-"""Edge cases of the C ABI on the device: a space without elements and an empty point set."""
+"""Edge cases of the C ABI and the host wrappers on the device: a space without elements, an empty
+point set, a coefficient vector of the wrong length."""
 
 import ctypes
 
@@ -47,3 +48,19 @@ def test_potential_of_empty_space_is_zero(op, kind, degree, kdim, precision_name
                                       ctypes.byref(rule), 2.0, 0.0, precision, None,
                                       hb.DTYPES[str(out_dtype).replace("torch.", "")], hb.ptr(ws), need,
                                       hb.stream_handle()))
+
+
+def test_potential_rejects_wrong_coefficient_count(hip_available):
+    """A coefficient vector of the wrong length is refused on the host, before the device gather
+    that maps it to the elements."""
+    import bempp_amd.api as bempp
+    from bempp_amd.api.operators.potential import helmholtz as hpot
+
+    space = bempp.function_space(bempp.shapes.icosphere(1), "P", 1)
+    pts = np.array([[2.0, 0.0, 0.0], [0.0, 2.0, 0.0]]).T
+    pot = hpot.single_layer(space, pts, 1.5)
+    impl = pot._assembler._implementation._implementation
+    with pytest.raises(ValueError):
+        impl(np.ones(space.global_dof_count + 1))
+    u = impl(np.ones(space.global_dof_count))
+    assert u.shape == (1, 2) and np.all(np.isfinite(u))
