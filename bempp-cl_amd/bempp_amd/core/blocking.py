@@ -30,6 +30,8 @@ _CACHE = {}
 
 def _morton_keys(points):
     """63-bit Morton codes of points (n, 3) in their bounding box."""
+    if len(points) == 0:
+        return _np.zeros(0, dtype=_np.uint64)
     lo = points.min(axis=0)
     span = points.max(axis=0) - lo
     span[span == 0] = 1.0

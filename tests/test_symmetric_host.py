@@ -100,3 +100,20 @@ def test_upper_triangle_predicate():
     assert not use_upper_triangle(dp0, dp0, "helmholtz_double_layer")
     assert not use_upper_triangle(p1, p1, "laplace_single_layer")
     assert not use_upper_triangle(dp0, function_space(g, "DP", 0, support_elements=np.arange(10)), "laplace_single_layer")
+
+
+def test_blockings_of_a_space_without_support_elements():
+    """A space restricted to no elements (here RWG on one triangle: no interior edge) gives one
+    empty block, one empty tile and one work item over it, which the kernels skip."""
+    import bempp_amd.api as bempp
+    from bempp_amd.core.blocking import RWG_MIXED_TRIAL_BLOCK_DOFS, make_blocks, make_symmetric_blocks
+
+    v = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]]).T
+    rwg = bempp.function_space(bempp.Grid(v, np.array([[0, 1, 2]]).T), "RWG", 0)
+    assert len(rwg.support_elements) == 0
+    b = make_blocks(rwg, 256, RWG_MIXED_TRIAL_BLOCK_DOFS)
+    assert b.n_blocks == 1 and b.ndofs.sum() == 0
+    trial, tiles, owner, work = make_symmetric_blocks(rwg, trial_dofs=RWG_MIXED_TRIAL_BLOCK_DOFS,
+                                                      min_tiles_per_item=2)
+    assert len(trial.order) == 0 and tiles.ndofs.sum() == 0
+    assert (tiles.start[work[:, 2]] - tiles.start[work[:, 1]]).max() == 0
