@@ -494,8 +494,9 @@ def regular_assemble_sl_dl_symmetric(op, sym, rule, wavenumber, precision, out_s
 
 def regular_assemble_symmetric(op, space, rule, wavenumber, precision, out, near_factor=DEFAULT_NEAR_FIELD_FACTOR,
                                kahan=False):
-    """Regular part of a symmetric single layer on one DP0 space (dof = list position): the upper
-    triangle is evaluated, the lower one copied. `out` (n x n, contiguous) is overwritten."""
+    """Regular part of a symmetric single layer on one DP0 space (dof = list position): the pairs of
+    the upper triangle are evaluated and each is stored with its mirror image. `out` (n x n,
+    contiguous) is overwritten."""
     k = complex(wavenumber)
     n = out.shape[0]
     if out.dim() != 2 or out.shape[1] != n or not out.is_contiguous():

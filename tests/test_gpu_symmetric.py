@@ -289,3 +289,40 @@ def test_symmetric_efie_partial_support(default_parameters, hip_available):
           f"symmetric vs ordered {rel(s, o):.2e}, mixed {rel(m, o):.2e}", flush=True)
     assert rel(s, o) < 1e-13
     assert rel(m, o) < 1e-6
+
+
+@pytest.mark.parametrize("level", [3, 4])
+@pytest.mark.parametrize("precision_name", ["mixed", "fp64"])
+def test_dp0_upper_triangle_strided_output(level, precision_name, hip_available):
+    """bem_regular_assemble_symmetric into rows longer than the matrix (ld = n + 5): the n x n block
+    equals the contiguous result, mirror images included, and the 5 padding columns of every row
+    keep their contents. Level 3 takes the 4-element test tiles, level 4 the 16-element ones;
+    complex64 output stages the mirror images in LDS, complex128 stores them directly."""
+    import ctypes
+
+    import torch
+
+    from bempp_amd.api import function_space
+    from bempp_amd.api.grid.shapes import icosphere
+    from bempp_amd.api.integration.triangle_gauss import rule as triangle_rule
+    from bempp_amd.core import hip_backend as hb
+
+    sp = function_space(icosphere(level), "DP", 0)
+    dsp = hb.device_space(sp)
+    pts, w = triangle_rule(4)
+    rule = hb.quad_rule(pts, w)
+    precision = hb.PRECISION_MIXED if precision_name == "mixed" else hb.PRECISION_FP64
+    dtype = torch.complex64 if precision_name == "mixed" else torch.complex128
+    n = sp.global_dof_count
+    ref = torch.empty((n, n), dtype=dtype, device=hb.device())
+    hb.regular_assemble_symmetric("helmholtz_single_layer", dsp, rule, 2.5, precision, ref)
+    pad = 5
+    big = torch.full((n, n + pad), 7.0 + 3.0j, dtype=dtype, device=hb.device())
+    hb.check(hb.lib().bem_regular_assemble_symmetric(
+        hb.OPERATORS["helmholtz_single_layer"], ctypes.byref(dsp.struct), ctypes.byref(rule), 2.5, 0.0, precision,
+        float(hb.DEFAULT_NEAR_FIELD_FACTOR), 0, hb.ptr(big), n + pad, hb._dtype_code(big), hb.stream_handle()))
+    torch.cuda.synchronize()
+    assert bool((big[:, :n] == ref).all())
+    assert bool((big[:, n:] == 7.0 + 3.0j).all())
+    assert bool((ref == ref.T).all())
+    assert bool((torch.diagonal(ref) == 0).all())
