@@ -12,9 +12,11 @@ the pass over ordered pairs, the two operators assembled separately, the FP64 as
 (separate = the plain path, ordered fused, symmetric) with the relative Frobenius error of
 the mixed result, a Kahan-summation variant, the
 potential of a seeded complex Gaussian P1 density at 16,384 points on the sphere of radius 2,
-the frequency sweep (32 wavenumbers, DP0, level 4), the loss/gradient workload (level 6),
-kernel-level roofline figures (HIP events on the launch stream) and the CPU baseline (FP64
-oracle on the host cores, row sample).
+the Laplace DP0 FP64 assembly at level 3, the frequency sweep (32 wavenumbers, DP0, level 4),
+the Maxwell electric field operator (RWG x SNC, level 4, symmetric and ordered traversals) and
+its potential at 65,536 points, the loss/gradient workload (level 6), kernel-level roofline
+figures (HIP events on the launch stream) and the CPU baseline (FP64 oracle on the host cores,
+row sample).
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-sweep] [--no-adjoint]
 """
